@@ -1,0 +1,121 @@
+"""Throughput of the fused D3Q19 MRT-LES stream-collide step on one MI355X.
+
+Workload (the metric's configuration): channel 512x128x128, D3Q19 MRT with
+d'Humieres rates, s_nu from tau=0.52, Smagorinsky Cs=0.1, FP32 populations,
+square cylinder of side 16 at x=128 centred in y/z, halfway bounce-back walls
+at the y/z bounds, West_Extrap inlet u=0.05 with a seeded +-1e-3 perturbation,
+East_Extrap outlet.  A step is one stream-collide pass over all 8.39M nodes.
+
+Prints one JSON line with GLUPS, the roofline fraction of the stream-collide
+kernel (152 algorithmic bytes per node update in FP32, 76 in 16-bit, 304 in
+FP64, over its launch time from HIP events on the library's stream) and the
+CPU baseline (the oracle, OpenMP, on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GLUPS and % HBM roofline (D3Q19 MRT-LES); learned vs Smagorinsky error/time"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_NODE = {"f32": 152, "f16": 76, "bf16": 76, "f64": 304}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stream_collide_channel512_traffic.json")
+
+
+def cpu_baseline(cfg, budget_s=12.0):
+    """The oracle on the same workload for a bounded number of steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+
+    try:
+        import oracle_lib as O
+        O.lib()
+    except OSError:
+        return None
+    import lbm_multigpu_amd as L
+    from test_gpu_parity import NODE_TO_ORC, orc_params
+
+    c = cfg.to_c()
+    with L.Solver(c) as s:
+        s.set_geometry(None)
+        m = s.get_mask()
+        s.init_equilibrium(L.INIT_BGK)
+        f = s.populations()
+    lut = np.zeros(256, np.uint8)
+    for k, v in NODE_TO_ORC.items():
+        lut[k] = v
+    mask = lut[m]
+    u_in = O.inlet_profile(c.ny, c.nz, c.umax, c.inlet_amp, c.seed)
+    p = orc_params(c)
+    fb = np.empty_like(f)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        out, _ = O.step(p, f, mask=mask, u_in=u_in)
+        f, fb = out, f
+        steps += 1
+        if time.perf_counter() - t0 > budget_s or steps >= 50:
+            break
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cells = c.nx * c.ny * c.nz
+    return {"value": cells * steps / dt / 1e9, "unit": "GLUPS", "cores": threads, "kind": "port",
+            "sample": f"oracle (C, OpenMP {threads} threads) on the full {c.nx}x{c.ny}x{c.nz} workload, "
+                      f"{steps} steps in {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16", "f64"])
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the sharded path is a design, not a build.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import lbm_multigpu_amd as L
+    from lbm_multigpu_amd import config as C
+
+    cfg = C.channel(storage=a.storage)
+    with L.Solver(cfg.to_c()) as s:
+        s.set_geometry(None)
+        s.init_equilibrium(L.INIT_BGK)
+        s.step(a.warmup)
+        s.step_timed(1)  # drains the warm-up
+        t0 = time.perf_counter()
+        ms = s.step_timed(a.steps)
+        wall = time.perf_counter() - t0
+        mm = s.mass_momentum()
+    cells = cfg.nx * cfg.ny * cfg.nz
+    ms_step = ms / a.steps
+    glups = cells / (ms_step * 1e-3) / 1e9
+    ach = BYTES_PER_NODE[a.storage] * cells / (ms_step * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE) and a.storage == "f32":
+        with open(TRAFFIC_FILE) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+    roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic}
+    cpu = None if a.no_cpu else cpu_baseline(cfg)
+    line = {
+        "metric": METRIC, "value": round(glups, 3), "unit": "GLUPS", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": a.storage, "data": "synthetic",
+        "config": {"workload": f"channel {cfg.nx}x{cfg.ny}x{cfg.nz} D3Q19 MRT (d'Humieres rates, tau=0.52) "
+                               "+ Smagorinsky Cs=0.1, square cylinder side 16, halfway walls, seeded inlet",
+                   "cells": cells, "storage": a.storage, "wall_s": round(wall, 3),
+                   "mass_sum": mm[0]},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,192 @@
+/*
+ * lbm.h -- C ABI of liblbm, the MI355X (gfx950) D3Q19 stream-collide and
+ * local-refinement library.
+ *
+ * The reference has no library boundary: its hot path is a set of
+ * __global__ kernels launched by one main() per program, configured by
+ * compile-time #defines and an ImageFcn() cell classifier.  Each entry point
+ * below names the reference interface it replaces (file:line, relative to
+ * the reference tree).
+ *
+ * Conventions
+ *  - Every int-returning call returns 0 on success and a negative LBM_E*
+ *    code on failure; lbm_last_error() then holds a message.  No C++
+ *    exception crosses this boundary.
+ *  - The library owns all device memory; the caller owns host buffers.
+ *    Sizes come from lbm_query_sizes().
+ *  - A context is single-threaded: the caller serialises calls on it.  Each
+ *    context owns one HIP stream; lbm_step() is asynchronous and every
+ *    lbm_get_*() call synchronises that stream.
+ *  - Host arrays are dense, x fastest: scalar fields are [z][y][x],
+ *    populations [q][z][y][x] with the reference's D3Q19 ordering
+ *    (MRTtest/DynInlet3.cu:603-605, :1625-1671).
+ */
+#ifndef LBM_H
+#define LBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define LBM_OK 0
+#define LBM_EINVAL (-22)    /* bad argument or configuration */
+#define LBM_ENOMEM (-12)    /* device or host allocation failed */
+#define LBM_EDEVICE (-5)    /* HIP runtime error, or no usable gfx950 device */
+#define LBM_ESTATE (-71)    /* call not valid in the current state */
+#define LBM_EIO (-2)        /* file could not be read or written */
+
+/* collision model: MODEL / SmagLES #defines (MRTtest/DynInlet3.cu:52-56) */
+#define LBM_MODEL_BGK 0           /* CPU_3D/CPU_3D.cpp:116-160; DynInlet3.cu:802-829 */
+#define LBM_MODEL_MRT_CPU3D 1     /* CPU_3D/CPU_3D.cpp:164-279 (m13, m14 at rate 1) */
+#define LBM_MODEL_MRT_LES 2       /* MRTtest/DynInlet3.cu:599-832 */
+#define LBM_MODEL_MRT_DHUMIERES 3 /* d'Humieres rates, MRTtest/DynInlet3.cu:774-783 */
+
+/* what a pull that leaves the domain along an axis does */
+#define LBM_BC_PERIODIC 0 /* wrap (f_mem y-wrap, MRTtest/DynInlet3.cu:177-185) */
+#define LBM_BC_CLAMP 1    /* clamp to edge (dmin/dmax, CPU_3D/CPU_3D.cpp:104-113) */
+#define LBM_BC_WALL 2     /* halfway (link) bounce-back wall at the face */
+
+/* population storage in HBM */
+#define LBM_STORE_F32 0  /* reference layout precision */
+#define LBM_STORE_F16 1  /* f_i - w_i in IEEE half, FP32 collide */
+#define LBM_STORE_BF16 2 /* f_i - w_i in bfloat16, FP32 collide */
+#define LBM_STORE_F64 3  /* FP64 storage and arithmetic (the FP64 yardstick) */
+
+/* geometry generators standing in for ImageFcn() */
+#define LBM_GEOM_NONE 0    /* all fluid */
+#define LBM_GEOM_CHANNEL 1 /* west velocity inlet (im 400), east rho=1 outlet (im 300),
+                              square obstacle |x-X0|<R && |y-Y0|<R over all z
+                              (LaminarChannelCylinder1/LRMultiGPU6.cu:90-129) */
+#define LBM_GEOM_CPU3D 2   /* CPU_3D/CPU_3D.cpp:39-75: circular cylinder (im 10),
+                              regularised moving-lid x=0 plane (im 53), walls (im 1) */
+
+/* node codes of lbm_set_geometry()/lbm_get_mask(), with the reference im code */
+#define LBM_NODE_FLUID 0        /* im 0 */
+#define LBM_NODE_SOLID 1        /* im 1: bounce-back wall */
+#define LBM_NODE_SOLID_FORCE 2  /* im 10: bounce-back wall, momentum exchange summed */
+#define LBM_NODE_INLET_WEST 3   /* im 400: West_Extrap (MRTtest/DynInlet3.cu:973-1017) */
+#define LBM_NODE_OUTLET_EAST 4  /* im 300: East_Extrap (MRTtest/DynInlet3.cu:926-971) */
+#define LBM_NODE_LID_WEST_REG 5 /* im 53: DirichletWest_Regularized (CPU_3D/CPU_3D.cpp:974) */
+
+/* initial state of lbm_init_equilibrium() */
+#define LBM_INIT_BGK 0 /* explicit BGK feq (CPU_3D/CPU_3D.cpp:1218-1254) */
+#define LBM_INIT_MRT 1 /* mrt_meq + InvertMoments (MRTtest/DynInlet3.cu:1186-1216) */
+
+/*
+ * Configuration.  Field names follow the reference's #defines and .inputs
+ * echo keys (MRTtest/DynInlet3.cu:10-60, WriteInputs :2805-2833).
+ */
+typedef struct lbm_config {
+    int nx, ny, nz;        /* "Base domain size" XDIM x YDIM x ZDIM */
+    int bc[3];             /* LBM_BC_* per axis x, y, z */
+    int model;             /* LBM_MODEL_* ("MODEL") */
+    int halfway;           /* 0: node (full-way) bounce-back as the reference;
+                              1: link-wise halfway bounce-back */
+    int storage;           /* LBM_STORE_* */
+    float omega;           /* "omega": 1/tau0 */
+    float cs;              /* "CS" (reference meaning: Cs^2); 0 disables LES */
+    float dpdy;            /* "DPDY": AddForce body force (MRTtest/DynInlet3.cu:229-253) */
+    float umax;            /* "UMAX": inlet / lid velocity */
+    float inlet_amp;       /* amplitude of the seeded inlet perturbation (0 = uniform) */
+    uint64_t seed;         /* seed of that perturbation */
+    float s_dhum[19];      /* d'Humieres rates for LBM_MODEL_MRT_DHUMIERES (indices
+                              9,11,13,14,15 are replaced by the LES rate) */
+    int geometry;          /* LBM_GEOM_* used when lbm_set_geometry(mask=NULL) */
+    float obst_x, obst_y, obst_z, obst_r; /* "Obst1 location" / "Obst1 radius" */
+    int startf;            /* "STARTF": first step whose force is recorded */
+    int force_steps;       /* capacity of the per-step force history (0 = off) */
+    /* local refinement ("REFINEMENT", "LR domain size", "LR location") */
+    int refinement;        /* 0 off, 1 one 2:1 fine patch */
+    int lr_nx, lr_ny, lr_nz;      /* fine nodes of the patch */
+    float lr_x0, lr_y0, lr_z0;    /* position of fine node 0 in coarse units */
+    int temporal_interp;   /* 1: coarse state interpolated at t+1/2 for the
+                              first fine sub-step; 0: reused as the reference */
+} lbm_config;
+
+typedef struct lbm_sizes {
+    int nx, ny, nz;        /* nodes of the level */
+    int nx_pad;            /* padded x stride in HBM */
+    size_t cells;          /* nx*ny*nz */
+    size_t device_bytes;   /* device memory held for the level */
+    int force_steps;       /* recorded force steps so far */
+} lbm_sizes;
+
+typedef struct lbm_ctx lbm_ctx;
+
+/* Fill a config with the defaults of MRTtest/DynInlet3.cu:10-60 that apply
+ * to one GPU (periodic box, BGK, FP32, no refinement). */
+void lbm_config_default(lbm_config* cfg);
+
+/* Replaces main()'s allocation and setup (MRTtest/DynInlet3.cu:2834-2990):
+ * validates cfg, selects the device, allocates the A/B lattices. */
+int lbm_create(const lbm_config* cfg, lbm_ctx** out);
+
+/* Replaces ImageFcn()/ImageFcnLR() (MRTtest/DynInlet3.cu:62-94).  mask is a
+ * dense [z][y][x] array of LBM_NODE_* codes, or NULL to generate it from
+ * cfg->geometry. */
+int lbm_set_geometry(lbm_ctx* ctx, const uint8_t* mask);
+
+/* Mask as classified (codes LBM_NODE_*) for level 0 (coarse) or 1 (fine). */
+int lbm_get_mask(lbm_ctx* ctx, int level, uint8_t* mask);
+
+/* Replaces initialize()/initializeLR() (MRTtest/DynInlet3.cu:1186-1251).
+ * rho, u, v, w: dense fields, or NULL for rho=1 and u=(umax,0,0); solids get
+ * u=0.  kind = LBM_INIT_*. */
+int lbm_init_equilibrium(lbm_ctx* ctx, int kind, const float* rho, const float* u,
+                         const float* v, const float* w);
+
+/* Taylor-Green initial state (no reference counterpart):
+ * u = u0 cos(kx) sin(ky), v = -u0 sin(kx) cos(ky), w = 0, rho = 1, k = 2 pi/nx,
+ * BGK equilibrium populations. */
+int lbm_init_taylor_green(lbm_ctx* ctx, float u0);
+
+/* Replaces one iteration of main()'s time loop (MRTtest/DynInlet3.cu:3119-3250):
+ * update_inn/top/bot (pull, BC dispatch, collide, bounce-back, momentum
+ * exchange), then, with refinement, two fine sub-steps with InterpCF and
+ * InterpFC.  Asynchronous. */
+int lbm_step(lbm_ctx* ctx, int nsteps);
+
+/* As lbm_step, and the elapsed time of those launches on the context's
+ * stream, from HIP events (synchronises). */
+int lbm_step_timed(lbm_ctx* ctx, int nsteps, float* elapsed_ms);
+
+/* Node fields as WriteResults computes them (MRTtest/DynInlet3.cu:2611-2620):
+ * rho = sum f, (u,v,w) = momentum sums.  Any pointer may be NULL. */
+int lbm_get_moments(lbm_ctx* ctx, int level, double* rho, double* u, double* v, double* w);
+
+/* Populations, dense [q][z][y][x], converted to float or double. */
+int lbm_get_populations(lbm_ctx* ctx, int level, float* f);
+int lbm_get_populations_f64(lbm_ctx* ctx, int level, double* f);
+int lbm_set_populations(lbm_ctx* ctx, int level, const float* f);
+int lbm_set_populations_f64(lbm_ctx* ctx, int level, const double* f);
+
+/* Momentum-exchange force per recorded step, FxFyFz (3 doubles per step),
+ * un-normalised (WriteForces, :2788-2797, divides by UMAX^2 ZDIM OBSTR1).
+ * Replaces the FX/FY/FZ atomicAdd arrays (:1819-1838). */
+int lbm_get_forces(lbm_ctx* ctx, double* fxyz, int max_steps, int* nsteps);
+
+/* Global mass and momentum (sum rho, sum u, sum v, sum w) with FP64 sums in a
+ * fixed order (deterministic).  No reference counterpart. */
+int lbm_get_mass_momentum(lbm_ctx* ctx, int level, double out[4]);
+
+/* Tecplot ASCII output as WriteResults (MRTtest/DynInlet3.cu:2601-2694):
+ * writes <casename>.dat (POINT zone, X,Y,Z,u,v,w,rho). */
+int lbm_write_tecplot(lbm_ctx* ctx, const char* casename);
+
+int lbm_query_sizes(lbm_ctx* ctx, int level, lbm_sizes* out);
+
+/* Steps taken so far (the reference's t). */
+int lbm_get_time(lbm_ctx* ctx, long long* t);
+
+const char* lbm_last_error(lbm_ctx* ctx);
+
+void lbm_destroy(lbm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,237 @@
+"""lbm_multigpu_amd -- Python binding of liblbm, the MI355X (gfx950) D3Q19
+MRT-LES stream-collide and local-refinement library.
+
+The library is the C ABI in include/lbm.h, built in-tree as
+lbm_multigpu_amd/liblbm.so.  This module is a thin ctypes layer over it:
+the hot path runs only in the HIP kernels; there is no CPU fallback, and
+importing the solver without the built library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .config import Config, read_inputs, write_inputs  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblbm.so")
+
+# constants of include/lbm.h
+MODEL_BGK, MODEL_MRT_CPU3D, MODEL_MRT_LES, MODEL_MRT_DHUMIERES = 0, 1, 2, 3
+BC_PERIODIC, BC_CLAMP, BC_WALL = 0, 1, 2
+STORE_F32, STORE_F16, STORE_BF16, STORE_F64 = 0, 1, 2, 3
+GEOM_NONE, GEOM_CHANNEL, GEOM_CPU3D = 0, 1, 2
+NODE_FLUID, NODE_SOLID, NODE_SOLID_FORCE, NODE_INLET_WEST, NODE_OUTLET_EAST, NODE_LID_WEST_REG = range(6)
+INIT_BGK, INIT_MRT = 0, 1
+
+# reference `im` code of each node code (MRTtest/DynInlet3.cu:1286-1390, CPU_3D/CPU_3D.cpp:1142)
+NODE_TO_IM = {NODE_FLUID: 0, NODE_SOLID: 1, NODE_SOLID_FORCE: 10, NODE_INLET_WEST: 400,
+              NODE_OUTLET_EAST: 300, NODE_LID_WEST_REG: 53}
+IM_TO_NODE = {v: k for k, v in NODE_TO_IM.items()}
+
+
+class LBMError(RuntimeError):
+    """A liblbm call returned a negative LBM_E* code."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"liblbm error {code}: {msg}")
+        self.code = code
+
+
+class lbm_config(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("bc", C.c_int * 3),
+                ("model", C.c_int), ("halfway", C.c_int), ("storage", C.c_int),
+                ("omega", C.c_float), ("cs", C.c_float), ("dpdy", C.c_float), ("umax", C.c_float),
+                ("inlet_amp", C.c_float), ("seed", C.c_uint64), ("s_dhum", C.c_float * 19),
+                ("geometry", C.c_int), ("obst_x", C.c_float), ("obst_y", C.c_float),
+                ("obst_z", C.c_float), ("obst_r", C.c_float), ("startf", C.c_int),
+                ("force_steps", C.c_int), ("refinement", C.c_int), ("lr_nx", C.c_int),
+                ("lr_ny", C.c_int), ("lr_nz", C.c_int), ("lr_x0", C.c_float), ("lr_y0", C.c_float),
+                ("lr_z0", C.c_float), ("temporal_interp", C.c_int)]
+
+
+class lbm_sizes(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nx_pad", C.c_int),
+                ("cells", C.c_size_t), ("device_bytes", C.c_size_t), ("force_steps", C.c_int)]
+
+
+EXPORTS = [
+    "lbm_config_default", "lbm_create", "lbm_set_geometry", "lbm_get_mask",
+    "lbm_init_equilibrium", "lbm_init_taylor_green", "lbm_step", "lbm_step_timed",
+    "lbm_get_moments", "lbm_get_populations", "lbm_get_populations_f64",
+    "lbm_set_populations", "lbm_set_populations_f64", "lbm_get_forces",
+    "lbm_get_mass_momentum", "lbm_write_tecplot", "lbm_query_sizes", "lbm_get_time",
+    "lbm_last_error", "lbm_destroy",
+]
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load liblbm.so (raises OSError if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"liblbm.so not built at {path}: run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    f32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    f64 = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+    u8 = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+    lib.lbm_config_default.argtypes = [C.POINTER(lbm_config)]
+    lib.lbm_config_default.restype = None
+    lib.lbm_create.argtypes = [C.POINTER(lbm_config), C.POINTER(P)]
+    lib.lbm_set_geometry.argtypes = [P, P]
+    lib.lbm_get_mask.argtypes = [P, C.c_int, u8]
+    lib.lbm_init_equilibrium.argtypes = [P, C.c_int, P, P, P, P]
+    lib.lbm_init_taylor_green.argtypes = [P, C.c_float]
+    lib.lbm_step.argtypes = [P, C.c_int]
+    lib.lbm_step_timed.argtypes = [P, C.c_int, C.POINTER(C.c_float)]
+    lib.lbm_get_moments.argtypes = [P, C.c_int, P, P, P, P]
+    lib.lbm_get_populations.argtypes = [P, C.c_int, f32]
+    lib.lbm_get_populations_f64.argtypes = [P, C.c_int, f64]
+    lib.lbm_set_populations.argtypes = [P, C.c_int, f32]
+    lib.lbm_set_populations_f64.argtypes = [P, C.c_int, f64]
+    lib.lbm_get_forces.argtypes = [P, P, C.c_int, C.POINTER(C.c_int)]
+    lib.lbm_get_mass_momentum.argtypes = [P, C.c_int, f64]
+    lib.lbm_write_tecplot.argtypes = [P, C.c_char_p]
+    lib.lbm_query_sizes.argtypes = [P, C.c_int, C.POINTER(lbm_sizes)]
+    lib.lbm_get_time.argtypes = [P, C.POINTER(C.c_longlong)]
+    lib.lbm_last_error.argtypes = [P]
+    lib.lbm_last_error.restype = C.c_char_p
+    lib.lbm_destroy.argtypes = [P]
+    lib.lbm_destroy.restype = None
+    lib.lbm_hash_u32.argtypes = [C.c_uint64, C.c_uint64]
+    lib.lbm_hash_u32.restype = C.c_uint32
+    lib.lbm_inlet_profile.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, f32]
+    lib.lbm_inlet_profile.restype = None
+    _lib = lib
+    return lib
+
+
+def default_config():
+    cfg = lbm_config()
+    load_library().lbm_config_default(C.byref(cfg))
+    return cfg
+
+
+def inlet_profile(ny, nz, u0, amp, seed):
+    """The library's seeded inlet table u_in[z*ny + y] (host code, no GPU)."""
+    out = np.empty(ny * nz, np.float32)
+    load_library().lbm_inlet_profile(ny, nz, u0, amp, seed, out)
+    return out
+
+
+class Solver:
+    """One liblbm context.  Methods mirror the C ABI one to one."""
+
+    def __init__(self, cfg):
+        if isinstance(cfg, Config):
+            cfg = cfg.to_c()
+        self.lib = load_library()
+        self.cfg = cfg
+        h = C.c_void_p()
+        rc = self.lib.lbm_create(C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise LBMError(rc, self.lib.lbm_last_error(None).decode())
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise LBMError(rc, self.lib.lbm_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lbm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sizes(self, level=0):
+        s = lbm_sizes()
+        self._check(self.lib.lbm_query_sizes(self.h, level, C.byref(s)))
+        return s
+
+    def shape(self, level=0):
+        s = self.sizes(level)
+        return (s.nz, s.ny, s.nx)
+
+    def set_geometry(self, mask=None):
+        if mask is None:
+            self._check(self.lib.lbm_set_geometry(self.h, None))
+        else:
+            m = np.ascontiguousarray(mask, dtype=np.uint8)
+            self._check(self.lib.lbm_set_geometry(self.h, m.ctypes.data))
+
+    def get_mask(self, level=0):
+        out = np.empty(self.shape(level), np.uint8)
+        self._check(self.lib.lbm_get_mask(self.h, level, out.reshape(-1)))
+        return out
+
+    def init_equilibrium(self, kind=INIT_BGK, rho=None, u=None, v=None, w=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float32) for a in (rho, u, v, w)]
+        ptrs = [None if a is None else a.ctypes.data for a in arrs]
+        self._check(self.lib.lbm_init_equilibrium(self.h, kind, *ptrs))
+
+    def init_taylor_green(self, u0):
+        self._check(self.lib.lbm_init_taylor_green(self.h, u0))
+
+    def step(self, n=1):
+        self._check(self.lib.lbm_step(self.h, n))
+
+    def step_timed(self, n):
+        ms = C.c_float()
+        self._check(self.lib.lbm_step_timed(self.h, n, C.byref(ms)))
+        return ms.value
+
+    def moments(self, level=0):
+        shp = self.shape(level)
+        out = [np.empty(shp, np.float64) for _ in range(4)]
+        self._check(self.lib.lbm_get_moments(self.h, level, *[o.ctypes.data for o in out]))
+        return out
+
+    def populations(self, level=0, f64=False):
+        shp = (19,) + self.shape(level)
+        out = np.empty(shp, np.float64 if f64 else np.float32)
+        fn = self.lib.lbm_get_populations_f64 if f64 else self.lib.lbm_get_populations
+        self._check(fn(self.h, level, out.reshape(-1)))
+        return out
+
+    def set_populations(self, f, level=0):
+        f = np.ascontiguousarray(f)
+        if f.dtype == np.float64:
+            self._check(self.lib.lbm_set_populations_f64(self.h, level, f.reshape(-1)))
+        else:
+            self._check(self.lib.lbm_set_populations(self.h, level, f.astype(np.float32).reshape(-1)))
+
+    def forces(self):
+        n = C.c_int()
+        self._check(self.lib.lbm_get_forces(self.h, None, 0, C.byref(n)))
+        out = np.zeros((max(n.value, 0), 3), np.float64)
+        if n.value:
+            self._check(self.lib.lbm_get_forces(self.h, out.ctypes.data, n.value, C.byref(n)))
+        return out
+
+    def mass_momentum(self, level=0):
+        out = np.zeros(4, np.float64)
+        self._check(self.lib.lbm_get_mass_momentum(self.h, level, out))
+        return out
+
+    def write_tecplot(self, casename):
+        self._check(self.lib.lbm_write_tecplot(self.h, casename.encode()))
+
+    def time(self):
+        t = C.c_longlong()
+        self._check(self.lib.lbm_get_time(self.h, C.byref(t)))
+        return t.value

@@ -1,0 +1,333 @@
+// collide.h -- device-side D3Q19 collision operators and node boundary
+// conditions, written for 64-wide CDNA wavefronts: every loop over the 19
+// directions or 19 moments is unrolled at compile time so populations stay in
+// VGPRs, and the moment transforms use the pair sums/differences of opposite
+// directions (even moments from sums, odd moments from differences), which
+// halves both the forward and the inverse transform.
+#pragma once
+
+#include "d3q19.h"
+
+namespace lbm {
+
+enum Model : int { MODEL_BGK = 0, MODEL_MRT_CPU3D = 1, MODEL_MRT_LES = 2, MODEL_MRT_DHUM = 3 };
+
+template <class R>
+struct CollideParams {
+    R omega;       // base rate 1/tau0
+    R cs;          // reference CS (Cs^2); 0 = no LES
+    R dpdy;        // AddForce pressure gradient
+    R s1, s2, s4, s10, s16; // d'Humieres rates (e, eps, q, pi, m)
+};
+
+template <class R>
+__device__ __forceinline__ R wgt(int i)
+{
+    return R(WNUM[i]) / R(36);
+}
+
+// BGK equilibrium w_i (rho + 3 c.u + 4.5 (c.u)^2 - 1.5 u.u), u the momentum
+// (CPU_3D/CPU_3D.cpp:135-153).
+template <class R>
+__device__ __forceinline__ void feq_bgk(R rho, R u, R v, R w, R feq[Q])
+{
+    const R base = rho - R(1.5) * (u * u + v * v + w * w);
+    feq[0] = wgt<R>(0) * base;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        const int i = PP[j];
+        const R cu = R(CX[i]) * u + R(CY[i]) * v + R(CZ[i]) * w;
+        const R a = wgt<R>(i) * (base + R(4.5) * cu * cu);
+        const R b = wgt<R>(i) * R(3) * cu;
+        feq[i] = a + b;
+        feq[OPP[i]] = a - b;
+    }
+}
+
+// Pair sums s[j] = f_i + f_opp and differences d[j] = f_i - f_opp.
+template <class R>
+__device__ __forceinline__ void pair_split(const R f[Q], R s[NP], R d[NP])
+{
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        s[j] = f[PP[j]] + f[OPP[PP[j]]];
+        d[j] = f[PP[j]] - f[OPP[PP[j]]];
+    }
+}
+
+// m_k = sum_i M[k][i] f_i from the pair sums/differences.
+template <class R>
+__device__ __forceinline__ R moment(int k, R f0, const R s[NP], const R d[NP])
+{
+    R acc = R(mrow(k, 0)) * f0;
+    if (even_moment(k)) {
+#pragma unroll
+        for (int j = 0; j < NP; ++j)
+            if (mrow(k, PP[j]) != 0) acc += R(mrow(k, PP[j])) * s[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < NP; ++j)
+            if (mrow(k, PP[j]) != 0) acc += R(mrow(k, PP[j])) * d[j];
+    }
+    return acc;
+}
+
+// Second moments P_ab = sum c_a c_b f and conserved moments.
+template <class R>
+struct Macro {
+    R rho, jx, jy, jz, pxx, pyy, pzz, pxy, pyz, pxz;
+};
+
+template <class R>
+__device__ __forceinline__ Macro<R> macro_from_pairs(R f0, const R s[NP], const R d[NP])
+{
+    Macro<R> m;
+    R r = f0;
+    R jx = 0, jy = 0, jz = 0, pxx = 0, pyy = 0, pzz = 0, pxy = 0, pyz = 0, pxz = 0;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        const int i = PP[j];
+        r += s[j];
+        if (CX[i]) jx += R(CX[i]) * d[j];
+        if (CY[i]) jy += R(CY[i]) * d[j];
+        if (CZ[i]) jz += R(CZ[i]) * d[j];
+        if (CX[i]) pxx += s[j];
+        if (CY[i]) pyy += s[j];
+        if (CZ[i]) pzz += s[j];
+        if (CX[i] * CY[i]) pxy += R(CX[i] * CY[i]) * s[j];
+        if (CY[i] * CZ[i]) pyz += R(CY[i] * CZ[i]) * s[j];
+        if (CX[i] * CZ[i]) pxz += R(CX[i] * CZ[i]) * s[j];
+    }
+    m.rho = r; m.jx = jx; m.jy = jy; m.jz = jz;
+    m.pxx = pxx; m.pyy = pyy; m.pzz = pzz; m.pxy = pxy; m.pyz = pyz; m.pxz = pxz;
+    return m;
+}
+
+// Smagorinsky effective rate (MRTtest/DynInlet3.cu:628-735):
+// Pi_neq = P - rho/3 I - j j (the second moment of f - feq_BGK),
+// Q = sqrt(Pi:Pi), tau = tau0/2 + sqrt(tau0^2 + 18 sqrt(2) CS Q)/2.
+template <class R>
+__device__ __forceinline__ R les_omega(const Macro<R>& m, R omega0, R cs)
+{
+    const R third = m.rho * R(1.0 / 3.0);
+    const R nxx = m.pxx - third - m.jx * m.jx;
+    const R nyy = m.pyy - third - m.jy * m.jy;
+    const R nzz = m.pzz - third - m.jz * m.jz;
+    const R nxy = m.pxy - m.jx * m.jy;
+    const R nyz = m.pyz - m.jy * m.jz;
+    const R nxz = m.pxz - m.jx * m.jz;
+    const R q2 = nxx * nxx + nyy * nyy + nzz * nzz + R(2) * (nxy * nxy + nyz * nyz + nxz * nxz);
+    const R q = sqrt(q2);
+    const R tau0 = R(1) / omega0;
+    const R tau = R(0.5) * tau0 + R(0.5) * sqrt(tau0 * tau0 + R(18.0 * 1.4142135623730951) * cs * q);
+    return R(1) / tau;
+}
+
+// AddForce (MRTtest/DynInlet3.cu:229-253): f_i -= 3 w_i c_iy dpdy.
+template <class R>
+__device__ __forceinline__ void add_force(R f[Q], R dpdy)
+{
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+        if (CY[i]) f[i] -= R(3 * CY[i]) * wgt<R>(i) * dpdy;
+}
+
+// BGK collide (CPU_3D/CPU_3D.cpp:116-160; DynInlet3.cu:802-830 adds the LES
+// rate and AddForce).
+template <class R>
+__device__ __forceinline__ void collide_bgk(R f[Q], const CollideParams<R>& cp)
+{
+    R s[NP], d[NP];
+    pair_split(f, s, d);
+    R rho = f[0], u = 0, v = 0, w = 0;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        const int i = PP[j];
+        rho += s[j];
+        if (CX[i]) u += R(CX[i]) * d[j];
+        if (CY[i]) v += R(CY[i]) * d[j];
+        if (CZ[i]) w += R(CZ[i]) * d[j];
+    }
+    R omega = cp.omega;
+    if (cp.cs > R(0)) {
+        const Macro<R> m = macro_from_pairs(f[0], s, d);
+        omega = les_omega(m, cp.omega, cp.cs);
+    }
+    R feq[Q];
+    feq_bgk(rho, u, v, w, feq);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] -= omega * (f[i] - feq[i]);
+    if (cp.dpdy != R(0)) add_force(f, cp.dpdy);
+}
+
+// MRT collide, f <- f - M^-1 S (m - meq), for the three presets:
+//  MODEL_MRT_CPU3D: CPU_3D/CPU_3D.cpp:229-279.  meq has eps = -475/63 j^2,
+//    3pixx = piww = m16..18 = 0; S = 1 except s9 = s11 = s15 = omega
+//    (m13, m14 at rate 1 as coded there).  No LES, no body force.
+//  MODEL_MRT_LES: MRTtest/DynInlet3.cu:759-789 with bgk_meq (:347-365,
+//    whose 3pixx entry is -meq9/6 as coded); s9,11,13,14,15 = omega_LES,
+//    every other moment to equilibrium; then AddForce.
+//  MODEL_MRT_DHUM: as MRT_LES with the d'Humieres rates of :774-783.
+template <int MODEL, class R>
+__device__ __forceinline__ void collide_mrt(R f[Q], const CollideParams<R>& cp)
+{
+    R s[NP], d[NP];
+    pair_split(f, s, d);
+    R m[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) m[k] = moment(k, f[0], s, d);
+    const R rho = m[0], u = m[3], v = m[5], w = m[7];
+    const R usqr = u * u + v * v + w * w;
+
+    R omega = cp.omega;
+    if (MODEL != MODEL_MRT_CPU3D && cp.cs > R(0)) {
+        const Macro<R> mm = macro_from_pairs(f[0], s, d);
+        omega = les_omega(mm, cp.omega, cp.cs);
+    }
+
+    R meq[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) meq[k] = R(0);
+    meq[1] = R(-11) * rho + R(19) * usqr;
+    meq[4] = R(-2.0 / 3.0) * u;
+    meq[6] = R(-2.0 / 3.0) * v;
+    meq[8] = R(-2.0 / 3.0) * w;
+    meq[9] = R(2) * u * u - (v * v + w * w);
+    meq[11] = v * v - w * w;
+    meq[13] = u * v;
+    meq[14] = v * w;
+    meq[15] = u * w;
+    if (MODEL == MODEL_MRT_CPU3D) {
+        meq[2] = R(-475.0 / 63.0) * usqr;
+    } else {
+        meq[2] = R(3) * rho - R(5.5) * usqr;
+        meq[10] = R(-0.5) * meq[9] * R(1.0 / 3.0);
+        meq[12] = R(-0.5) * meq[11];
+    }
+
+    // rates
+    R rate[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) rate[k] = R(1);
+    rate[9] = omega; rate[11] = omega; rate[15] = omega;
+    if (MODEL != MODEL_MRT_CPU3D) { rate[13] = omega; rate[14] = omega; }
+    if (MODEL == MODEL_MRT_DHUM) {
+        rate[1] = cp.s1; rate[2] = cp.s2;
+        rate[4] = cp.s4; rate[6] = cp.s4; rate[8] = cp.s4;
+        rate[10] = cp.s10; rate[12] = cp.s10;
+        rate[16] = cp.s16; rate[17] = cp.s16; rate[18] = cp.s16;
+    }
+
+    // scaled non-equilibrium parts dm_k = s_k (m_k - meq_k) / |row_k|^2
+    R dm[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const bool conserved = (k == 0 || k == 3 || k == 5 || k == 7);
+        dm[k] = conserved ? R(0) : rate[k] * (m[k] - meq[k]) * (R(1) / R(NORM[k]));
+    }
+
+    // f_i -= sum_k M[k][i] dm_k, split into even (E) and odd (O) parts per pair
+    {
+        R e0 = 0;
+#pragma unroll
+        for (int k = 0; k < Q; ++k)
+            if (even_moment(k) && mrow(k, 0) != 0 && !(k == 0)) e0 += R(mrow(k, 0)) * dm[k];
+        f[0] -= e0;
+    }
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        const int i = PP[j];
+        R E = 0, O = 0;
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            if (k == 0 || k == 3 || k == 5 || k == 7) continue;
+            if (mrow(k, i) == 0) continue;
+            if (even_moment(k)) E += R(mrow(k, i)) * dm[k];
+            else O += R(mrow(k, i)) * dm[k];
+        }
+        f[i] -= E + O;
+        f[OPP[i]] -= E - O;
+    }
+    if (MODEL != MODEL_MRT_CPU3D && cp.dpdy != R(0)) add_force(f, cp.dpdy);
+}
+
+template <int MODEL, class R>
+__device__ __forceinline__ void collide(R f[Q], const CollideParams<R>& cp)
+{
+    if (MODEL == MODEL_BGK) collide_bgk(f, cp);
+    else collide_mrt<MODEL>(f, cp);
+}
+
+// West_Extrap (MRTtest/DynInlet3.cu:973-1017) applied to the neighbour's
+// unstreamed populations: f = M^-1 m with m3 = u_in, m5 = m7 = 0 and every
+// other moment from f.  As a correction: f_i += c_ix (u_in - jx)/10 - c_iy jy/10 - c_iz jz/10.
+template <class R>
+__device__ __forceinline__ void west_extrap(R f[Q], R u_in)
+{
+    R jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        if (CX[i]) jx += R(CX[i]) * f[i];
+        if (CY[i]) jy += R(CY[i]) * f[i];
+        if (CZ[i]) jz += R(CZ[i]) * f[i];
+    }
+    const R ax = (u_in - jx) * R(0.1), ay = -jy * R(0.1), az = -jz * R(0.1);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] += R(CX[i]) * ax + R(CY[i]) * ay + R(CZ[i]) * az;
+}
+
+// East_Extrap (MRTtest/DynInlet3.cu:926-971): f = M^-1 m with m0 = rho_out,
+// i.e. f_i += (rho_out - sum f)/19.
+template <class R>
+__device__ __forceinline__ void east_extrap(R f[Q], R rho_out)
+{
+    R rho = 0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) rho += f[i];
+    const R a = (rho_out - rho) * R(1.0 / 19.0);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] += a;
+}
+
+// DirichletWest_Regularized (CPU_3D/CPU_3D.cpp:974-1054) as coded there:
+// velocity (0, u_lid, 0); feq11/feq13 use (v+w)*(u+w) and (-v+w)*(u+w); the
+// unknown f8 becomes feq6 + f6 - feq6; then f_i = feq_i + 4.5 w_i Q_i:Pi_neq.
+template <class R>
+__device__ __forceinline__ void lid_west_reg(R f[Q], R u_lid)
+{
+    const R u = 0, v = u_lid, w = 0;
+    const R usqr = v * v;
+    const R rho = u + (f[0] + f[2] + f[4] + f[9] + f[11] + f[13] + f[14] + f[16] + f[18] +
+                       R(2) * (f[3] + f[6] + f[7] + f[12] + f[17]));
+    R feq[Q];
+    feq_bgk(rho, u, v, w, feq);
+    feq[11] = wgt<R>(11) * (rho + R(3) * (v + w) + R(4.5) * (v + w) * (u + w) - R(1.5) * usqr);
+    feq[13] = wgt<R>(13) * (rho + R(3) * (-v + w) + R(4.5) * (-v + w) * (u + w) - R(1.5) * usqr);
+    f[1] = feq[1] + f[3] - feq[3];
+    f[5] = feq[5] + f[7] - feq[7];
+    f[8] = feq[6] + f[6] - feq[6];
+    f[10] = feq[10] + f[17] - feq[17];
+    f[15] = feq[15] + f[12] - feq[12];
+    R pxx = 0, pyy = 0, pzz = 0, pxy = 0, pyz = 0, pxz = 0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const R dd = f[i] - feq[i];
+        if (CX[i]) pxx += dd;
+        if (CY[i]) pyy += dd;
+        if (CZ[i]) pzz += dd;
+        if (CX[i] * CY[i]) pxy += R(CX[i] * CY[i]) * dd;
+        if (CY[i] * CZ[i]) pyz += R(CY[i] * CZ[i]) * dd;
+        if (CX[i] * CZ[i]) pxz += R(CX[i] * CZ[i]) * dd;
+    }
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const R qxx = R(CX[i] * CX[i]) - R(1.0 / 3.0);
+        const R qyy = R(CY[i] * CY[i]) - R(1.0 / 3.0);
+        const R qzz = R(CZ[i] * CZ[i]) - R(1.0 / 3.0);
+        const R qs = qxx * pxx + qyy * pyy + qzz * pzz +
+                     R(2) * (R(CX[i] * CY[i]) * pxy + R(CY[i] * CZ[i]) * pyz + R(CX[i] * CZ[i]) * pxz);
+        f[i] = feq[i] + R(4.5) * wgt<R>(i) * qs;
+    }
+}
+
+} // namespace lbm

@@ -1,0 +1,401 @@
+// kernels.h -- HIP kernels of the D3Q19 stream-collide path for gfx950.
+//
+// One thread per node, workgroups of 64x4 threads: each 64-lane wavefront
+// owns 64 consecutive x of one (y, z) row, so every one of the 19 population
+// loads and stores of a wavefront is a contiguous 256-byte (FP32) row segment;
+// only the 10 directions with c_x != 0 are offset by one element.
+// Workgroups are renumbered so that each of the 8 XCDs receives one
+// contiguous slab of the domain (hardware hands workgroup ids to XCDs
+// round-robin); neighbouring rows, whose cache lines the x-shifted loads
+// share, then live in the same XCD's L2.
+#pragma once
+
+#include "collide.h"
+#include "storage.h"
+
+namespace lbm {
+
+enum Bc : int { BC_PERIODIC = 0, BC_CLAMP = 1, BC_WALL = 2 };
+
+constexpr int TX = 64; // threads along x per wavefront
+constexpr int TY = 4;  // wavefronts (rows along y) per workgroup
+
+struct StepParams {
+    const void* fin;
+    void* fout;
+    const unsigned char* mask; // node codes, padded layout; nullptr = all fluid
+    const float* u_in;         // west inlet profile [z*ny + y]
+    double* fpart;             // momentum-exchange partials [blocks*3], or nullptr
+    long long qs;              // elements per direction plane
+    int nx, ny, nz, nxp;
+    int nseg, nyb;             // x segments of 64 and y groups of 4
+    int bcx, bcy, bcz;
+    int halfway;
+    float omega, cs, dpdy, u_lid;
+    float s1, s2, s4, s10, s16;
+};
+
+// XCD-aware renumbering: the hardware id hw goes to XCD hw % 8; give XCD j
+// the logical block range [j*G/8, (j+1)*G/8) (bijective for any G).
+__device__ __forceinline__ int logical_block()
+{
+    const int G = gridDim.x, hw = blockIdx.x;
+    const int per = G >> 3, rem = G & 7, xcd = hw & 7;
+    return xcd * per + (xcd < rem ? xcd : rem) + (hw >> 3);
+}
+
+__device__ __forceinline__ void axis_nb(int c, int n, int bc, int& m, int& p, bool& om, bool& op)
+{
+    om = (c == 0);
+    op = (c == n - 1);
+    if (bc == BC_PERIODIC) {
+        m = om ? n - 1 : c - 1;
+        p = op ? 0 : c + 1;
+        om = op = false;
+    } else {
+        m = om ? 0 : c - 1;
+        p = op ? n - 1 : c + 1;
+        if (bc != BC_WALL) om = op = false;
+    }
+}
+
+template <class R>
+__device__ __forceinline__ R wave_sum(R v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Fused pull stream + boundary dispatch + collide + bounce-back for one node
+// (the work of update_inn, MRTtest/DynInlet3.cu:1612-1840, and of the CPU
+// sweep CPU_3D/CPU_3D.cpp:1320-1443).
+template <int MODEL, class S>
+__device__ __forceinline__ void node_update(const StepParams& p, int x, int y, int z,
+                                            typename Arith<S>::R& fx, typename Arith<S>::R& fy,
+                                            typename Arith<S>::R& fz)
+{
+    using R = typename Arith<S>::R;
+    const S* __restrict__ fin = static_cast<const S*>(p.fin);
+    S* __restrict__ fout = static_cast<S*>(p.fout);
+    const long long qs = p.qs;
+    const int cell = x + p.nxp * (y + p.ny * z);
+    const unsigned char code = p.mask ? p.mask[cell] : (unsigned char)0;
+    const unsigned char base = code & 0x7f;
+    if (p.halfway && is_solid(base)) return; // inert under link bounce-back
+
+    int xm, xp, ym, yp, zm, zp;
+    bool oxm, oxp, oym, oyp, ozm, ozp;
+    axis_nb(x, p.nx, p.bcx, xm, xp, oxm, oxp);
+    axis_nb(y, p.ny, p.bcy, ym, yp, oym, oyp);
+    axis_nb(z, p.nz, p.bcz, zm, zp, ozm, ozp);
+    const int Y[3] = {ym, y, yp};
+    const int Z[3] = {zm, z, zp};
+    const int X[3] = {xm, x, xp};
+    int row[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) row[a][b] = p.nxp * (Y[a] + p.ny * Z[b]);
+
+    R f[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const int src = row[1 - CY[i]][1 - CZ[i]] + X[1 - CX[i]];
+        f[i] = Store<S>::load(fin + i * qs + src, i);
+    }
+
+    if (code & NODE_NEAR) {
+        // links that bounce: out of the domain through a wall face, or (link
+        // mode) from a solid node; the population comes back reversed from
+        // this node's own previous post-collision value.
+#pragma unroll
+        for (int i = 1; i < Q; ++i) {
+            const bool out = (CX[i] > 0 && oxm) || (CX[i] < 0 && oxp) || (CY[i] > 0 && oym) ||
+                             (CY[i] < 0 && oyp) || (CZ[i] > 0 && ozm) || (CZ[i] < 0 && ozp);
+            unsigned char sc = 0;
+            if (!out && p.halfway) sc = p.mask[row[1 - CY[i]][1 - CZ[i]] + X[1 - CX[i]]] & 0x7f;
+            const bool bounce = out || is_solid(sc);
+            if (bounce) {
+                const R v = Store<S>::load(fin + OPP[i] * qs + cell, OPP[i]);
+                f[i] = v;
+                if (p.fpart && sc == NODE_SOLID_FORCE) {
+                    fx += R(2 * CX[OPP[i]]) * v;
+                    fy += R(2 * CY[OPP[i]]) * v;
+                    fz += R(2 * CZ[OPP[i]]) * v;
+                }
+            }
+        }
+    }
+
+    if (is_solid(base)) {
+        // node bounce-back (MRTtest/DynInlet3.cu:1673-1705): store reversed;
+        // momentum exchange 2 sum c_i f_i on im 10 nodes (:1675-1687)
+        if (p.fpart && base == NODE_SOLID_FORCE) {
+#pragma unroll
+            for (int i = 1; i < Q; ++i) {
+                if (CX[i]) fx += R(2 * CX[i]) * f[i];
+                if (CY[i]) fy += R(2 * CY[i]) * f[i];
+                if (CZ[i]) fz += R(2 * CZ[i]) * f[i];
+            }
+        }
+        Store<S>::store(fout + cell, f[0], 0);
+#pragma unroll
+        for (int i = 1; i < Q; ++i) Store<S>::store(fout + OPP[i] * qs + cell, f[i], OPP[i]);
+        return;
+    }
+
+    if (base == NODE_INLET_WEST || base == NODE_OUTLET_EAST) {
+        // Extrap conditions start from the unstreamed neighbour (:1725-1746)
+        const int nb = base == NODE_INLET_WEST ? (x + 1 < p.nx ? cell + 1 : cell)
+                                               : (x > 0 ? cell - 1 : cell);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] = Store<S>::load(fin + i * qs + nb, i);
+        if (base == NODE_INLET_WEST) west_extrap(f, R(p.u_in[z * p.ny + y]));
+        else east_extrap(f, R(1));
+    } else if (base == NODE_LID_WEST_REG) {
+        lid_west_reg(f, R(p.u_lid));
+    }
+
+    CollideParams<R> cp;
+    cp.omega = R(p.omega);
+    cp.cs = R(p.cs);
+    cp.dpdy = R(p.dpdy);
+    cp.s1 = R(p.s1); cp.s2 = R(p.s2); cp.s4 = R(p.s4); cp.s10 = R(p.s10); cp.s16 = R(p.s16);
+    collide<MODEL>(f, cp);
+
+#pragma unroll
+    for (int i = 0; i < Q; ++i) Store<S>::store(fout + i * qs + cell, f[i], i);
+}
+
+template <int MODEL, class S>
+__global__ void __launch_bounds__(TX* TY) k_stream_collide(StepParams p)
+{
+    using R = typename Arith<S>::R;
+    const int lb = logical_block();
+    const int seg = lb % p.nseg;
+    const int t2 = lb / p.nseg;
+    const int yb = t2 % p.nyb;
+    const int z = t2 / p.nyb;
+    const int x = seg * TX + threadIdx.x;
+    const int y = yb * TY + threadIdx.y;
+    R fx = 0, fy = 0, fz = 0;
+    if (x < p.nx && y < p.ny) node_update<MODEL, S>(p, x, y, z, fx, fy, fz);
+    if (p.fpart) {
+        __shared__ double red[3][TY];
+        fx = wave_sum(fx);
+        fy = wave_sum(fy);
+        fz = wave_sum(fz);
+        if (threadIdx.x == 0) {
+            red[0][threadIdx.y] = double(fx);
+            red[1][threadIdx.y] = double(fy);
+            red[2][threadIdx.y] = double(fz);
+        }
+        __syncthreads();
+        if (threadIdx.x < 3 && threadIdx.y == 0) {
+            double s = 0;
+#pragma unroll
+            for (int w = 0; w < TY; ++w) s += red[threadIdx.x][w];
+            p.fpart[(long long)lb * 3 + threadIdx.x] = s;
+        }
+    }
+}
+
+// Deterministic FP64 reduction of ncomp-component partials (fixed order:
+// strided per thread, then a fixed tree), one workgroup of 256 threads.
+__global__ void __launch_bounds__(256) k_reduce_partials(const double* part, long long n, int ncomp,
+                                                        double* out)
+{
+    __shared__ double sh[256];
+    for (int c = 0; c < ncomp; ++c) {
+        double s = 0;
+        for (long long i = threadIdx.x; i < n; i += 256) s += part[i * ncomp + c];
+        sh[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[c] = sh[0];
+        __syncthreads();
+    }
+}
+
+// Per-block FP64 sums of (rho, jx, jy, jz) over non-solid nodes.
+template <class S>
+__global__ void __launch_bounds__(TX* TY) k_mass_momentum(const S* f, const unsigned char* mask,
+                                                        long long qs, int nx, int ny, int nxp,
+                                                        int nseg, int nyb, double* part)
+{
+    using R = typename Arith<S>::R;
+    const int lb = blockIdx.x;
+    const int seg = lb % nseg;
+    const int t2 = lb / nseg;
+    const int yb = t2 % nyb;
+    const int z = t2 / nyb;
+    const int x = seg * TX + threadIdx.x;
+    const int y = yb * TY + threadIdx.y;
+    double acc[4] = {0, 0, 0, 0};
+    if (x < nx && y < ny) {
+        const int cell = x + nxp * (y + ny * z);
+        if (!(mask && is_solid(mask[cell]))) {
+            R r = 0, a = 0, b = 0, c = 0;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                const R v = Store<S>::to_real(f[i * qs + cell], i);
+                r += v;
+                if (CX[i]) a += R(CX[i]) * v;
+                if (CY[i]) b += R(CY[i]) * v;
+                if (CZ[i]) c += R(CZ[i]) * v;
+            }
+            acc[0] = r; acc[1] = a; acc[2] = b; acc[3] = c;
+        }
+    }
+    __shared__ double red[4][TY];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double s = wave_sum(acc[k]);
+        if (threadIdx.x == 0) red[k][threadIdx.y] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 && threadIdx.y == 0) {
+        double s = 0;
+        for (int w = 0; w < TY; ++w) s += red[threadIdx.x][w];
+        part[(long long)lb * 4 + threadIdx.x] = s;
+    }
+}
+
+// Node fields as WriteResults computes them: rho = sum f, (u,v,w) = momentum.
+template <class S>
+__global__ void k_macro(const S* f, long long qs, int nx, int ny, int nz, int nxp, double* rho,
+                        double* u, double* v, double* w)
+{
+    using R = typename Arith<S>::R;
+    const long long n = (long long)nx * ny * nz;
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int x = int(c % nx);
+        const long long r = c / nx;
+        const int y = int(r % ny);
+        const int z = int(r / ny);
+        const long long cell = x + (long long)nxp * (y + (long long)ny * z);
+        R s = 0, a = 0, b = 0, d = 0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const R val = Store<S>::to_real(f[i * qs + cell], i);
+            s += val;
+            if (CX[i]) a += R(CX[i]) * val;
+            if (CY[i]) b += R(CY[i]) * val;
+            if (CZ[i]) d += R(CZ[i]) * val;
+        }
+        rho[c] = s; u[c] = a; v[c] = b; w[c] = d;
+    }
+}
+
+// Dense <-> padded population copies with conversion.
+template <class S, class T>
+__global__ void k_pack(const S* f, long long qs, int nx, int ny, int nz, int nxp, T* out)
+{
+    const long long n = (long long)nx * ny * nz;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n * Q;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int i = int(t / n);
+        const long long c = t % n;
+        const int x = int(c % nx);
+        const long long r = c / nx;
+        const int y = int(r % ny);
+        const int z = int(r / ny);
+        const long long cell = x + (long long)nxp * (y + (long long)ny * z);
+        out[t] = T(Store<S>::to_real(f[i * qs + cell], i));
+    }
+}
+
+template <class S, class T>
+__global__ void k_unpack(const T* in, long long qs, int nx, int ny, int nz, int nxp, S* f, S* f2)
+{
+    using R = typename Arith<S>::R;
+    const long long n = (long long)nx * ny * nz;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n * Q;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int i = int(t / n);
+        const long long c = t % n;
+        const int x = int(c % nx);
+        const long long r = c / nx;
+        const int y = int(r % ny);
+        const int z = int(r / ny);
+        const long long cell = x + (long long)nxp * (y + (long long)ny * z);
+        const S v = Store<S>::from_real(R(in[t]), i);
+        f[i * qs + cell] = v;
+        if (f2) f2[i * qs + cell] = v;
+    }
+}
+
+// Equilibrium initialisation of both lattices (initialize,
+// MRTtest/DynInlet3.cu:1186-1216; CPU_3D/CPU_3D.cpp:1218-1254).
+// Fields are dense; nullptr fields mean rho = 1, u = (u0, 0, 0).  Solid
+// nodes get u = 0.  kind 0: BGK feq; kind 1: mrt_meq + InvertMoments.
+// kind 2: Taylor-Green (u0 cos kx sin ky, -u0 sin kx cos ky, 0).
+template <class S>
+__global__ void k_init(S* fa, S* fb, long long qs, int nx, int ny, int nz, int nxp,
+                       const unsigned char* mask, const float* rho_in, const float* u_in,
+                       const float* v_in, const float* w_in, float u0, int kind)
+{
+    using R = typename Arith<S>::R;
+    const long long n = (long long)nx * ny * nz;
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n;
+         c += (long long)gridDim.x * blockDim.x) {
+        const int x = int(c % nx);
+        const long long r = c / nx;
+        const int y = int(r % ny);
+        const int z = int(r / ny);
+        const long long cell = x + (long long)nxp * (y + (long long)ny * z);
+        R rho = rho_in ? R(rho_in[c]) : R(1);
+        R u = u_in ? R(u_in[c]) : R(u0);
+        R v = v_in ? R(v_in[c]) : R(0);
+        R w = w_in ? R(w_in[c]) : R(0);
+        if (kind == 2) {
+            const double kx = 2.0 * 3.14159265358979323846 * x / nx;
+            const double ky = 2.0 * 3.14159265358979323846 * y / nx;
+            u = R(double(u0) * cos(kx) * sin(ky));
+            v = R(-double(u0) * sin(kx) * cos(ky));
+            w = R(0);
+            rho = R(1);
+        }
+        if (mask && is_solid(mask[cell])) { u = v = w = R(0); }
+        R feq[Q];
+        if (kind == 1) {
+            // mrt_meq (MRTtest/DynInlet3.cu:329-345) then InvertMoments
+            const R usqr = u * u + v * v + w * w;
+            R m[Q];
+#pragma unroll
+            for (int k = 0; k < Q; ++k) m[k] = R(0);
+            m[0] = rho;
+            m[1] = R(-11) * rho + R(19) * usqr;
+            m[2] = R(475.0 / 63.0) * usqr;
+            m[3] = u; m[4] = R(-2.0 / 3.0) * u;
+            m[5] = v; m[6] = R(-2.0 / 3.0) * v;
+            m[7] = w; m[8] = R(-2.0 / 3.0) * w;
+            m[9] = R(2) * u * u - (v * v + w * w);
+            m[11] = v * v - w * w;
+            m[13] = u * v; m[14] = v * w; m[15] = u * w;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                R s = 0;
+#pragma unroll
+                for (int k = 0; k < Q; ++k)
+                    if (mrow(k, i) != 0) s += (R(mrow(k, i)) / R(NORM[k])) * m[k];
+                feq[i] = s;
+            }
+        } else {
+            feq_bgk(rho, u, v, w, feq);
+        }
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const S sv = Store<S>::from_real(feq[i], i);
+            fa[i * qs + cell] = sv;
+            fb[i * qs + cell] = sv;
+        }
+    }
+}
+
+} // namespace lbm

@@ -1,0 +1,782 @@
+// liblbm.hip -- host side of the C ABI declared in include/lbm.h.
+//
+// Replaces the reference's main() for the hot path (MRTtest/DynInlet3.cu:
+// 2834-3335): allocation, geometry classification, initialisation, the time
+// loop of update kernels, force collection and output.  One context owns one
+// HIP stream on the current device; all launches go to that stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm.h"
+#include "kernels.h"
+#include "refine.h"
+
+using namespace lbm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Level {
+    int nx = 0, ny = 0, nz = 0, nxp = 0, nseg = 0, nyb = 0;
+    long long qs = 0;
+    size_t cells = 0;
+    int storage = LBM_STORE_F32;
+    size_t esize = 4;
+    void* fa = nullptr; // current state
+    void* fb = nullptr; // next state
+    unsigned char* mask = nullptr; // device, padded
+    std::vector<unsigned char> host_mask; // dense codes
+    int blocks = 0;
+    size_t bytes = 0;
+};
+
+} // namespace
+
+struct lbm_ctx {
+    lbm_config cfg;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    Level lv[2];
+    int nlevels = 1;
+    float* u_in = nullptr;     // inlet profile [nz*ny]
+    double* fpart = nullptr;   // force partials per block
+    double* fhist = nullptr;   // [force_steps*3]
+    double* red = nullptr;     // small reduction output
+    double* mpart = nullptr;   // mass partials
+    long long t = 0;
+    int force_recorded = 0;
+    bool geometry_set = false;
+    bool initialised = false;
+    RefineState refine;
+    std::string err;
+};
+
+static int fail(lbm_ctx* ctx, int code, const std::string& msg)
+{
+    if (ctx) ctx->err = msg;
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                  \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(ctx, LBM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static size_t elem_size(int storage)
+{
+    switch (storage) {
+    case LBM_STORE_F32: return 4;
+    case LBM_STORE_F64: return 8;
+    default: return 2;
+    }
+}
+
+extern "C" void lbm_config_default(lbm_config* c)
+{
+    std::memset(c, 0, sizeof(*c));
+    c->nx = c->ny = c->nz = 64;
+    c->bc[0] = c->bc[1] = c->bc[2] = LBM_BC_PERIODIC;
+    c->model = LBM_MODEL_BGK;
+    c->halfway = 0;
+    c->storage = LBM_STORE_F32;
+    c->omega = 1.0f;
+    c->cs = 0.0f;
+    c->dpdy = 0.0f;
+    c->umax = 0.0f;
+    c->inlet_amp = 0.0f;
+    c->seed = 0;
+    // d'Humieres 2002 rates (MRTtest/DynInlet3.cu:774-783)
+    c->s_dhum[1] = 1.19f;
+    c->s_dhum[2] = 1.4f;
+    c->s_dhum[4] = c->s_dhum[6] = c->s_dhum[8] = 1.2f;
+    c->s_dhum[10] = c->s_dhum[12] = 1.4f;
+    c->s_dhum[16] = c->s_dhum[17] = c->s_dhum[18] = 1.98f;
+    c->geometry = LBM_GEOM_NONE;
+    c->startf = 0;
+    c->force_steps = 0;
+    c->refinement = 0;
+    c->temporal_interp = 1;
+}
+
+// ---------------------------------------------------------------- helpers
+
+// splitmix64 finaliser over (seed, counter), top 32 bits.
+static uint32_t hash_u32(uint64_t seed, uint64_t counter)
+{
+    uint64_t x = seed ^ (counter * 0x9E3779B97F4A7C15ULL);
+    x += 0x632BE59BD9B4E019ULL;
+    x ^= x >> 30;
+    x *= 0xBF58476D1CE4E5B9ULL;
+    x ^= x >> 27;
+    x *= 0x94D049BB133111EBULL;
+    x ^= x >> 31;
+    return (uint32_t)(x >> 32);
+}
+
+// u_in[z*ny+y] = u0 + amp*(2r-1), r = (h>>8) 2^-24 (separate statements: no
+// contraction, so the table is bit-identical to any IEEE FP32 evaluation).
+static void inlet_profile(int ny, int nz, float u0, float amp, uint64_t seed, float* out)
+{
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y) {
+            const uint64_t cell = (uint64_t)z * (uint64_t)ny + (uint64_t)y;
+            const uint32_t h = hash_u32(seed, cell);
+            volatile float r = (float)(h >> 8) * (1.0f / 16777216.0f);
+            volatile float t = 2.0f * r - 1.0f;
+            volatile float d = amp * t;
+            out[cell] = u0 + d;
+        }
+}
+
+extern "C" uint32_t lbm_hash_u32(uint64_t seed, uint64_t counter) { return hash_u32(seed, counter); }
+
+extern "C" void lbm_inlet_profile(int ny, int nz, float u0, float amp, uint64_t seed, float* out)
+{
+    inlet_profile(ny, nz, u0, amp, seed, out);
+}
+
+static int validate(const lbm_config* c, std::string& why)
+{
+    if (c->nx < 2 || c->ny < 1 || c->nz < 1) { why = "domain must be at least 2x1x1"; return 0; }
+    if ((long long)c->nx * c->ny * c->nz > (1LL << 30)) { why = "domain exceeds 2^30 nodes"; return 0; }
+    for (int a = 0; a < 3; ++a)
+        if (c->bc[a] < 0 || c->bc[a] > 2) { why = "bc must be LBM_BC_PERIODIC/CLAMP/WALL"; return 0; }
+    if (c->model < 0 || c->model > 3) { why = "unknown model"; return 0; }
+    if (c->storage < 0 || c->storage > 3) { why = "unknown storage"; return 0; }
+    if (!(c->omega > 0.0f && c->omega < 2.0f)) { why = "omega must lie in (0, 2) (tau > 0.5)"; return 0; }
+    if (c->cs < 0.0f) { why = "CS must be >= 0"; return 0; }
+    if (c->cs > 0.0f && c->model == LBM_MODEL_MRT_CPU3D) { why = "MRT_CPU3D has no LES"; return 0; }
+    if (c->force_steps < 0 || c->startf < 0) { why = "force window must be >= 0"; return 0; }
+    if (c->geometry < 0 || c->geometry > 2) { why = "unknown geometry"; return 0; }
+    if (c->model == LBM_MODEL_MRT_DHUMIERES)
+        for (int k : {1, 2, 4, 6, 8, 10, 12, 16, 17, 18})
+            if (!(c->s_dhum[k] > 0.0f && c->s_dhum[k] < 2.0f)) { why = "d'Humieres rates must lie in (0,2)"; return 0; }
+    if (c->refinement) {
+        std::string w2;
+        if (!refine_validate(*c, w2)) { why = w2; return 0; }
+    }
+    return 1;
+}
+
+static int alloc_level(lbm_ctx* ctx, Level& L, int nx, int ny, int nz, int storage)
+{
+    L.nx = nx; L.ny = ny; L.nz = nz;
+    L.nxp = ((nx + TX - 1) / TX) * TX;
+    L.nseg = L.nxp / TX;
+    L.nyb = (ny + TY - 1) / TY;
+    L.qs = (long long)L.nxp * ny * nz;
+    L.cells = (size_t)nx * ny * nz;
+    L.storage = storage;
+    L.esize = elem_size(storage);
+    L.blocks = L.nseg * L.nyb * nz;
+    const size_t lattice = (size_t)Q * (size_t)L.qs * L.esize;
+    HIP_TRY(ctx, hipMalloc(&L.fa, lattice));
+    HIP_TRY(ctx, hipMalloc(&L.fb, lattice));
+    HIP_TRY(ctx, hipMalloc(&L.mask, (size_t)L.qs));
+    HIP_TRY(ctx, hipMemsetAsync(L.fa, 0, lattice, ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(L.fb, 0, lattice, ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(L.mask, 0, (size_t)L.qs, ctx->stream));
+    L.host_mask.assign(L.cells, 0);
+    L.bytes = 2 * lattice + (size_t)L.qs;
+    return LBM_OK;
+}
+
+static void free_level(Level& L)
+{
+    if (L.fa) (void)hipFree(L.fa);
+    if (L.fb) (void)hipFree(L.fb);
+    if (L.mask) (void)hipFree(L.mask);
+    L.fa = L.fb = nullptr;
+    L.mask = nullptr;
+}
+
+extern "C" int lbm_create(const lbm_config* cfg, lbm_ctx** out)
+{
+    if (!cfg || !out) return fail(nullptr, LBM_EINVAL, "lbm_create: null argument");
+    *out = nullptr;
+    std::string why;
+    if (!validate(cfg, why)) return fail(nullptr, LBM_EINVAL, "lbm_create: " + why);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(nullptr, LBM_EDEVICE, "lbm_create: no HIP device visible");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return fail(nullptr, LBM_EDEVICE, "lbm_create: hipGetDeviceProperties failed");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(nullptr, LBM_EDEVICE,
+                    std::string("lbm_create: liblbm is built for gfx950, device is ") + prop.gcnArchName);
+
+    lbm_ctx* ctx = new lbm_ctx();
+    ctx->cfg = *cfg;
+    ctx->device = dev;
+    int rc;
+    auto bail = [&](int code) {
+        lbm_destroy(ctx);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(nullptr, LBM_EDEVICE, "lbm_create: stream creation failed"));
+    if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess)
+        return bail(fail(nullptr, LBM_EDEVICE, "lbm_create: event creation failed"));
+    if ((rc = alloc_level(ctx, ctx->lv[0], cfg->nx, cfg->ny, cfg->nz, cfg->storage)) != LBM_OK) {
+        g_last_error = ctx->err;
+        return bail(rc);
+    }
+    Level& L = ctx->lv[0];
+    std::vector<float> prof((size_t)cfg->ny * cfg->nz);
+    inlet_profile(cfg->ny, cfg->nz, cfg->umax, cfg->inlet_amp, cfg->seed, prof.data());
+    if (hipMalloc(&ctx->u_in, prof.size() * sizeof(float)) != hipSuccess ||
+        hipMalloc(&ctx->fpart, (size_t)L.blocks * 3 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ctx->mpart, (size_t)L.blocks * 4 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ctx->red, 8 * sizeof(double)) != hipSuccess)
+        return bail(fail(nullptr, LBM_ENOMEM, "lbm_create: device allocation failed"));
+    if (cfg->force_steps > 0) {
+        if (hipMalloc(&ctx->fhist, (size_t)cfg->force_steps * 3 * sizeof(double)) != hipSuccess)
+            return bail(fail(nullptr, LBM_ENOMEM, "lbm_create: force history allocation failed"));
+        (void)hipMemsetAsync(ctx->fhist, 0, (size_t)cfg->force_steps * 3 * sizeof(double), ctx->stream);
+    }
+    (void)hipMemcpyAsync(ctx->u_in, prof.data(), prof.size() * sizeof(float), hipMemcpyHostToDevice,
+                         ctx->stream);
+    if (cfg->refinement) {
+        ctx->nlevels = 2;
+        if ((rc = alloc_level(ctx, ctx->lv[1], cfg->lr_nx, cfg->lr_ny, cfg->lr_nz, cfg->storage)) != LBM_OK) {
+            g_last_error = ctx->err;
+            return bail(rc);
+        }
+        if ((rc = refine_create(ctx->refine, *cfg, ctx->stream, ctx->err)) != LBM_OK) {
+            g_last_error = ctx->err;
+            return bail(rc);
+        }
+    }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return bail(fail(nullptr, LBM_EDEVICE, "lbm_create: setup failed on the device"));
+    *out = ctx;
+    return LBM_OK;
+}
+
+extern "C" void lbm_destroy(lbm_ctx* ctx)
+{
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (int l = 0; l < 2; ++l) free_level(ctx->lv[l]);
+    refine_destroy(ctx->refine);
+    for (void* p : {(void*)ctx->u_in, (void*)ctx->fpart, (void*)ctx->fhist, (void*)ctx->red, (void*)ctx->mpart})
+        if (p) (void)hipFree(p);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+extern "C" const char* lbm_last_error(lbm_ctx* ctx)
+{
+    return ctx ? ctx->err.c_str() : g_last_error.c_str();
+}
+
+// ---------------------------------------------------------------- geometry
+
+// ImageFcn stand-ins (see LBM_GEOM_* in lbm.h).
+static unsigned char classify(const lbm_config& c, int x, int y, int z)
+{
+    if (c.geometry == LBM_GEOM_CHANNEL) {
+        // LaminarChannelCylinder1/LRMultiGPU6.cu:90-129: obstacle first, then
+        // the face chain y walls -> x inlet/outlet -> z walls; node walls only
+        // where the axis is not a halfway wall.
+        unsigned char v = LBM_NODE_FLUID;
+        if (std::fabs((float)x - c.obst_x) < c.obst_r && std::fabs((float)y - c.obst_y) < c.obst_r)
+            v = LBM_NODE_SOLID_FORCE;
+        const bool ywall = c.bc[1] == LBM_BC_CLAMP, zwall = c.bc[2] == LBM_BC_CLAMP;
+        if (ywall && (y == 0 || y == c.ny - 1)) v = LBM_NODE_SOLID;
+        else if (x == 0) v = LBM_NODE_INLET_WEST;
+        else if (x == c.nx - 1) v = LBM_NODE_OUTLET_EAST;
+        else if (zwall && (z == 0 || z == c.nz - 1)) v = LBM_NODE_SOLID;
+        return v;
+    }
+    if (c.geometry == LBM_GEOM_CPU3D) {
+        // CPU_3D/CPU_3D.cpp:39-75 (float arithmetic as there)
+        const float dx = (float)x - c.obst_x, dy = (float)y - c.obst_y;
+        if ((dx * dx + dy * dy) < c.obst_r * c.obst_r) return LBM_NODE_SOLID_FORCE;
+        if (x == 0) return LBM_NODE_LID_WEST_REG;
+        if (x == c.nx - 1 || y == 0 || y == c.ny - 1 || z == 0 || z == c.nz - 1) return LBM_NODE_SOLID;
+        return LBM_NODE_FLUID;
+    }
+    return LBM_NODE_FLUID;
+}
+
+// Device mask for one level: codes plus the NEAR bit on nodes with a link
+// that bounces (out through a wall face, or from a solid node in link mode).
+static void build_device_mask(const Level& L, const int bc[3], int halfway, std::vector<unsigned char>& dev)
+{
+    dev.assign((size_t)L.qs, 0);
+    const int n[3] = {L.nx, L.ny, L.nz};
+    for (int z = 0; z < L.nz; ++z)
+        for (int y = 0; y < L.ny; ++y)
+            for (int x = 0; x < L.nx; ++x) {
+                const size_t c = (size_t)x + (size_t)L.nx * (y + (size_t)L.ny * z);
+                unsigned char code = L.host_mask[c];
+                bool near = false;
+                const int p[3] = {x, y, z};
+                for (int i = 1; i < Q && !near; ++i) {
+                    const int cc[3] = {CX[i], CY[i], CZ[i]};
+                    int s[3];
+                    bool out = false;
+                    for (int a = 0; a < 3; ++a) {
+                        s[a] = p[a] - cc[a];
+                        if (s[a] < 0 || s[a] >= n[a]) {
+                            if (bc[a] == LBM_BC_PERIODIC) s[a] = (s[a] + n[a]) % n[a];
+                            else if (bc[a] == LBM_BC_CLAMP) s[a] = s[a] < 0 ? 0 : n[a] - 1;
+                            else out = true;
+                        }
+                    }
+                    if (out) near = true;
+                    else if (halfway && !is_solid(code)) {
+                        const size_t sc = (size_t)s[0] + (size_t)L.nx * (s[1] + (size_t)L.ny * s[2]);
+                        if (is_solid(L.host_mask[sc])) near = true;
+                    }
+                }
+                dev[(size_t)x + (size_t)L.nxp * (y + (size_t)L.ny * z)] = code | (near ? NODE_NEAR : 0);
+            }
+}
+
+extern "C" int lbm_set_geometry(lbm_ctx* ctx, const uint8_t* mask)
+{
+    if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_set_geometry: null context");
+    Level& L = ctx->lv[0];
+    const lbm_config& c = ctx->cfg;
+    if (mask) {
+        for (size_t i = 0; i < L.cells; ++i)
+            if (mask[i] > LBM_NODE_LID_WEST_REG)
+                return fail(ctx, LBM_EINVAL, "lbm_set_geometry: unknown node code " + std::to_string(mask[i]));
+        std::memcpy(L.host_mask.data(), mask, L.cells);
+    } else {
+        for (int z = 0; z < L.nz; ++z)
+            for (int y = 0; y < L.ny; ++y)
+                for (int x = 0; x < L.nx; ++x)
+                    L.host_mask[(size_t)x + (size_t)L.nx * (y + (size_t)L.ny * z)] = classify(c, x, y, z);
+    }
+    std::vector<unsigned char> dev;
+    build_device_mask(L, c.bc, c.halfway, dev);
+    HIP_TRY(ctx, hipMemcpyAsync(L.mask, dev.data(), dev.size(), hipMemcpyHostToDevice, ctx->stream));
+    if (ctx->nlevels > 1) {
+        Level& F = ctx->lv[1];
+        refine_classify_fine(ctx->refine, c, L.host_mask, F.host_mask);
+        const int fbc[3] = {LBM_BC_CLAMP, LBM_BC_CLAMP, LBM_BC_CLAMP};
+        std::vector<unsigned char> fdev;
+        build_device_mask(F, fbc, c.halfway, fdev);
+        HIP_TRY(ctx, hipMemcpyAsync(F.mask, fdev.data(), fdev.size(), hipMemcpyHostToDevice, ctx->stream));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->geometry_set = true;
+    return LBM_OK;
+}
+
+extern "C" int lbm_get_mask(lbm_ctx* ctx, int level, uint8_t* mask)
+{
+    if (!ctx || !mask) return fail(ctx, LBM_EINVAL, "lbm_get_mask: null argument");
+    if (level < 0 || level >= ctx->nlevels) return fail(ctx, LBM_EINVAL, "lbm_get_mask: bad level");
+    const Level& L = ctx->lv[level];
+    std::memcpy(mask, L.host_mask.data(), L.cells);
+    return LBM_OK;
+}
+
+// ---------------------------------------------------------------- dispatch
+
+template <class F>
+static int with_storage(int storage, F&& fn)
+{
+    switch (storage) {
+    case LBM_STORE_F32: return fn((float*)nullptr);
+    case LBM_STORE_F64: return fn((double*)nullptr);
+    case LBM_STORE_F16: return fn((Half16*)nullptr);
+    case LBM_STORE_BF16: return fn((BHalf16*)nullptr);
+    }
+    return LBM_EINVAL;
+}
+
+static int grid_for(size_t n)
+{
+    size_t g = (n + 255) / 256;
+    return (int)std::min<size_t>(g, 65536);
+}
+
+static int init_level(lbm_ctx* ctx, int level, int kind, const float* rho, const float* u, const float* v,
+                      const float* w, float u0)
+{
+    Level& L = ctx->lv[level];
+    float* d[4] = {nullptr, nullptr, nullptr, nullptr};
+    const float* h[4] = {rho, u, v, w};
+    for (int k = 0; k < 4; ++k)
+        if (h[k]) {
+            HIP_TRY(ctx, hipMalloc(&d[k], L.cells * sizeof(float)));
+            HIP_TRY(ctx, hipMemcpyAsync(d[k], h[k], L.cells * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+        }
+    int rc = with_storage(L.storage, [&](auto* tag) {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL(k_init<S>, dim3(grid_for(L.cells)), dim3(256), 0, ctx->stream, (S*)L.fa, (S*)L.fb,
+                           L.qs, L.nx, L.ny, L.nz, L.nxp, L.mask, d[0], d[1], d[2], d[3], u0, kind);
+        return LBM_OK;
+    });
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < 4; ++k)
+        if (d[k]) (void)hipFree(d[k]);
+    return rc;
+}
+
+extern "C" int lbm_init_equilibrium(lbm_ctx* ctx, int kind, const float* rho, const float* u, const float* v,
+                                    const float* w)
+{
+    if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_init_equilibrium: null context");
+    if (kind != LBM_INIT_BGK && kind != LBM_INIT_MRT)
+        return fail(ctx, LBM_EINVAL, "lbm_init_equilibrium: kind must be LBM_INIT_BGK or LBM_INIT_MRT");
+    if (!ctx->geometry_set) {
+        int rc = lbm_set_geometry(ctx, nullptr);
+        if (rc) return rc;
+    }
+    int rc = init_level(ctx, 0, kind, rho, u, v, w, ctx->cfg.umax);
+    if (rc) return rc;
+    if (ctx->nlevels > 1) {
+        rc = init_level(ctx, 1, kind, nullptr, nullptr, nullptr, nullptr, ctx->cfg.umax);
+        if (rc) return rc;
+        rc = refine_init(ctx->refine, ctx->stream, ctx->err);
+        if (rc) return rc;
+    }
+    ctx->t = 0;
+    ctx->force_recorded = 0;
+    ctx->initialised = true;
+    return LBM_OK;
+}
+
+extern "C" int lbm_init_taylor_green(lbm_ctx* ctx, float u0)
+{
+    if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_init_taylor_green: null context");
+    if (!ctx->geometry_set) {
+        int rc = lbm_set_geometry(ctx, nullptr);
+        if (rc) return rc;
+    }
+    int rc = init_level(ctx, 0, 2, nullptr, nullptr, nullptr, nullptr, u0);
+    if (rc) return rc;
+    ctx->t = 0;
+    ctx->force_recorded = 0;
+    ctx->initialised = true;
+    return LBM_OK;
+}
+
+static StepParams step_params(lbm_ctx* ctx, const Level& L, float omega)
+{
+    const lbm_config& c = ctx->cfg;
+    StepParams p;
+    p.fin = L.fa;
+    p.fout = L.fb;
+    p.mask = L.mask;
+    p.u_in = ctx->u_in;
+    p.fpart = nullptr;
+    p.qs = L.qs;
+    p.nx = L.nx; p.ny = L.ny; p.nz = L.nz; p.nxp = L.nxp;
+    p.nseg = L.nseg; p.nyb = L.nyb;
+    p.bcx = c.bc[0]; p.bcy = c.bc[1]; p.bcz = c.bc[2];
+    p.halfway = c.halfway;
+    p.omega = omega;
+    p.cs = c.cs;
+    p.dpdy = c.dpdy;
+    p.u_lid = c.umax;
+    p.s1 = c.s_dhum[1]; p.s2 = c.s_dhum[2]; p.s4 = c.s_dhum[4]; p.s10 = c.s_dhum[10]; p.s16 = c.s_dhum[16];
+    return p;
+}
+
+template <class S>
+static void launch_sc(int model, const StepParams& p, int blocks, hipStream_t s)
+{
+    const dim3 blk(TX, TY), grd(blocks);
+    switch (model) {
+    case LBM_MODEL_BGK: hipLaunchKernelGGL((k_stream_collide<MODEL_BGK, S>), grd, blk, 0, s, p); break;
+    case LBM_MODEL_MRT_CPU3D: hipLaunchKernelGGL((k_stream_collide<MODEL_MRT_CPU3D, S>), grd, blk, 0, s, p); break;
+    case LBM_MODEL_MRT_LES: hipLaunchKernelGGL((k_stream_collide<MODEL_MRT_LES, S>), grd, blk, 0, s, p); break;
+    default: hipLaunchKernelGGL((k_stream_collide<MODEL_MRT_DHUM, S>), grd, blk, 0, s, p); break;
+    }
+}
+
+// One coarse-level stream-collide launch (update_inn/top/bot on one GPU),
+// plus the deterministic force reduction when this step is recorded.
+static int coarse_step(lbm_ctx* ctx)
+{
+    Level& L = ctx->lv[0];
+    const lbm_config& c = ctx->cfg;
+    StepParams p = step_params(ctx, L, c.omega);
+    const bool rec = c.force_steps > 0 && ctx->t >= c.startf && (ctx->t - c.startf) < c.force_steps;
+    if (rec) p.fpart = ctx->fpart;
+    with_storage(L.storage, [&](auto* tag) {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        launch_sc<S>(c.model, p, L.blocks, ctx->stream);
+        return 0;
+    });
+    if (rec) {
+        const long long slot = ctx->t - c.startf;
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, ctx->stream, ctx->fpart,
+                           (long long)L.blocks, 3, ctx->fhist + slot * 3);
+        ctx->force_recorded = (int)std::max<long long>(ctx->force_recorded, slot + 1);
+    }
+    std::swap(L.fa, L.fb);
+    return LBM_OK;
+}
+
+// Fine-level sub-step launched by the refinement driver.
+int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f)
+{
+    Level& F = ctx->lv[1];
+    const lbm_config& c = ctx->cfg;
+    StepParams p = step_params(ctx, F, omega_f);
+    p.bcx = p.bcy = p.bcz = LBM_BC_CLAMP;
+    with_storage(F.storage, [&](auto* tag) {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        launch_sc<S>(c.model, p, F.blocks, ctx->stream);
+        return 0;
+    });
+    std::swap(F.fa, F.fb);
+    return LBM_OK;
+}
+
+extern "C" int lbm_step(lbm_ctx* ctx, int nsteps)
+{
+    if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_step: null context");
+    if (!ctx->initialised) return fail(ctx, LBM_ESTATE, "lbm_step: call lbm_init_equilibrium first");
+    if (nsteps < 0) return fail(ctx, LBM_EINVAL, "lbm_step: nsteps < 0");
+    for (int s = 0; s < nsteps; ++s) {
+        if (ctx->nlevels > 1) {
+            RefineLevels rl{&ctx->lv[0].fa, &ctx->lv[0].fb, ctx->lv[0].mask, ctx->lv[0].qs, ctx->lv[0].nxp,
+                            &ctx->lv[1].fa, &ctx->lv[1].fb, ctx->lv[1].mask, ctx->lv[1].qs, ctx->lv[1].nxp};
+            int rc = refine_step(ctx->refine, ctx->cfg, rl, ctx->stream,
+                                 [&]() { return coarse_step(ctx); },
+                                 [&](float om) { return lbm_internal_fine_step(ctx, om); }, ctx->err);
+            if (rc) return rc;
+        } else {
+            coarse_step(ctx);
+        }
+        ctx->t += 1;
+    }
+    HIP_TRY(ctx, hipGetLastError());
+    return LBM_OK;
+}
+
+extern "C" int lbm_step_timed(lbm_ctx* ctx, int nsteps, float* elapsed_ms)
+{
+    if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_step_timed: null context");
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    int rc = lbm_step(ctx, nsteps);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (elapsed_ms) *elapsed_ms = ms;
+    return LBM_OK;
+}
+
+// ---------------------------------------------------------------- readback
+
+extern "C" int lbm_get_moments(lbm_ctx* ctx, int level, double* rho, double* u, double* v, double* w)
+{
+    if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_get_moments: null context");
+    if (level < 0 || level >= ctx->nlevels) return fail(ctx, LBM_EINVAL, "lbm_get_moments: bad level");
+    Level& L = ctx->lv[level];
+    double* d = nullptr;
+    HIP_TRY(ctx, hipMalloc(&d, 4 * L.cells * sizeof(double)));
+    with_storage(L.storage, [&](auto* tag) {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL(k_macro<S>, dim3(grid_for(L.cells)), dim3(256), 0, ctx->stream, (const S*)L.fa, L.qs,
+                           L.nx, L.ny, L.nz, L.nxp, d, d + L.cells, d + 2 * L.cells, d + 3 * L.cells);
+        return 0;
+    });
+    double* out[4] = {rho, u, v, w};
+    hipError_t e = hipGetLastError();
+    for (int k = 0; k < 4 && e == hipSuccess; ++k)
+        if (out[k])
+            e = hipMemcpyAsync(out[k], d + k * L.cells, L.cells * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(ctx, LBM_EDEVICE, std::string("lbm_get_moments: ") + hipGetErrorString(e));
+    return LBM_OK;
+}
+
+template <class T>
+static int get_pops(lbm_ctx* ctx, int level, T* f, const char* who)
+{
+    if (!ctx || !f) return fail(ctx, LBM_EINVAL, std::string(who) + ": null argument");
+    if (level < 0 || level >= ctx->nlevels) return fail(ctx, LBM_EINVAL, std::string(who) + ": bad level");
+    Level& L = ctx->lv[level];
+    T* d = nullptr;
+    const size_t n = (size_t)Q * L.cells;
+    HIP_TRY(ctx, hipMalloc(&d, n * sizeof(T)));
+    with_storage(L.storage, [&](auto* tag) {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL((k_pack<S, T>), dim3(grid_for(n)), dim3(256), 0, ctx->stream, (const S*)L.fa, L.qs,
+                           L.nx, L.ny, L.nz, L.nxp, d);
+        return 0;
+    });
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(f, d, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(ctx, LBM_EDEVICE, std::string(who) + ": " + hipGetErrorString(e));
+    return LBM_OK;
+}
+
+template <class T>
+static int set_pops(lbm_ctx* ctx, int level, const T* f, const char* who)
+{
+    if (!ctx || !f) return fail(ctx, LBM_EINVAL, std::string(who) + ": null argument");
+    if (level < 0 || level >= ctx->nlevels) return fail(ctx, LBM_EINVAL, std::string(who) + ": bad level");
+    if (!ctx->geometry_set) {
+        int rc = lbm_set_geometry(ctx, nullptr);
+        if (rc) return rc;
+    }
+    Level& L = ctx->lv[level];
+    T* d = nullptr;
+    const size_t n = (size_t)Q * L.cells;
+    HIP_TRY(ctx, hipMalloc(&d, n * sizeof(T)));
+    hipError_t e = hipMemcpyAsync(d, f, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        with_storage(L.storage, [&](auto* tag) {
+            using S = std::remove_pointer_t<decltype(tag)>;
+            hipLaunchKernelGGL((k_unpack<S, T>), dim3(grid_for(n)), dim3(256), 0, ctx->stream, d, L.qs, L.nx,
+                               L.ny, L.nz, L.nxp, (S*)L.fa, (S*)L.fb);
+            return 0;
+        });
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(ctx, LBM_EDEVICE, std::string(who) + ": " + hipGetErrorString(e));
+    if (level == 0) {
+        ctx->initialised = true;
+        if (ctx->nlevels > 1) {
+            int rc = refine_init(ctx->refine, ctx->stream, ctx->err);
+            if (rc) return rc;
+        }
+    }
+    return LBM_OK;
+}
+
+extern "C" int lbm_get_populations(lbm_ctx* ctx, int level, float* f)
+{
+    return get_pops<float>(ctx, level, f, "lbm_get_populations");
+}
+extern "C" int lbm_get_populations_f64(lbm_ctx* ctx, int level, double* f)
+{
+    return get_pops<double>(ctx, level, f, "lbm_get_populations_f64");
+}
+extern "C" int lbm_set_populations(lbm_ctx* ctx, int level, const float* f)
+{
+    return set_pops<float>(ctx, level, f, "lbm_set_populations");
+}
+extern "C" int lbm_set_populations_f64(lbm_ctx* ctx, int level, const double* f)
+{
+    return set_pops<double>(ctx, level, f, "lbm_set_populations_f64");
+}
+
+extern "C" int lbm_get_forces(lbm_ctx* ctx, double* fxyz, int max_steps, int* nsteps)
+{
+    if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_get_forces: null context");
+    const int n = std::min(ctx->force_recorded, std::max(max_steps, 0));
+    if (nsteps) *nsteps = ctx->force_recorded;
+    if (n > 0 && fxyz) {
+        HIP_TRY(ctx, hipMemcpyAsync(fxyz, ctx->fhist, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return LBM_OK;
+}
+
+extern "C" int lbm_get_mass_momentum(lbm_ctx* ctx, int level, double out[4])
+{
+    if (!ctx || !out) return fail(ctx, LBM_EINVAL, "lbm_get_mass_momentum: null argument");
+    if (level < 0 || level >= ctx->nlevels) return fail(ctx, LBM_EINVAL, "lbm_get_mass_momentum: bad level");
+    Level& L = ctx->lv[level];
+    double* part = ctx->mpart;
+    double* tmp = nullptr;
+    if (level > 0) {
+        HIP_TRY(ctx, hipMalloc(&tmp, (size_t)L.blocks * 4 * sizeof(double)));
+        part = tmp;
+    }
+    with_storage(L.storage, [&](auto* tag) {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL(k_mass_momentum<S>, dim3(L.blocks), dim3(TX, TY), 0, ctx->stream, (const S*)L.fa,
+                           L.mask, L.qs, L.nx, L.ny, L.nxp, L.nseg, L.nyb, part);
+        return 0;
+    });
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, ctx->stream, part, (long long)L.blocks, 4,
+                       ctx->red);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, ctx->red, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (tmp) (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(ctx, LBM_EDEVICE, std::string("lbm_get_mass_momentum: ") + hipGetErrorString(e));
+    return LBM_OK;
+}
+
+extern "C" int lbm_query_sizes(lbm_ctx* ctx, int level, lbm_sizes* out)
+{
+    if (!ctx || !out) return fail(ctx, LBM_EINVAL, "lbm_query_sizes: null argument");
+    if (level < 0 || level >= ctx->nlevels) return fail(ctx, LBM_EINVAL, "lbm_query_sizes: bad level");
+    const Level& L = ctx->lv[level];
+    out->nx = L.nx; out->ny = L.ny; out->nz = L.nz;
+    out->nx_pad = L.nxp;
+    out->cells = L.cells;
+    out->device_bytes = L.bytes;
+    out->force_steps = ctx->force_recorded;
+    return LBM_OK;
+}
+
+extern "C" int lbm_get_time(lbm_ctx* ctx, long long* t)
+{
+    if (!ctx || !t) return fail(ctx, LBM_EINVAL, "lbm_get_time: null argument");
+    *t = ctx->t;
+    return LBM_OK;
+}
+
+// Tecplot POINT zone as CPU_3D writes output.dat (CPU_3D/CPU_3D.cpp:1590-1614)
+// and WriteResults its first seven columns (MRTtest/DynInlet3.cu:2604-2620).
+extern "C" int lbm_write_tecplot(lbm_ctx* ctx, const char* casename)
+{
+    if (!ctx || !casename) return fail(ctx, LBM_EINVAL, "lbm_write_tecplot: null argument");
+    for (int level = 0; level < ctx->nlevels; ++level) {
+        const Level& L = ctx->lv[level];
+        std::vector<double> r(L.cells), u(L.cells), v(L.cells), w(L.cells);
+        int rc = lbm_get_moments(ctx, level, r.data(), u.data(), v.data(), w.data());
+        if (rc) return rc;
+        std::string name = std::string(casename) + (level == 0 ? ".dat" : "_LR.dat");
+        std::ofstream out(name);
+        if (!out) return fail(ctx, LBM_EIO, "lbm_write_tecplot: cannot open " + name);
+        out << "VARIABLES = \"X\",\"Y\",\"Z\",\"u\",\"v\",\"w\",\"rho\"\n";
+        out << "ZONE F=POINT, I=" << L.nx << ", J=" << L.ny << ", K=" << L.nz << "\n";
+        for (int z = 0; z < L.nz; ++z)
+            for (int y = 0; y < L.ny; ++y)
+                for (int x = 0; x < L.nx; ++x) {
+                    const size_t c = (size_t)x + (size_t)L.nx * (y + (size_t)L.ny * z);
+                    if (level == 0) out << x << ", " << y << ", " << z << ", ";
+                    else {
+                        float X0, Y0, Z0;
+                        refine_fine_coords(ctx->cfg, x, y, z, X0, Y0, Z0);
+                        out << X0 << ", " << Y0 << ", " << Z0 << ", ";
+                    }
+                    out << (float)u[c] << "," << (float)v[c] << "," << (float)w[c] << "," << (float)r[c] << "\n";
+                }
+        if (!out) return fail(ctx, LBM_EIO, "lbm_write_tecplot: write failed for " + name);
+    }
+    return LBM_OK;
+}

@@ -1,0 +1,121 @@
+// storage.h -- how populations sit in HBM.
+//
+// Layout: SoA, one plane per direction, plane q holds f_q[z][y][x] with the
+// x stride padded to a multiple of 64 elements so every 64-lane wavefront
+// row access starts on a 256-byte boundary (FP32).  Four element types:
+//   float      FP32 populations, FP32 arithmetic (reference precision)
+//   double     FP64 populations and arithmetic (the FP64 yardstick)
+//   Half16     f_i - w_i as IEEE binary16, FP32 arithmetic
+//   BHalf16    f_i - w_i as bfloat16 (upper half of an FP32), FP32 arithmetic
+// The 16-bit types store the shifted deviation: the lattice weight is added
+// back in FP32 after the load and subtracted before the store.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "d3q19.h"
+
+namespace lbm {
+
+struct Half16 { uint16_t bits; };
+struct BHalf16 { uint16_t bits; };
+
+template <class S> struct Arith { using R = float; };
+template <> struct Arith<double> { using R = double; };
+
+template <class S>
+struct Store;
+
+template <>
+struct Store<float> {
+    __device__ static __forceinline__ float load(const float* p, int) { return *p; }
+    __device__ static __forceinline__ void store(float* p, float v, int) { __builtin_nontemporal_store(v, p); }
+    __host__ __device__ static float to_real(float v, int) { return v; }
+    __host__ __device__ static float from_real(float v, int) { return v; }
+};
+
+template <>
+struct Store<double> {
+    __device__ static __forceinline__ double load(const double* p, int) { return *p; }
+    __device__ static __forceinline__ void store(double* p, double v, int) { __builtin_nontemporal_store(v, p); }
+    __host__ __device__ static double to_real(double v, int) { return v; }
+    __host__ __device__ static double from_real(double v, int) { return v; }
+};
+
+__host__ __device__ inline float half_bits_to_float(uint16_t h)
+{
+    __half x;
+    __builtin_memcpy(&x, &h, 2);
+    return __half2float(x);
+}
+
+__host__ __device__ inline uint16_t float_to_half_bits(float v)
+{
+    __half x = __float2half_rn(v);
+    uint16_t h;
+    __builtin_memcpy(&h, &x, 2);
+    return h;
+}
+
+__host__ __device__ inline float bf16_bits_to_float(uint16_t h)
+{
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+
+// round to nearest even (inputs here are finite)
+__host__ __device__ inline uint16_t float_to_bf16_bits(float v)
+{
+    uint32_t u;
+    __builtin_memcpy(&u, &v, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+template <>
+struct Store<Half16> {
+    __device__ static __forceinline__ float load(const Half16* p, int I)
+    {
+        const uint16_t h = p->bits;
+        return half_bits_to_float(h) + float(WNUM[I]) * (1.0f / 36.0f);
+    }
+    __device__ static __forceinline__ void store(Half16* p, float v, int I)
+    {
+        __builtin_nontemporal_store(float_to_half_bits(v - float(WNUM[I]) * (1.0f / 36.0f)), &p->bits);
+    }
+    __host__ __device__ static float to_real(Half16 v, int i)
+    {
+        return half_bits_to_float(v.bits) + float(WNUM[i]) * (1.0f / 36.0f);
+    }
+    __host__ __device__ static Half16 from_real(float v, int i)
+    {
+        return Half16{float_to_half_bits(v - float(WNUM[i]) * (1.0f / 36.0f))};
+    }
+};
+
+template <>
+struct Store<BHalf16> {
+    __device__ static __forceinline__ float load(const BHalf16* p, int I)
+    {
+        const uint16_t h = p->bits;
+        return bf16_bits_to_float(h) + float(WNUM[I]) * (1.0f / 36.0f);
+    }
+    __device__ static __forceinline__ void store(BHalf16* p, float v, int I)
+    {
+        __builtin_nontemporal_store(float_to_bf16_bits(v - float(WNUM[I]) * (1.0f / 36.0f)), &p->bits);
+    }
+    __host__ __device__ static float to_real(BHalf16 v, int i)
+    {
+        return bf16_bits_to_float(v.bits) + float(WNUM[i]) * (1.0f / 36.0f);
+    }
+    __host__ __device__ static BHalf16 from_real(float v, int i)
+    {
+        return BHalf16{float_to_bf16_bits(v - float(WNUM[i]) * (1.0f / 36.0f))};
+    }
+};
+
+} // namespace lbm

@@ -1,0 +1,106 @@
+/*
+ * oracle.h -- CPU restatement of the reference's D3Q19 stream-collide path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product (liblbm.so) never links or calls it.
+ *
+ * Every function restates an algorithm of the reference; the file:line it
+ * follows is cited at the definition in oracle_impl.h.  Two precisions are
+ * built from the same source: *_f32 (float arithmetic, like the reference)
+ * and *_f64 (double arithmetic, the FP64 yardstick).
+ *
+ * Layout: populations are dense SoA, f[q][z][y][x] (x fastest), 19 planes of
+ * nx*ny*nz values each.  Masks are uint8 node codes in [z][y][x] order.
+ */
+#ifndef LBM_ORACLE_H
+#define LBM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* collision operators */
+enum {
+    ORC_BGK = 0,        /* CPU_3D/CPU_3D.cpp:116-160 (also MRTtest/DynInlet3.cu:802-829) */
+    ORC_MRT_CPU3D = 1,  /* CPU_3D/CPU_3D.cpp:164-279: m13,m14 relaxed at rate 1 */
+    ORC_MRT_LES = 2,    /* MRTtest/DynInlet3.cu:599-832: m9,m11,m13,m14,m15 at omega */
+    ORC_MRT_DHUM = 3    /* MRTtest/DynInlet3.cu:774-783 d'Humieres 2002 rates */
+};
+
+/* per-axis treatment of pulls that leave the domain */
+enum {
+    ORC_PERIODIC = 0,   /* wrap (f_mem y-wrap, MRTtest/DynInlet3.cu:177-185) */
+    ORC_CLAMP = 1,      /* clamp to edge (dmin/dmax, CPU_3D/CPU_3D.cpp:104-113) */
+    ORC_WALL = 2        /* halfway (link) bounce-back at the face */
+};
+
+/* node codes (uint8) and the reference `im` code each one stands for */
+enum {
+    ORC_FLUID = 0,          /* im 0 */
+    ORC_SOLID = 1,          /* im 1: bounce-back wall */
+    ORC_SOLID_FORCE = 2,    /* im 10: bounce-back wall, momentum exchange summed */
+    ORC_INLET_WEST = 3,     /* im 400: West_Extrap velocity inlet */
+    ORC_OUTLET_EAST = 4,    /* im 300: East_Extrap rho=1 outlet */
+    ORC_LID_WEST_REG = 5    /* CPU_3D im 53: DirichletWest_Regularized */
+};
+
+typedef struct {
+    int nx, ny, nz;
+    int bc[3];          /* ORC_PERIODIC / ORC_CLAMP / ORC_WALL for x, y, z */
+    int collision;      /* ORC_BGK ... */
+    int halfway;        /* 0: node (full-way) bounce-back of the reference;
+                           1: link-wise halfway bounce-back */
+    double omega;       /* base rate 1/tau0 (s_nu) */
+    double cs;          /* Smagorinsky constant as the reference's CS (= Cs^2); 0 = off */
+    double dpdy;        /* AddForce pressure gradient (LES-family operators only) */
+    double u_lid;       /* UMAX used by DirichletWest_Regularized */
+    double s_dhum[19];  /* rates for ORC_MRT_DHUM (entries 9,11,13,14,15 ignored: omega) */
+} orc_params;
+
+void orc_default_params(orc_params* p);
+
+/* D3Q19 tables and the moment matrix built from its defining polynomials */
+void orc_tables(int c[19][3], double w[19], int opp[19]);
+void orc_moment_matrix(double M[19][19]);
+
+/* ---- float restatement ---- */
+void orc_moments_f32(const float f[19], float m[19]);
+void orc_collide_f32(const orc_params* p, float f[19]);
+void orc_les_omega_f32(const orc_params* p, const float f[19], float* omega_out, float* q_out);
+void orc_west_extrap_f32(float f[19], float u);
+void orc_east_extrap_f32(float f[19], float rho);
+void orc_lid_west_reg_f32(float f[19], float u_lid);
+void orc_feq_bgk_f32(float rho, float u, float v, float w, float f[19]);
+void orc_feq_mrt_f32(float rho, float u, float v, float w, float f[19]);
+void orc_step_f32(const orc_params* p, const uint8_t* mask, const float* u_in,
+                  const float* fin, float* fout, double force[3]);
+void orc_run_f32(const orc_params* p, const uint8_t* mask, const float* u_in,
+                 float* fa, float* fb, int nsteps, double* forces);
+void orc_macro_f32(const orc_params* p, const float* f, float* rho, float* u, float* v, float* w);
+
+/* ---- double restatement ---- */
+void orc_moments_f64(const double f[19], double m[19]);
+void orc_collide_f64(const orc_params* p, double f[19]);
+void orc_les_omega_f64(const orc_params* p, const double f[19], double* omega_out, double* q_out);
+void orc_west_extrap_f64(double f[19], double u);
+void orc_east_extrap_f64(double f[19], double rho);
+void orc_lid_west_reg_f64(double f[19], double u_lid);
+void orc_feq_bgk_f64(double rho, double u, double v, double w, double f[19]);
+void orc_feq_mrt_f64(double rho, double u, double v, double w, double f[19]);
+void orc_step_f64(const orc_params* p, const uint8_t* mask, const double* u_in,
+                  const double* fin, double* fout, double force[3]);
+void orc_run_f64(const orc_params* p, const uint8_t* mask, const double* u_in,
+                 double* fa, double* fb, int nsteps, double* forces);
+void orc_macro_f64(const orc_params* p, const double* f, double* rho, double* u, double* v, double* w);
+
+/* seeded inlet perturbation: counter-based hash of (seed, cell), no state */
+uint32_t orc_hash_u32(uint64_t seed, uint64_t counter);
+void orc_inlet_profile(int ny, int nz, float u0, float amp, uint64_t seed, float* u_in);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

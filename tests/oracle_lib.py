@@ -1,0 +1,158 @@
+"""ctypes bindings of the oracle (oracle/liboracle.so) and of the reference's
+own CPU solver built into oracle/_ref/libcpu3d_ref.so.
+
+Test infrastructure: these are the checkers, never the thing measured.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libcpu3d_ref.so")
+REF_EXE = os.path.join(ROOT, "oracle", "_ref", "cpu3d_ref")
+
+BGK, MRT_CPU3D, MRT_LES, MRT_DHUM = 0, 1, 2, 3
+PERIODIC, CLAMP, WALL = 0, 1, 2
+FLUID, SOLID, SOLID_FORCE, INLET_WEST, OUTLET_EAST, LID_WEST_REG = 0, 1, 2, 3, 4, 5
+
+
+class OrcParams(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("bc", C.c_int * 3), ("collision", C.c_int), ("halfway", C.c_int),
+                ("omega", C.c_double), ("cs", C.c_double), ("dpdy", C.c_double),
+                ("u_lid", C.c_double), ("s_dhum", C.c_double * 19)]
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = C.CDLL(ORACLE_SO)
+        f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+        f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+        for suf, fp in (("f32", f32p), ("f64", f64p)):
+            getattr(_lib, "orc_collide_" + suf).argtypes = [C.POINTER(OrcParams), fp]
+            getattr(_lib, "orc_moments_" + suf).argtypes = [fp, fp]
+            getattr(_lib, "orc_step_" + suf).argtypes = [C.POINTER(OrcParams), C.c_void_p, C.c_void_p, fp, fp, C.c_void_p]
+            getattr(_lib, "orc_run_" + suf).argtypes = [C.POINTER(OrcParams), C.c_void_p, C.c_void_p, fp, fp, C.c_int, C.c_void_p]
+            getattr(_lib, "orc_macro_" + suf).argtypes = [C.POINTER(OrcParams), fp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+            getattr(_lib, "orc_west_extrap_" + suf).argtypes = [fp, C.c_float if suf == "f32" else C.c_double]
+            getattr(_lib, "orc_east_extrap_" + suf).argtypes = [fp, C.c_float if suf == "f32" else C.c_double]
+            getattr(_lib, "orc_lid_west_reg_" + suf).argtypes = [fp, C.c_float if suf == "f32" else C.c_double]
+            sc = C.c_float if suf == "f32" else C.c_double
+            getattr(_lib, "orc_feq_bgk_" + suf).argtypes = [sc, sc, sc, sc, fp]
+            getattr(_lib, "orc_feq_mrt_" + suf).argtypes = [sc, sc, sc, sc, fp]
+            getattr(_lib, "orc_les_omega_" + suf).argtypes = [C.POINTER(OrcParams), fp, C.POINTER(sc), C.POINTER(sc)]
+        _lib.orc_default_params.argtypes = [C.POINTER(OrcParams)]
+        _lib.orc_moment_matrix.argtypes = [f64p]
+        _lib.orc_hash_u32.argtypes = [C.c_uint64, C.c_uint64]
+        _lib.orc_hash_u32.restype = C.c_uint32
+        _lib.orc_inlet_profile.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, f32p]
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = C.CDLL(REF_SO)
+        f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+        _ref.ref_bgk_collide.argtypes = [f32p, C.c_float]
+        _ref.ref_mrt_collide.argtypes = [f32p, C.c_float]
+        _ref.ref_boundaries.argtypes = [f32p, C.c_int, C.c_int, C.c_int]
+        _ref.ref_image.argtypes = [C.c_int, C.c_int, C.c_int]
+        _ref.ref_case.argtypes = [np.ctypeslib.ndpointer(np.int32), f32p]
+    return _ref
+
+
+def params(nx=1, ny=1, nz=1, bc=(PERIODIC,) * 3, collision=BGK, halfway=0, omega=1.0,
+           cs=0.0, dpdy=0.0, u_lid=0.0):
+    p = OrcParams()
+    lib().orc_default_params(C.byref(p))
+    p.nx, p.ny, p.nz = nx, ny, nz
+    for a in range(3):
+        p.bc[a] = bc[a]
+    p.collision, p.halfway = collision, halfway
+    p.omega, p.cs, p.dpdy, p.u_lid = omega, cs, dpdy, u_lid
+    return p
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def collide(p, f):
+    f = np.ascontiguousarray(f).copy()
+    suf = "f32" if f.dtype == np.float32 else "f64"
+    getattr(lib(), "orc_collide_" + suf)(C.byref(p), f)
+    return f
+
+
+def moments(f):
+    f = np.ascontiguousarray(f)
+    m = np.zeros(19, f.dtype)
+    suf = "f32" if f.dtype == np.float32 else "f64"
+    getattr(lib(), "orc_moments_" + suf)(f, m)
+    return m
+
+
+def step(p, f, mask=None, u_in=None):
+    f = np.ascontiguousarray(f)
+    out = np.empty_like(f)
+    force = np.zeros(3, np.float64)
+    suf = "f32" if f.dtype == np.float32 else "f64"
+    getattr(lib(), "orc_step_" + suf)(C.byref(p), _ptr(mask), _ptr(u_in), f, out, force.ctypes.data)
+    return out, force
+
+
+def run(p, f, nsteps, mask=None, u_in=None, forces=False):
+    """Advance f (shape (19, nz, ny, nx)) nsteps; returns (f, forces[nsteps,3])."""
+    fa = np.ascontiguousarray(f).copy()
+    fb = np.empty_like(fa)
+    fr = np.zeros((max(nsteps, 1), 3), np.float64) if forces else None
+    suf = "f32" if fa.dtype == np.float32 else "f64"
+    getattr(lib(), "orc_run_" + suf)(C.byref(p), _ptr(mask), _ptr(u_in), fa, fb, nsteps, _ptr(fr))
+    return fa, fr
+
+
+def macro(p, f):
+    n = p.nx * p.ny * p.nz
+    out = [np.empty(n, f.dtype) for _ in range(4)]
+    suf = "f32" if f.dtype == np.float32 else "f64"
+    getattr(lib(), "orc_macro_" + suf)(C.byref(p), np.ascontiguousarray(f), *[o.ctypes.data for o in out])
+    return [o.reshape(p.nz, p.ny, p.nx) for o in out]
+
+
+def feq_bgk(rho, u, v, w, dtype=np.float32):
+    f = np.zeros(19, dtype)
+    suf = "f32" if dtype == np.float32 else "f64"
+    getattr(lib(), "orc_feq_bgk_" + suf)(rho, u, v, w, f)
+    return f
+
+
+def feq_mrt(rho, u, v, w, dtype=np.float32):
+    f = np.zeros(19, dtype)
+    suf = "f32" if dtype == np.float32 else "f64"
+    getattr(lib(), "orc_feq_mrt_" + suf)(rho, u, v, w, f)
+    return f
+
+
+def moment_matrix():
+    M = np.zeros((19, 19), np.float64)
+    lib().orc_moment_matrix(M)
+    return M
+
+
+def inlet_profile(ny, nz, u0, amp, seed):
+    out = np.empty(ny * nz, np.float32)
+    lib().orc_inlet_profile(ny, nz, u0, amp, seed, out)
+    return out

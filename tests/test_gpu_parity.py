@@ -1,0 +1,241 @@
+"""HIP path (liblbm.so through its C ABI) against the oracle on the same
+inputs, against the reference program's golden output, and against
+size-independent properties at full size.
+
+Tolerances: FP32 runs agree with the FP32 oracle within 1e-5 relative to the
+field's scale (rho, u) after up to 1000 steps; single-step checks within
+2e-6; FP64 storage within 1e-12.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import lbm_multigpu_amd as L
+import oracle_lib as O
+from lbm_multigpu_amd import config as cfgmod
+from test_oracle import cpu3d_mask
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+MODEL_TO_ORC = {L.MODEL_BGK: O.BGK, L.MODEL_MRT_CPU3D: O.MRT_CPU3D, L.MODEL_MRT_LES: O.MRT_LES,
+                L.MODEL_MRT_DHUMIERES: O.MRT_DHUM}
+NODE_TO_ORC = {L.NODE_FLUID: O.FLUID, L.NODE_SOLID: O.SOLID, L.NODE_SOLID_FORCE: O.SOLID_FORCE,
+               L.NODE_INLET_WEST: O.INLET_WEST, L.NODE_OUTLET_EAST: O.OUTLET_EAST,
+               L.NODE_LID_WEST_REG: O.LID_WEST_REG}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def orc_params(c):
+    p = O.params(c.nx, c.ny, c.nz, bc=tuple(c.bc), collision=MODEL_TO_ORC[c.model], halfway=c.halfway,
+                 omega=float(np.float32(c.omega)), cs=float(np.float32(c.cs)), dpdy=float(np.float32(c.dpdy)),
+                 u_lid=float(np.float32(c.umax)))
+    for k in range(19):
+        p.s_dhum[k] = float(np.float32(c.s_dhum[k]))
+    return p
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+
+
+def random_state(rng, shape, amp=1e-3, umax=0.05, dtype=np.float32):
+    nz, ny, nx = shape
+    u = rng.uniform(-umax, umax, (3, nz, ny, nx))
+    f = np.empty((19, nz, ny, nx), np.float64)
+    C = np.array([(0, 0, 0), (1, 0, 0), (0, 1, 0), (-1, 0, 0), (0, -1, 0), (1, 1, 0), (-1, 1, 0), (-1, -1, 0),
+                  (1, -1, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1), (-1, 0, 1), (0, -1, 1), (0, 0, -1), (1, 0, -1),
+                  (0, 1, -1), (-1, 0, -1), (0, -1, -1)])
+    W = np.array([1 / 3] + [1 / 18] * 4 + [1 / 36] * 4 + [1 / 18] + [1 / 36] * 4 + [1 / 18] + [1 / 36] * 4)
+    usq = (u ** 2).sum(0)
+    for i in range(19):
+        cu = C[i, 0] * u[0] + C[i, 1] * u[1] + C[i, 2] * u[2]
+        f[i] = W[i] * (1.0 + 3 * cu + 4.5 * cu * cu - 1.5 * usq)
+    f += rng.uniform(-amp, amp, f.shape) * W[:, None, None, None]
+    return f.astype(dtype)
+
+
+@pytest.mark.parametrize("model", [L.MODEL_BGK, L.MODEL_MRT_CPU3D, L.MODEL_MRT_LES, L.MODEL_MRT_DHUMIERES])
+@pytest.mark.parametrize("les", [0.0, 0.01])
+def test_single_step_each_model(model, les):
+    if model == L.MODEL_MRT_CPU3D and les:
+        pytest.skip("no LES in the CPU_3D operator")
+    say("single step model", model, "cs", les)
+    c = L.default_config()
+    c.nx, c.ny, c.nz = 70, 9, 7   # nx not a multiple of 64: exercises the padded rows
+    c.model, c.omega, c.cs, c.dpdy = model, 1.7, les, (1e-5 if model != L.MODEL_MRT_CPU3D else 0.0)
+    rng = np.random.default_rng(model * 10 + int(les * 100))
+    f = random_state(rng, (c.nz, c.ny, c.nx))
+    with L.Solver(c) as s:
+        s.set_populations(f)
+        s.step(1)
+        g = s.populations()
+    ref, _ = O.step(orc_params(c), f)
+    assert np.abs(g - ref).max() < 2e-6 * np.abs(ref).max()
+
+
+def test_taylor_green_1000_steps_matches_oracle():
+    """32^3 periodic BGK tau=0.6 Taylor-Green, 1000 steps, GPU vs FP32 oracle."""
+    n, tau, u0, T = 32, 0.6, 0.05, 1000
+    say("TGV parity", n, T)
+    c = L.default_config()
+    c.nx = c.ny = c.nz = n
+    c.omega = 1 / tau
+    with L.Solver(c) as s:
+        s.init_taylor_green(u0)
+        f0 = s.populations()
+        s.step(T)
+        rho, u, v, w = s.moments()
+    fo, _ = O.run(orc_params(c), f0, T)
+    r2, u2, v2, w2 = O.macro(orc_params(c), fo)
+    assert rel(rho, r2) < 1e-5 and rel(u, u2) < 1e-5 and rel(v, v2) < 1e-5
+    assert np.abs(w).max() < 1e-6
+
+
+def test_taylor_green_64_analytic_decay():
+    """The 64^3 Taylor-Green workload (BGK tau=0.6, u0=0.05, k=2pi/64, 1000
+    steps): kinetic energy follows exp(-4 nu k^2 t) to within 1%."""
+    n, tau, u0, T = 64, 0.6, 0.05, 1000
+    say("TGV decay", n, T)
+    c = L.default_config()
+    c.nx = c.ny = c.nz = n
+    c.omega = 1 / tau
+    with L.Solver(c) as s:
+        s.init_taylor_green(u0)
+        _, u, v, _ = s.moments()
+        ke0 = (u ** 2 + v ** 2).sum()
+        s.step(T)
+        _, u, v, w = s.moments()
+        mm = s.mass_momentum()
+    k = 2 * np.pi / n
+    nu = (tau - 0.5) / 3
+    ratio = (u ** 2 + v ** 2).sum() / ke0
+    assert abs(ratio / np.exp(-4 * nu * k * k * T) - 1) < 0.01
+    assert abs(mm[0] - n ** 3) < 1e-6 * n ** 3
+
+
+def test_cpu3d_case_matches_reference_program():
+    """The reference program's own case (CPU_3D/CPU_3D.cpp, 251x43x43,
+    clamped pulls, node bounce-back, regularised moving x=0 plane, MRT with
+    m13/m14 at rate 1), 100 steps on the GPU, against the reference's
+    output.dat (golden) and the oracle; masks bit-exact."""
+    say("cpu3d case")
+    g = np.load(os.path.join(GOLD, "cpu3d_t100.npz"))
+    nx, ny, nz, tmax = (int(d) for d in g["dims"])
+    umax, omega = float(g["consts"][0]), float(g["consts"][3])
+    cc = cfgmod.cpu3d_case()
+    assert abs(np.float32(cc.omega) - np.float32(omega)) == 0
+    c = cc.to_c()
+    with L.Solver(c) as s:
+        s.set_geometry(None)
+        m = s.get_mask()
+        exp = cpu3d_mask(nx, ny, nz)
+        assert np.array_equal(np.vectorize(NODE_TO_ORC.get)(m).astype(np.uint8), exp)
+        s.init_equilibrium(L.INIT_BGK)
+        f0 = s.populations()
+        s.step(tmax)
+        rho, u, v, w = s.moments()
+    zs = g["zs"]
+    for got, ref in ((rho[zs], g["rho"]), (u[zs], g["u"]), (v[zs], g["v"])):
+        assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()
+    fo, _ = O.run(orc_params(c), f0, tmax, mask=exp)
+    r2, u2, v2, _ = O.macro(orc_params(c), fo)
+    assert rel(rho, r2) < 1e-5 and rel(u, u2) < 1e-5 and rel(v, v2) < 1e-5
+
+
+@pytest.mark.parametrize("rates", ["lrmgpu", "dhumieres"])
+def test_channel_mrt_les_matches_oracle(rates):
+    """Channel with a square cylinder, halfway walls at y/z, West_Extrap inlet
+    with the seeded perturbation, East_Extrap outlet, MRT + Smagorinsky:
+    200 steps GPU vs oracle, fields and per-step momentum-exchange forces."""
+    say("channel", rates)
+    cc = cfgmod.channel(nx=96, ny=32, nz=24, side=8, x_center=32, tau=0.52, rates=rates, force_steps=200)
+    c = cc.to_c()
+    T = 200
+    with L.Solver(c) as s:
+        s.set_geometry(None)
+        mask = np.vectorize(NODE_TO_ORC.get)(s.get_mask()).astype(np.uint8)
+        s.init_equilibrium(L.INIT_BGK)
+        f0 = s.populations()
+        s.step(T)
+        rho, u, v, w = s.moments()
+        F = s.forces()
+    u_in = O.inlet_profile(c.ny, c.nz, c.umax, c.inlet_amp, c.seed)
+    fo, Fo = O.run(orc_params(c), f0, T, mask=mask, u_in=u_in, forces=True)
+    r2, u2, v2, w2 = O.macro(orc_params(c), fo)
+    fl = mask == O.FLUID
+    assert rel(rho[fl], r2[fl]) < 1e-5 and rel(u[fl], u2[fl]) < 1e-5
+    assert rel(v[fl], v2[fl]) < 1e-5 and rel(w[fl], w2[fl]) < 1e-5
+    assert F.shape == (T, 3)
+    assert np.abs(F - Fo).max() < 1e-4 * np.abs(Fo).max()
+    assert Fo[-1, 0] > 0  # drag points downstream
+
+
+def test_halfway_closed_box_conserves_mass_fp64_sums():
+    say("closed box")
+    c = L.default_config()
+    c.nx, c.ny, c.nz = 64, 20, 12
+    c.bc[0] = c.bc[1] = c.bc[2] = L.BC_WALL
+    c.model, c.halfway, c.omega, c.cs = L.MODEL_MRT_LES, 1, 1.8, 0.01
+    mask = np.zeros((c.nz, c.ny, c.nx), np.uint8)
+    mask[:, 8:12, 20:24] = L.NODE_SOLID_FORCE
+    rng = np.random.default_rng(11)
+    f = random_state(rng, (c.nz, c.ny, c.nx), amp=1e-2, umax=0.02)
+    with L.Solver(c) as s:
+        s.set_geometry(mask)
+        s.set_populations(f)
+        m0 = s.mass_momentum()
+        s.step(500)
+        m1 = s.mass_momentum()
+    assert abs(m1[0] - m0[0]) < 2e-6 * m0[0]
+
+
+def test_fp64_storage_matches_fp64_oracle():
+    say("fp64 storage")
+    cc = cfgmod.channel(nx=64, ny=16, nz=16, side=6, x_center=20, tau=0.55, rates="dhumieres", storage="f64")
+    c = cc.to_c()
+    with L.Solver(c) as s:
+        s.set_geometry(None)
+        mask = np.vectorize(NODE_TO_ORC.get)(s.get_mask()).astype(np.uint8)
+        s.init_equilibrium(L.INIT_MRT)
+        f0 = s.populations(f64=True)
+        s.step(50)
+        g = s.populations(f64=True)
+    u_in = O.inlet_profile(c.ny, c.nz, c.umax, c.inlet_amp, c.seed).astype(np.float64)
+    fo, _ = O.run(orc_params(c), f0, 50, mask=mask, u_in=u_in)
+    fl = np.broadcast_to(mask == O.FLUID, g.shape)
+    assert np.abs(g[fl] - fo[fl]).max() < 1e-12
+
+
+@pytest.mark.parametrize("storage,tol", [("f16", 2e-4), ("bf16", 3e-3)])
+def test_16bit_storage_error_vs_fp64(storage, tol):
+    """Shifted-deviation 16-bit storage with FP32 collide vs an FP64 run of the
+    same case: velocity error relative to the field's scale."""
+    say("16-bit", storage)
+    out = {}
+    for st in ("f64", storage):
+        c = cfgmod.channel(nx=128, ny=32, nz=32, side=8, x_center=32, storage=st).to_c()
+        with L.Solver(c) as s:
+            s.set_geometry(None)
+            s.init_equilibrium(L.INIT_BGK)
+            s.step(300)
+            out[st] = s.moments()
+    for k in (1, 2):
+        assert rel(out[storage][k], out["f64"][k]) < tol
+
+
+def test_bad_mask_code_rejected():
+    c = L.default_config()
+    c.nx, c.ny, c.nz = 8, 4, 4
+    with L.Solver(c) as s:
+        bad = np.full((4, 4, 8), 9, np.uint8)
+        with pytest.raises(L.LBMError) as e:
+            s.set_geometry(bad)
+        assert e.value.code == -22
+        with pytest.raises(L.LBMError):
+            s.step(1)  # not initialised
