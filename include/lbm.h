@@ -133,9 +133,10 @@ int lbm_set_geometry(lbm_ctx* ctx, const uint8_t* mask);
 /* Mask as classified (codes LBM_NODE_*) for level 0 (coarse) or 1 (fine). */
 int lbm_get_mask(lbm_ctx* ctx, int level, uint8_t* mask);
 
-/* Replaces initialize()/initializeLR() (MRTtest/DynInlet3.cu:1186-1251).
- * rho, u, v, w: dense fields, or NULL for rho=1 and u=(umax,0,0); solids get
- * u=0.  kind = LBM_INIT_*. */
+/* Replaces initialize()/initializeLR() (MRTtest/DynInlet3.cu:1186-1251) and
+ * CPU_3D's initial fill (CPU_3D/CPU_3D.cpp:1218-1254).  rho, u, v, w: dense
+ * fields, or NULL for rho=1 and u=(umax,0,0).  kind = LBM_INIT_BGK fills
+ * every node (solids included, as CPU_3D); LBM_INIT_MRT gives solids u=0. */
 int lbm_init_equilibrium(lbm_ctx* ctx, int kind, const float* rho, const float* u,
                          const float* v, const float* w);
 

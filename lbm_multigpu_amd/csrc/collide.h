@@ -133,30 +133,36 @@ __device__ __forceinline__ void add_force(R f[Q], R dpdy)
 }
 
 // BGK collide (CPU_3D/CPU_3D.cpp:116-160; DynInlet3.cu:802-830 adds the LES
-// rate and AddForce).
+// rate and AddForce).  Evaluated as the same IEEE operation sequence as the
+// CPU restatement (index-order sums, feq = w (((rho + 3cu) + 4.5cu cu) - 1.5 u.u),
+// f - omega (f - feq), no FMA contraction), so both produce the same bits;
+// BGK is far from the VALU limit, so the unshared arithmetic costs nothing.
 template <class R>
 __device__ __forceinline__ void collide_bgk(R f[Q], const CollideParams<R>& cp)
 {
-    R s[NP], d[NP];
-    pair_split(f, s, d);
-    R rho = f[0], u = 0, v = 0, w = 0;
+#pragma clang fp contract(off)
+    R rho = 0, u = 0, v = 0, w = 0;
 #pragma unroll
-    for (int j = 0; j < NP; ++j) {
-        const int i = PP[j];
-        rho += s[j];
-        if (CX[i]) u += R(CX[i]) * d[j];
-        if (CY[i]) v += R(CY[i]) * d[j];
-        if (CZ[i]) w += R(CZ[i]) * d[j];
+    for (int i = 0; i < Q; ++i) {
+        rho += f[i];
+        u += R(CX[i]) * f[i];
+        v += R(CY[i]) * f[i];
+        w += R(CZ[i]) * f[i];
     }
     R omega = cp.omega;
     if (cp.cs > R(0)) {
+        R s[NP], d[NP];
+        pair_split(f, s, d);
         const Macro<R> m = macro_from_pairs(f[0], s, d);
         omega = les_omega(m, cp.omega, cp.cs);
     }
-    R feq[Q];
-    feq_bgk(rho, u, v, w, feq);
+    const R usqr = u * u + v * v + w * w;
 #pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] -= omega * (f[i] - feq[i]);
+    for (int i = 0; i < Q; ++i) {
+        const R cu = R(CX[i]) * u + R(CY[i]) * v + R(CZ[i]) * w;
+        const R feq = wgt<R>(i) * (rho + R(3) * cu + R(4.5) * cu * cu - R(1.5) * usqr);
+        f[i] -= omega * (f[i] - feq);
+    }
     if (cp.dpdy != R(0)) add_force(f, cp.dpdy);
 }
 

@@ -330,11 +330,12 @@ __global__ void k_unpack(const T* in, long long qs, int nx, int ny, int nz, int 
     }
 }
 
-// Equilibrium initialisation of both lattices (initialize,
-// MRTtest/DynInlet3.cu:1186-1216; CPU_3D/CPU_3D.cpp:1218-1254).
-// Fields are dense; nullptr fields mean rho = 1, u = (u0, 0, 0).  Solid
-// nodes get u = 0.  kind 0: BGK feq; kind 1: mrt_meq + InvertMoments.
-// kind 2: Taylor-Green (u0 cos kx sin ky, -u0 sin kx cos ky, 0).
+// Equilibrium initialisation of both lattices.  Fields are dense; nullptr
+// fields mean rho = 1, u = (u0, 0, 0).
+//  kind 0: BGK feq at every node, solids included (CPU_3D/CPU_3D.cpp:1218-1254)
+//  kind 1: mrt_meq + InvertMoments, solids at u = 0 (initialize,
+//          MRTtest/DynInlet3.cu:1186-1216)
+//  kind 2: Taylor-Green (u0 cos kx sin ky, -u0 sin kx cos ky, 0), BGK feq.
 template <class S>
 __global__ void k_init(S* fa, S* fb, long long qs, int nx, int ny, int nz, int nxp,
                        const unsigned char* mask, const float* rho_in, const float* u_in,
@@ -361,7 +362,7 @@ __global__ void k_init(S* fa, S* fb, long long qs, int nx, int ny, int nz, int n
             w = R(0);
             rho = R(1);
         }
-        if (mask && is_solid(mask[cell])) { u = v = w = R(0); }
+        if (kind == 1 && mask && is_solid(mask[cell])) { u = v = w = R(0); }
         R feq[Q];
         if (kind == 1) {
             // mrt_meq (MRTtest/DynInlet3.cu:329-345) then InvertMoments
