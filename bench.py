@@ -67,6 +67,38 @@ def cpu_baseline(cfg, budget_s=12.0):
                       f"{steps} steps in {dt:.1f} s"}
 
 
+def copy_calibration(nbytes=637_534_208, reps=20):
+    """Device-to-device copy of nbytes (one FP32 lattice of the workload: the
+    stream-collide kernel reads and writes this much per launch), timed with
+    HIP events: the practical HBM ceiling, and the reference kernel for
+    scaling rocprof's FETCH_SIZE/WRITE_SIZE counters."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    hip.hipEventCreate(C.byref(e0))
+    hip.hipEventCreate(C.byref(e1))
+    for _ in range(3):
+        hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), 3, None)
+    hip.hipEventRecord(e0, None)
+    for _ in range(reps):
+        hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), 3, None)
+    hip.hipEventRecord(e1, None)
+    hip.hipEventSynchronize(e1)
+    ms = C.c_float()
+    hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    per = ms.value / reps
+    gbs = 2 * nbytes / (per * 1e-3) / 1e9
+    print(json.dumps({"copy_bytes": nbytes, "copy_ms": round(per, 4), "copy_GBps_read_plus_write": round(gbs, 1),
+                      "frac_of_8TBps": round(gbs / HBM_PEAK_GBS, 4)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,7 +106,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16", "f64"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="time a plain device copy of one lattice instead (HBM ceiling / counter scale)")
     a = ap.parse_args()
+    if a.calibrate_copy:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the sharded path is a design, not a build.",
               file=sys.stderr)

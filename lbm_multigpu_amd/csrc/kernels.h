@@ -17,8 +17,19 @@ namespace lbm {
 
 enum Bc : int { BC_PERIODIC = 0, BC_CLAMP = 1, BC_WALL = 2 };
 
-constexpr int TX = 64; // threads along x per wavefront
-constexpr int TY = 4;  // wavefronts (rows along y) per workgroup
+// Tuning knobs (compile-time; the defaults are the measured best on MI355X)
+#ifndef LBM_TY
+#define LBM_TY 4 // wavefronts (rows along y) per workgroup
+#endif
+#ifndef LBM_XCD_REMAP
+#define LBM_XCD_REMAP 1
+#endif
+#ifndef LBM_MIN_BLOCKS
+#define LBM_MIN_BLOCKS 1 // __launch_bounds__ minimum workgroups per CU
+#endif
+
+constexpr int TX = 64;     // threads along x per wavefront
+constexpr int TY = LBM_TY; // wavefronts per workgroup
 
 struct StepParams {
     const void* fin;
@@ -39,6 +50,7 @@ struct StepParams {
 // the logical block range [j*G/8, (j+1)*G/8) (bijective for any G).
 __device__ __forceinline__ int logical_block()
 {
+    if (!LBM_XCD_REMAP) return blockIdx.x;
     const int G = gridDim.x, hw = blockIdx.x;
     const int per = G >> 3, rem = G & 7, xcd = hw & 7;
     return xcd * per + (xcd < rem ? xcd : rem) + (hw >> 3);
@@ -59,6 +71,20 @@ __device__ __forceinline__ void axis_nb(int c, int n, int bc, int& m, int& p, bo
     }
 }
 
+// Element idx of a direction plane: plane base (wave-uniform, SGPRs) plus a
+// 32-bit unsigned byte offset, so loads and stores use the SGPR-base +
+// VGPR-offset addressing form instead of per-lane 64-bit address math.
+template <class S>
+__device__ __forceinline__ const S* at(const S* plane, int idx)
+{
+    return reinterpret_cast<const S*>(reinterpret_cast<const char*>(plane) + (uint32_t)idx * (uint32_t)sizeof(S));
+}
+template <class S>
+__device__ __forceinline__ S* at(S* plane, int idx)
+{
+    return reinterpret_cast<S*>(reinterpret_cast<char*>(plane) + (uint32_t)idx * (uint32_t)sizeof(S));
+}
+
 template <class R>
 __device__ __forceinline__ R wave_sum(R v)
 {
@@ -66,6 +92,10 @@ __device__ __forceinline__ R wave_sum(R v)
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
+
+template <int MODEL, class S>
+__device__ __forceinline__ void node_boundary_collide(const StepParams& p, int x, int y, int z, int cell,
+                                                      typename Arith<S>::R f[Q]);
 
 // Fused pull stream + boundary dispatch + collide + bounce-back for one node
 // (the work of update_inn, MRTtest/DynInlet3.cu:1612-1840, and of the CPU
@@ -80,7 +110,7 @@ __device__ __forceinline__ void node_update(const StepParams& p, int x, int y, i
     S* __restrict__ fout = static_cast<S*>(p.fout);
     const long long qs = p.qs;
     const int cell = x + p.nxp * (y + p.ny * z);
-    const unsigned char code = p.mask ? p.mask[cell] : (unsigned char)0;
+    const unsigned char code = p.mask ? *at(p.mask, cell) : (unsigned char)0;
     const unsigned char base = code & 0x7f;
     if (p.halfway && is_solid(base)) return; // inert under link bounce-back
 
@@ -102,7 +132,7 @@ __device__ __forceinline__ void node_update(const StepParams& p, int x, int y, i
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const int src = row[1 - CY[i]][1 - CZ[i]] + X[1 - CX[i]];
-        f[i] = Store<S>::load(fin + i * qs + src, i);
+        f[i] = Store<S>::load(at(fin + i * qs, src), i);
     }
 
     if (code & NODE_NEAR) {
@@ -114,10 +144,10 @@ __device__ __forceinline__ void node_update(const StepParams& p, int x, int y, i
             const bool out = (CX[i] > 0 && oxm) || (CX[i] < 0 && oxp) || (CY[i] > 0 && oym) ||
                              (CY[i] < 0 && oyp) || (CZ[i] > 0 && ozm) || (CZ[i] < 0 && ozp);
             unsigned char sc = 0;
-            if (!out && p.halfway) sc = p.mask[row[1 - CY[i]][1 - CZ[i]] + X[1 - CX[i]]] & 0x7f;
+            if (!out && p.halfway) sc = *at(p.mask, row[1 - CY[i]][1 - CZ[i]] + X[1 - CX[i]]) & 0x7f;
             const bool bounce = out || is_solid(sc);
             if (bounce) {
-                const R v = Store<S>::load(fin + OPP[i] * qs + cell, OPP[i]);
+                const R v = Store<S>::load(at(fin + OPP[i] * qs, cell), OPP[i]);
                 f[i] = v;
                 if (p.fpart && sc == NODE_SOLID_FORCE) {
                     fx += R(2 * CX[OPP[i]]) * v;
@@ -129,8 +159,10 @@ __device__ __forceinline__ void node_update(const StepParams& p, int x, int y, i
     }
 
     if (is_solid(base)) {
-        // node bounce-back (MRTtest/DynInlet3.cu:1673-1705): store reversed;
-        // momentum exchange 2 sum c_i f_i on im 10 nodes (:1675-1687)
+        // node bounce-back (MRTtest/DynInlet3.cu:1673-1705): the pulled values
+        // are stored reversed (a register permutation, so every store below
+        // still writes plane i from a wave-uniform base); momentum exchange
+        // 2 sum c_i f_i on im 10 nodes (:1675-1687)
         if (p.fpart && base == NODE_SOLID_FORCE) {
 #pragma unroll
             for (int i = 1; i < Q; ++i) {
@@ -139,18 +171,34 @@ __device__ __forceinline__ void node_update(const StepParams& p, int x, int y, i
                 if (CZ[i]) fz += R(2 * CZ[i]) * f[i];
             }
         }
-        Store<S>::store(fout + cell, f[0], 0);
+        R g[Q];
 #pragma unroll
-        for (int i = 1; i < Q; ++i) Store<S>::store(fout + OPP[i] * qs + cell, f[i], OPP[i]);
-        return;
+        for (int i = 0; i < Q; ++i) g[i] = f[OPP[i]];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] = g[i];
+    } else {
+        node_boundary_collide<MODEL, S>(p, x, y, z, cell, f);
     }
 
+#pragma unroll
+    for (int i = 0; i < Q; ++i) Store<S>::store(at(fout + i * qs, cell), f[i], i);
+}
+
+// Boundary conditions of the non-solid node codes, then the collide.
+template <int MODEL, class S>
+__device__ __forceinline__ void node_boundary_collide(const StepParams& p, int x, int y, int z, int cell,
+                                                      typename Arith<S>::R f[Q])
+{
+    using R = typename Arith<S>::R;
+    const S* __restrict__ fin = static_cast<const S*>(p.fin);
+    const long long qs = p.qs;
+    const unsigned char base = p.mask ? (*at(p.mask, cell) & 0x7f) : (unsigned char)0;
     if (base == NODE_INLET_WEST || base == NODE_OUTLET_EAST) {
         // Extrap conditions start from the unstreamed neighbour (:1725-1746)
         const int nb = base == NODE_INLET_WEST ? (x + 1 < p.nx ? cell + 1 : cell)
                                                : (x > 0 ? cell - 1 : cell);
 #pragma unroll
-        for (int i = 0; i < Q; ++i) f[i] = Store<S>::load(fin + i * qs + nb, i);
+        for (int i = 0; i < Q; ++i) f[i] = Store<S>::load(at(fin + i * qs, nb), i);
         if (base == NODE_INLET_WEST) west_extrap(f, R(p.u_in[z * p.ny + y]));
         else east_extrap(f, R(1));
     } else if (base == NODE_LID_WEST_REG) {
@@ -163,13 +211,10 @@ __device__ __forceinline__ void node_update(const StepParams& p, int x, int y, i
     cp.dpdy = R(p.dpdy);
     cp.s1 = R(p.s1); cp.s2 = R(p.s2); cp.s4 = R(p.s4); cp.s10 = R(p.s10); cp.s16 = R(p.s16);
     collide<MODEL>(f, cp);
-
-#pragma unroll
-    for (int i = 0; i < Q; ++i) Store<S>::store(fout + i * qs + cell, f[i], i);
 }
 
 template <int MODEL, class S>
-__global__ void __launch_bounds__(TX* TY) k_stream_collide(StepParams p)
+__global__ void __launch_bounds__(TX* TY, LBM_MIN_BLOCKS) k_stream_collide(StepParams p)
 {
     using R = typename Arith<S>::R;
     const int lb = logical_block();

@@ -17,7 +17,28 @@
 
 #include "d3q19.h"
 
+#ifndef LBM_NT_LOAD
+#define LBM_NT_LOAD 0 // nontemporal population loads
+#endif
+#ifndef LBM_NT_STORE
+#define LBM_NT_STORE 1 // nontemporal population stores
+#endif
+
 namespace lbm {
+
+template <class T>
+__device__ __forceinline__ T ld(const T* p)
+{
+    if (LBM_NT_LOAD) return __builtin_nontemporal_load(p);
+    return *p;
+}
+
+template <class T>
+__device__ __forceinline__ void st(T* p, T v)
+{
+    if (LBM_NT_STORE) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
 
 struct Half16 { uint16_t bits; };
 struct BHalf16 { uint16_t bits; };
@@ -30,16 +51,16 @@ struct Store;
 
 template <>
 struct Store<float> {
-    __device__ static __forceinline__ float load(const float* p, int) { return *p; }
-    __device__ static __forceinline__ void store(float* p, float v, int) { __builtin_nontemporal_store(v, p); }
+    __device__ static __forceinline__ float load(const float* p, int) { return ld(p); }
+    __device__ static __forceinline__ void store(float* p, float v, int) { st(p, v); }
     __host__ __device__ static float to_real(float v, int) { return v; }
     __host__ __device__ static float from_real(float v, int) { return v; }
 };
 
 template <>
 struct Store<double> {
-    __device__ static __forceinline__ double load(const double* p, int) { return *p; }
-    __device__ static __forceinline__ void store(double* p, double v, int) { __builtin_nontemporal_store(v, p); }
+    __device__ static __forceinline__ double load(const double* p, int) { return ld(p); }
+    __device__ static __forceinline__ void store(double* p, double v, int) { st(p, v); }
     __host__ __device__ static double to_real(double v, int) { return v; }
     __host__ __device__ static double from_real(double v, int) { return v; }
 };
@@ -80,12 +101,12 @@ template <>
 struct Store<Half16> {
     __device__ static __forceinline__ float load(const Half16* p, int I)
     {
-        const uint16_t h = p->bits;
+        const uint16_t h = ld(&p->bits);
         return half_bits_to_float(h) + float(WNUM[I]) * (1.0f / 36.0f);
     }
     __device__ static __forceinline__ void store(Half16* p, float v, int I)
     {
-        __builtin_nontemporal_store(float_to_half_bits(v - float(WNUM[I]) * (1.0f / 36.0f)), &p->bits);
+        st(&p->bits, float_to_half_bits(v - float(WNUM[I]) * (1.0f / 36.0f)));
     }
     __host__ __device__ static float to_real(Half16 v, int i)
     {
@@ -101,12 +122,12 @@ template <>
 struct Store<BHalf16> {
     __device__ static __forceinline__ float load(const BHalf16* p, int I)
     {
-        const uint16_t h = p->bits;
+        const uint16_t h = ld(&p->bits);
         return bf16_bits_to_float(h) + float(WNUM[I]) * (1.0f / 36.0f);
     }
     __device__ static __forceinline__ void store(BHalf16* p, float v, int I)
     {
-        __builtin_nontemporal_store(float_to_bf16_bits(v - float(WNUM[I]) * (1.0f / 36.0f)), &p->bits);
+        st(&p->bits, float_to_bf16_bits(v - float(WNUM[I]) * (1.0f / 36.0f)));
     }
     __host__ __device__ static float to_real(BHalf16 v, int i)
     {
