@@ -103,8 +103,14 @@ typedef struct lbm_config {
     int refinement;        /* 0 off, 1 one 2:1 fine patch */
     int lr_nx, lr_ny, lr_nz;      /* fine nodes of the patch */
     float lr_x0, lr_y0, lr_z0;    /* position of fine node 0 in coarse units */
-    int temporal_interp;   /* 1: coarse state interpolated at t+1/2 for the
-                              first fine sub-step; 0: reused as the reference */
+    int temporal_interp;   /* 1: the frame is refilled before the second fine
+                              sub-step with the coarse moments interpolated to
+                              t+1/2; 0: filled once per coarse step, as the
+                              reference (MRTtest/DynInlet3.cu:3180-3245) */
+    int interface_eq;      /* equilibrium of the higher moments that InterpCF/
+                              InterpFC rebuild: 0 the collide's own (default);
+                              1 mrt_meq as InvertPhysicalMoments codes it
+                              (MRTtest/DynInlet3.cu:437-481) */
 } lbm_config;
 
 typedef struct lbm_sizes {

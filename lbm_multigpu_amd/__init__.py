@@ -47,7 +47,7 @@ class lbm_config(C.Structure):
                 ("obst_z", C.c_float), ("obst_r", C.c_float), ("startf", C.c_int),
                 ("force_steps", C.c_int), ("refinement", C.c_int), ("lr_nx", C.c_int),
                 ("lr_ny", C.c_int), ("lr_nz", C.c_int), ("lr_x0", C.c_float), ("lr_y0", C.c_float),
-                ("lr_z0", C.c_float), ("temporal_interp", C.c_int)]
+                ("lr_z0", C.c_float), ("temporal_interp", C.c_int), ("interface_eq", C.c_int)]
 
 
 class lbm_sizes(C.Structure):

@@ -66,6 +66,7 @@ class Config:
     lr_loc: tuple = (0.0, 0.0, 0.0)
     omega_lr: float = 0.0
     temporal_interp: bool = True
+    interface_eq: int = 0       # 0: the collide's own equilibrium; 1: mrt_meq as coded
     gpus: int = 1
 
     def model_code(self):
@@ -99,6 +100,7 @@ class Config:
         c.lr_nx, c.lr_ny, c.lr_nz = self.lr_size
         c.lr_x0, c.lr_y0, c.lr_z0 = self.lr_loc
         c.temporal_interp = int(self.temporal_interp)
+        c.interface_eq = int(self.interface_eq)
         return c
 
 
@@ -295,3 +297,30 @@ def cpu3d_case():
     return Config(nx=251, ny=43, nz=43, bc=("clamp", "clamp", "clamp"), model="MRT",
                   mrt_rates="cpu3d", omega=om, umax=umax, re=re, geometry="cpu3d",
                   obst1=(50.5, 20.5, 20.5), obst1_r=r, tmax=100)
+
+
+def patch_offset(seed):
+    """Seeded integer offset in [-4, 4]^3 (counter-based hash, as the inlet)."""
+    from . import load_library
+
+    lib = load_library()
+    return tuple(int(lib.lbm_hash_u32(seed, 1_000_000 + a) % 9) - 4 for a in range(3))
+
+
+def refined_channel(nx=256, ny=128, nz=128, side=16, x_center=64, patch=(128, 96, 96), tau=0.52, cs_smag=0.1,
+                    u0=0.05, amp=1e-3, seed=0, rates="dhumieres", temporal=True, storage="f32"):
+    """The channel with a 2:1 fine patch of `patch` fine nodes around the
+    cylinder; the patch's coarse origin is centred on (x_center + side,
+    ny/2, nz/2) and shifted by a seeded offset within +-4 coarse cells."""
+    c = channel(nx=nx, ny=ny, nz=nz, side=side, x_center=x_center, tau=tau, cs_smag=cs_smag, u0=u0, amp=amp,
+                seed=seed, storage=storage, rates=rates)
+    off = patch_offset(seed)
+    half = [p // 4 for p in patch]            # half the patch span in coarse cells
+    centre = (x_center + side, ny // 2, nz // 2)
+    a = [centre[k] - half[k] + off[k] for k in range(3)]
+    c.refinement = True
+    c.lr_size = tuple(patch)
+    c.lr_loc = tuple(float(a[k]) + 0.25 for k in range(3))
+    c.temporal_interp = temporal
+    c.tmax = 2000
+    return c

@@ -56,9 +56,22 @@ struct lbm_ctx {
     int force_recorded = 0;
     bool geometry_set = false;
     bool initialised = false;
-    RefineState refine;
+    // local refinement
+    RefineGeom rg{};
+    float* Mold = nullptr;     // footprint moments at the step's start
+    float* Mnew = nullptr;     // and at its end
+    int* cf_blocks = nullptr;  // origins of the 8^3 fine blocks that meet the frame
+    int n_cf = 0;
+    int fc_nb[3] = {0, 0, 0};  // FC workgroups per axis (4^3 coarse nodes each)
     std::string err;
 };
+
+static int refine_setup(lbm_ctx* ctx);
+static void classify_fine(lbm_ctx* ctx, bool user_mask);
+static int refine_extract(lbm_ctx* ctx, float* M);
+static int refine_cf(lbm_ctx* ctx, float alpha);
+static int refine_fc(lbm_ctx* ctx);
+static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f);
 
 static int fail(lbm_ctx* ctx, int code, const std::string& msg)
 {
@@ -164,8 +177,16 @@ static int validate(const lbm_config* c, std::string& why)
         for (int k : {1, 2, 4, 6, 8, 10, 12, 16, 17, 18})
             if (!(c->s_dhum[k] > 0.0f && c->s_dhum[k] < 2.0f)) { why = "d'Humieres rates must lie in (0,2)"; return 0; }
     if (c->refinement) {
-        std::string w2;
-        if (!refine_validate(*c, w2)) { why = w2; return 0; }
+        const int nf[3] = {c->lr_nx, c->lr_ny, c->lr_nz};
+        const float x0[3] = {c->lr_x0, c->lr_y0, c->lr_z0};
+        const int n[3] = {c->nx, c->ny, c->nz};
+        for (int a = 0; a < 3; ++a) {
+            if (nf[a] < 12 || nf[a] % 2) { why = "LR domain size must be even and >= 12 per axis"; return 0; }
+            const float ai = x0[a] - 0.25f;
+            if (ai != std::floor(ai)) { why = "LR location must be an integer + 0.25 per axis"; return 0; }
+            if (ai < 1.0f || ai + nf[a] / 2 > n[a] - 2) { why = "the refinement patch must lie inside the domain"; return 0; }
+        }
+        if (c->model == LBM_MODEL_MRT_CPU3D) { why = "refinement needs an LES-family or BGK model"; return 0; }
     }
     return 1;
 }
@@ -257,7 +278,7 @@ extern "C" int lbm_create(const lbm_config* cfg, lbm_ctx** out)
             g_last_error = ctx->err;
             return bail(rc);
         }
-        if ((rc = refine_create(ctx->refine, *cfg, ctx->stream, ctx->err)) != LBM_OK) {
+        if ((rc = refine_setup(ctx)) != LBM_OK) {
             g_last_error = ctx->err;
             return bail(rc);
         }
@@ -273,8 +294,8 @@ extern "C" void lbm_destroy(lbm_ctx* ctx)
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (int l = 0; l < 2; ++l) free_level(ctx->lv[l]);
-    refine_destroy(ctx->refine);
-    for (void* p : {(void*)ctx->u_in, (void*)ctx->fpart, (void*)ctx->fhist, (void*)ctx->red, (void*)ctx->mpart})
+    for (void* p : {(void*)ctx->u_in, (void*)ctx->fpart, (void*)ctx->fhist, (void*)ctx->red, (void*)ctx->mpart,
+                    (void*)ctx->Mold, (void*)ctx->Mnew, (void*)ctx->cf_blocks})
         if (p) (void)hipFree(p);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -373,7 +394,7 @@ extern "C" int lbm_set_geometry(lbm_ctx* ctx, const uint8_t* mask)
     HIP_TRY(ctx, hipMemcpyAsync(L.mask, dev.data(), dev.size(), hipMemcpyHostToDevice, ctx->stream));
     if (ctx->nlevels > 1) {
         Level& F = ctx->lv[1];
-        refine_classify_fine(ctx->refine, c, L.host_mask, F.host_mask);
+        classify_fine(ctx, mask != nullptr);
         const int fbc[3] = {LBM_BC_CLAMP, LBM_BC_CLAMP, LBM_BC_CLAMP};
         std::vector<unsigned char> fdev;
         build_device_mask(F, fbc, c.halfway, fdev);
@@ -452,7 +473,7 @@ extern "C" int lbm_init_equilibrium(lbm_ctx* ctx, int kind, const float* rho, co
     if (ctx->nlevels > 1) {
         rc = init_level(ctx, 1, kind, nullptr, nullptr, nullptr, nullptr, ctx->cfg.umax);
         if (rc) return rc;
-        rc = refine_init(ctx->refine, ctx->stream, ctx->err);
+        rc = refine_extract(ctx, ctx->Mold);
         if (rc) return rc;
     }
     ctx->t = 0;
@@ -534,8 +555,8 @@ static int coarse_step(lbm_ctx* ctx)
     return LBM_OK;
 }
 
-// Fine-level sub-step launched by the refinement driver.
-int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f)
+// Fine-level sub-step (update_inn_LR, MRTtest/DynInlet3.cu:2051-2201).
+static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f)
 {
     Level& F = ctx->lv[1];
     const lbm_config& c = ctx->cfg;
@@ -550,21 +571,128 @@ int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f)
     return LBM_OK;
 }
 
+// ---------------------------------------------------------------- refinement
+
+static int refine_setup(lbm_ctx* ctx)
+{
+    const lbm_config& c = ctx->cfg;
+    RefineGeom& g = ctx->rg;
+    const int nf[3] = {c.lr_nx, c.lr_ny, c.lr_nz};
+    const float x0[3] = {c.lr_x0, c.lr_y0, c.lr_z0};
+    for (int a = 0; a < 3; ++a) {
+        g.a[a] = (int)std::lround(x0[a] - 0.25f);
+        g.nf[a] = nf[a];
+        g.nc[a] = nf[a] / 2 + 1;
+        ctx->fc_nb[a] = (nf[a] / 2 - 3 + 3) / 4;
+    }
+    g.omega_c = c.omega;
+    // omegaLR as main() computes it (MRTtest/DynInlet3.cu:2870)
+    g.omega_f = 2.0f / (1.0f + 2.0f * (2.0f / c.omega - 1.0f));
+    g.cs = c.cs;
+    g.eq_kind = c.interface_eq ? EQ_MRT_MEQ : (c.model == LBM_MODEL_BGK ? EQ_BGK_PROJ : EQ_BGK_MEQ);
+    const size_t NC = (size_t)g.nc[0] * g.nc[1] * g.nc[2];
+    HIP_TRY(ctx, hipMalloc(&ctx->Mold, 9 * NC * sizeof(float)));
+    HIP_TRY(ctx, hipMalloc(&ctx->Mnew, 9 * NC * sizeof(float)));
+    std::vector<int> blocks;
+    for (int k = 0; k < nf[2]; k += 8)
+        for (int j = 0; j < nf[1]; j += 8)
+            for (int i = 0; i < nf[0]; i += 8) {
+                const bool interior = i >= 2 && i + 8 <= nf[0] - 2 && j >= 2 && j + 8 <= nf[1] - 2 && k >= 2 &&
+                                      k + 8 <= nf[2] - 2;
+                if (!interior) { blocks.push_back(i); blocks.push_back(j); blocks.push_back(k); }
+            }
+    ctx->n_cf = (int)(blocks.size() / 3);
+    HIP_TRY(ctx, hipMalloc(&ctx->cf_blocks, blocks.size() * sizeof(int)));
+    HIP_TRY(ctx, hipMemcpy(ctx->cf_blocks, blocks.data(), blocks.size() * sizeof(int), hipMemcpyHostToDevice));
+    return LBM_OK;
+}
+
+// Fine-level node codes: the geometry generator evaluated at the fine node's
+// coarse coordinates (ImageFcnLR, MRTtest/DynInlet3.cu:62-70), or, for a
+// caller-supplied coarse mask, the code of the nearest coarse node.
+static void classify_fine(lbm_ctx* ctx, bool user_mask)
+{
+    const lbm_config& c = ctx->cfg;
+    Level& L = ctx->lv[0];
+    Level& F = ctx->lv[1];
+    for (int k = 0; k < F.nz; ++k)
+        for (int j = 0; j < F.ny; ++j)
+            for (int i = 0; i < F.nx; ++i) {
+                const float X = c.lr_x0 + 0.5f * i, Y = c.lr_y0 + 0.5f * j, Z = c.lr_z0 + 0.5f * k;
+                unsigned char v = LBM_NODE_FLUID;
+                if (user_mask || c.geometry != LBM_GEOM_CHANNEL) {
+                    const int xi = std::min(std::max((int)std::lround(X), 0), L.nx - 1);
+                    const int yi = std::min(std::max((int)std::lround(Y), 0), L.ny - 1);
+                    const int zi = std::min(std::max((int)std::lround(Z), 0), L.nz - 1);
+                    v = L.host_mask[(size_t)xi + (size_t)L.nx * (yi + (size_t)L.ny * zi)];
+                    if (!is_solid(v)) v = LBM_NODE_FLUID;
+                } else if (std::fabs(X - c.obst_x) < c.obst_r && std::fabs(Y - c.obst_y) < c.obst_r) {
+                    v = LBM_NODE_SOLID_FORCE;
+                }
+                F.host_mask[(size_t)i + (size_t)F.nx * (j + (size_t)F.ny * k)] = v;
+            }
+}
+
+static int refine_extract(lbm_ctx* ctx, float* M)
+{
+    Level& L = ctx->lv[0];
+    const size_t NC = (size_t)ctx->rg.nc[0] * ctx->rg.nc[1] * ctx->rg.nc[2];
+    with_storage(L.storage, [&](auto* tag) {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL(k_extract<S>, dim3((unsigned)((NC + 255) / 256)), dim3(256), 0, ctx->stream,
+                           (const S*)L.fa, L.qs, L.nxp, L.ny, ctx->rg, M);
+        return 0;
+    });
+    HIP_TRY(ctx, hipGetLastError());
+    return LBM_OK;
+}
+
+static int refine_cf(lbm_ctx* ctx, float alpha)
+{
+    Level& F = ctx->lv[1];
+    with_storage(F.storage, [&](auto* tag) {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL(k_interp_cf<S>, dim3(ctx->n_cf), dim3(8, 8, 8), 0, ctx->stream, (S*)F.fa, F.qs, F.nxp,
+                           F.mask, ctx->cf_blocks, ctx->Mold, ctx->Mnew, alpha, ctx->rg);
+        return 0;
+    });
+    HIP_TRY(ctx, hipGetLastError());
+    return LBM_OK;
+}
+
+static int refine_fc(lbm_ctx* ctx)
+{
+    Level& L = ctx->lv[0];
+    Level& F = ctx->lv[1];
+    const int nb = ctx->fc_nb[0] * ctx->fc_nb[1] * ctx->fc_nb[2];
+    with_storage(F.storage, [&](auto* tag) {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL(k_interp_fc<S>, dim3(nb), dim3(8, 8, 8), 0, ctx->stream, (const S*)F.fa, F.qs, F.nxp,
+                           (S*)L.fa, L.qs, L.nxp, L.ny, L.mask, ctx->rg, ctx->fc_nb[0], ctx->fc_nb[1]);
+        return 0;
+    });
+    HIP_TRY(ctx, hipGetLastError());
+    return LBM_OK;
+}
+
 extern "C" int lbm_step(lbm_ctx* ctx, int nsteps)
 {
     if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_step: null context");
     if (!ctx->initialised) return fail(ctx, LBM_ESTATE, "lbm_step: call lbm_init_equilibrium first");
     if (nsteps < 0) return fail(ctx, LBM_EINVAL, "lbm_step: nsteps < 0");
     for (int s = 0; s < nsteps; ++s) {
+        coarse_step(ctx);
         if (ctx->nlevels > 1) {
-            RefineLevels rl{&ctx->lv[0].fa, &ctx->lv[0].fb, ctx->lv[0].mask, ctx->lv[0].qs, ctx->lv[0].nxp,
-                            &ctx->lv[1].fa, &ctx->lv[1].fb, ctx->lv[1].mask, ctx->lv[1].qs, ctx->lv[1].nxp};
-            int rc = refine_step(ctx->refine, ctx->cfg, rl, ctx->stream,
-                                 [&]() { return coarse_step(ctx); },
-                                 [&](float om) { return lbm_internal_fine_step(ctx, om); }, ctx->err);
+            // coarse t -> t+1 done; fine sub-steps t -> t+1/2 -> t+1 (main(),
+            // MRTtest/DynInlet3.cu:3180-3245), then the fine -> coarse restriction
+            int rc = refine_extract(ctx, ctx->Mnew);
+            if (!rc) rc = refine_cf(ctx, 0.0f);
+            if (!rc) rc = lbm_internal_fine_step(ctx, ctx->rg.omega_f);
+            if (!rc && ctx->cfg.temporal_interp) rc = refine_cf(ctx, 0.5f);
+            if (!rc) rc = lbm_internal_fine_step(ctx, ctx->rg.omega_f);
+            if (!rc) rc = refine_fc(ctx);
             if (rc) return rc;
-        } else {
-            coarse_step(ctx);
+            std::swap(ctx->Mold, ctx->Mnew);
         }
         ctx->t += 1;
     }
@@ -665,7 +793,7 @@ static int set_pops(lbm_ctx* ctx, int level, const T* f, const char* who)
     if (level == 0) {
         ctx->initialised = true;
         if (ctx->nlevels > 1) {
-            int rc = refine_init(ctx->refine, ctx->stream, ctx->err);
+            int rc = refine_extract(ctx, ctx->Mold);
             if (rc) return rc;
         }
     }
@@ -770,9 +898,9 @@ extern "C" int lbm_write_tecplot(lbm_ctx* ctx, const char* casename)
                     const size_t c = (size_t)x + (size_t)L.nx * (y + (size_t)L.ny * z);
                     if (level == 0) out << x << ", " << y << ", " << z << ", ";
                     else {
-                        float X0, Y0, Z0;
-                        refine_fine_coords(ctx->cfg, x, y, z, X0, Y0, Z0);
-                        out << X0 << ", " << Y0 << ", " << Z0 << ", ";
+                        // fine node position in coarse units (WriteResultsLR)
+                        out << ctx->cfg.lr_x0 + 0.5f * x << ", " << ctx->cfg.lr_y0 + 0.5f * y << ", "
+                            << ctx->cfg.lr_z0 + 0.5f * z << ", ";
                     }
                     out << (float)u[c] << "," << (float)v[c] << "," << (float)w[c] << "," << (float)r[c] << "\n";
                 }

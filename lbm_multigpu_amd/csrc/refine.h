@@ -1,46 +1,326 @@
-// refine.h -- 2:1 local refinement (coarse-fine interface) driver.
+// refine.h -- 2:1 local refinement: the coarse-fine interface kernels.
+//
+// Geometry: fine node (i,j,k) sits at coarse coordinates
+// (a_x + 0.25 + 0.5 i, a_y + 0.25 + 0.5 j, a_z + 0.25 + 0.5 k) (the
+// reference's LRX0 = 128.25 layout, MRTtest/DynInlet3.cu:41-47).  The coarse
+// footprint is nodes a .. a + n_f/2 per axis; its post-collision physical
+// moments (PhysicalMoments, :396-407) are kept for the step's start (Mold)
+// and end (Mnew).  Per coarse step:
+//   coarse stream-collide -> extract Mnew -> [CF(Mold) -> fine step] ->
+//   [CF(Mold/2 + Mnew/2) if temporal] -> fine step -> FC -> Mold = Mnew
+// CF (InterpCF, :2202-2382) fills a frame two fine nodes thick on all six
+// faces: trilinear interpolation of the 8 surrounding coarse moments, the
+// ORDER-2 momentum correction (:2312-2357), and the LES-aware
+// non-equilibrium rescale (InvertPhysicalMoments_LES_cf, :540-595).  FC
+// (InterpFC, :2383-2560) replaces the coarse nodes a+2 .. a+n_f/2-2 by the
+// mean of their 8 fine neighbours, the same correction with unit weights, and
+// InvertPhysicalMoments_LES_fc (:483-538).
 #pragma once
 
-#include <hip/hip_runtime.h>
+#include "collide.h"
+#include "kernels.h"
+#include "storage.h"
 
-#include <functional>
-#include <string>
-#include <vector>
+namespace lbm {
 
-#include "../../include/lbm.h"
+// Higher-moment equilibrium of the populations CF/FC rebuild.
+// InvertPhysicalMoments (MRTtest/DynInlet3.cu:437-481) takes them from
+// mrt_meq (eps = +475/63 j^2, 3pixx = piww = 0), which is not what the
+// LES-family collide relaxes to (bgk_meq: eps = 3 rho - 11/2 j^2,
+// 3pixx = -meq9/6, piww = -meq11/2): that mismatch writes an eps jump of
+// ~3 rho into the frame, which streaming turns into density errors.
+enum EqKind : int { EQ_BGK_MEQ = 0, EQ_BGK_PROJ = 1, EQ_MRT_MEQ = 2 };
 
-struct RefineState {
-    bool active = false;
+struct RefineGeom {
+    int a[3];     // footprint origin (coarse)
+    int nf[3];    // fine nodes per axis
+    int nc[3];    // footprint nodes per axis (nf/2 + 1)
+    float omega_c, omega_f, cs;
+    int eq_kind;  // EqKind
 };
 
-struct RefineLevels {
-    void** ca; void** cb; const unsigned char* cmask; long long cqs; int cnxp;
-    void** fa; void** fb; const unsigned char* fmask; long long fqs; int fnxp;
-};
+// PhysicalMoments: rho, jx, jy, jz, 3pxx row, pww row, pxy, pyz, pxz.
+template <class R>
+__device__ __forceinline__ void phys_moments(const R f[Q], R mom[9])
+{
+    R s[NP], d[NP];
+    pair_split(f, s, d);
+    mom[0] = moment(0, f[0], s, d);
+    mom[1] = moment(3, f[0], s, d);
+    mom[2] = moment(5, f[0], s, d);
+    mom[3] = moment(7, f[0], s, d);
+    mom[4] = moment(9, f[0], s, d);
+    mom[5] = moment(11, f[0], s, d);
+    mom[6] = moment(13, f[0], s, d);
+    mom[7] = moment(14, f[0], s, d);
+    mom[8] = moment(15, f[0], s, d);
+}
 
-inline bool refine_validate(const lbm_config&, std::string& why)
+// StrainRate (:367-393) as coded: omega = 1, m1 = 0.
+__device__ __forceinline__ void strain_rate(const float m[9], float S[6])
 {
-    why = "local refinement is not available in this build";
-    return false;
+    const float u = m[1], v = m[2], w = m[3];
+    const float m9 = m[4] - (2.f * u * u - (v * v + w * w));
+    const float m11 = m[5] - (v * v - w * w);
+    S[0] = -0.026315789f * (19.f * m9);
+    S[1] = -0.013157895f * (-19.f * (m9 - 3.f * m11));
+    S[2] = -0.013157895f * (-19.f * (m9 + 3.f * m11));
+    S[3] = -1.5f * (m[6] - u * v);
+    S[4] = -1.5f * (m[7] - v * w);
+    S[5] = -1.5f * (m[8] - u * w);
 }
-inline int refine_create(RefineState&, const lbm_config&, hipStream_t, std::string& err)
+
+__device__ __forceinline__ float smag_of(const float S[6])
 {
-    err = "local refinement is not available in this build";
-    return LBM_EINVAL;
+    return sqrtf(2.f * (S[0] * S[0] + S[1] * S[1] + S[2] * S[2] + 2.f * S[3] * S[3] + 2.f * S[4] * S[4] +
+                        2.f * S[5] * S[5]));
 }
-inline void refine_destroy(RefineState&) {}
-inline void refine_classify_fine(RefineState&, const lbm_config&, const std::vector<unsigned char>&,
-                                 std::vector<unsigned char>&) {}
-inline int refine_init(RefineState&, hipStream_t, std::string&) { return LBM_OK; }
-inline int refine_step(RefineState&, const lbm_config&, const RefineLevels&, hipStream_t,
-                       const std::function<int()>&, const std::function<int(float)>&, std::string& err)
+
+// Populations from moments: conserved and non-equilibrium-scaled second
+// moments; every other moment at the equilibrium selected by eq_kind.
+__device__ __forceinline__ void invert_phys(float f[Q], const float mom[9], float SF, int eq_kind)
 {
-    err = "local refinement is not available in this build";
-    return LBM_EINVAL;
+    float m[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) m[k] = 0.f;
+    const float u = mom[1], v = mom[2], w = mom[3];
+    const float usq = u * u + v * v + w * w;
+    m[0] = mom[0];
+    m[1] = -11.f * mom[0] + 19.f * usq;
+    if (eq_kind == EQ_MRT_MEQ) {
+        m[2] = 7.53968254f * usq;
+    } else {
+        const float m9eq = 2.f * u * u - (v * v + w * w), m11eq = v * v - w * w;
+        m[2] = 3.f * mom[0] - 5.5f * usq;
+        m[10] = eq_kind == EQ_BGK_MEQ ? -0.5f * m9eq * (1.0f / 3.0f) : -0.5f * m9eq;
+        m[12] = -0.5f * m11eq;
+    }
+    m[3] = u; m[4] = -0.666666667f * u;
+    m[5] = v; m[6] = -0.666666667f * v;
+    m[7] = w; m[8] = -0.666666667f * w;
+    m[9] = mom[4] * SF + (1.f - SF) * (2.f * u * u - (v * v + w * w));
+    m[11] = mom[5] * SF + (1.f - SF) * (v * v - w * w);
+    m[13] = mom[6] * SF + (1.f - SF) * u * v;
+    m[14] = mom[7] * SF + (1.f - SF) * v * w;
+    m[15] = mom[8] * SF + (1.f - SF) * u * w;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < Q; ++k)
+            if (mrow(k, i) != 0) acc += (float(mrow(k, i)) / float(NORM[k])) * m[k];
+        f[i] = acc;
+    }
 }
-inline void refine_fine_coords(const lbm_config& c, int x, int y, int z, float& X, float& Y, float& Z)
+
+// ORDER-2 correction (:2312-2357).  Corners c = 0..7 are (x,y,z) bits of c;
+// fc_form = 1 gives the InterpFC weights (xpr = ypr = zpr = 0).
+__device__ __forceinline__ void order2(const float (*Sc)[6], const float (*uc)[3], float mom[9], float xf, float yf,
+                                       float zf, bool fc_form)
 {
-    X = c.lr_x0 + 0.5f * x;
-    Y = c.lr_y0 + 0.5f * y;
-    Z = c.lr_z0 + 0.5f * z;
+#define SXX(k) Sc[(k)-1][0]
+#define SYY(k) Sc[(k)-1][1]
+#define SZZ(k) Sc[(k)-1][2]
+#define SXY(k) Sc[(k)-1][3]
+#define SYZ(k) Sc[(k)-1][4]
+#define SXZ(k) Sc[(k)-1][5]
+#define U(k) uc[(k)-1][0]
+#define V(k) uc[(k)-1][1]
+#define W(k) uc[(k)-1][2]
+    const float cx = -((SXX(8) - SXX(7) + SXX(6) - SXX(5) + SXX(4) - SXX(3) + SXX(2) - SXX(1))) * 0.03125f;
+    const float cy = -((SXY(8) + SXY(7) - SXY(6) - SXY(5) + SXY(4) + SXY(3) - SXY(2) - SXY(1)) - V(8) + V(7) +
+                       V(6) - V(5) - V(4) + V(3) + V(2) - V(1)) * 0.0625f;
+    const float cz = -((SXZ(8) + SXZ(7) + SXZ(6) + SXZ(5) - SXZ(4) - SXZ(3) - SXZ(2) - SXZ(1)) - W(8) + W(7) -
+                       W(6) + W(5) + W(4) - W(3) + W(2) - W(1)) * 0.0625f;
+    const float dx = -((SXY(8) - SXY(7) + SXY(6) - SXY(5) + SXY(4) - SXY(3) + SXY(2) - SXY(1)) - U(8) + U(7) +
+                       U(6) - U(5) - U(4) + U(3) + U(2) - U(1)) * 0.0625f;
+    const float dy = -((SYY(8) + SYY(7) - SYY(6) - SYY(5) + SYY(4) + SYY(3) - SYY(2) - SYY(1))) * 0.03125f;
+    const float dz = -((SYZ(8) + SYZ(7) + SYZ(6) + SYZ(5) - SYZ(4) - SYZ(3) - SYZ(2) - SYZ(1)) - W(8) - W(7) +
+                       W(6) + W(5) + W(4) + W(3) - W(2) - W(1)) * 0.0625f;
+    const float ex = -((SXZ(8) - SXZ(7) + SXZ(6) - SXZ(5) + SXZ(4) - SXZ(3) + SXZ(2) - SXZ(1)) - U(8) + U(7) -
+                       U(6) + U(5) + U(4) - U(3) + U(2) - U(1)) * 0.0625f;
+    const float ey = -((SYZ(8) + SYZ(7) - SYZ(6) - SYZ(5) + SYZ(4) + SYZ(3) - SYZ(2) - SYZ(1)) - V(8) - V(7) +
+                       V(6) + V(5) + V(4) + V(3) - V(2) - V(1)) * 0.0625f;
+    const float ez = -((SZZ(8) + SZZ(7) + SZZ(6) + SZZ(5) - SZZ(4) - SZZ(3) - SZZ(2) - SZZ(1))) * 0.03125f;
+#undef SXX
+#undef SYY
+#undef SZZ
+#undef SXY
+#undef SYZ
+#undef SXZ
+#undef U
+#undef V
+#undef W
+    float wx = 1.f, wy = 1.f, wz = 1.f;
+    if (!fc_form) {
+        wx = 1.f - (4.f * xf * xf - 4.f * xf + 1.f);
+        wy = 1.f - (4.f * yf * yf - 4.f * yf + 1.f);
+        wz = 1.f - (4.f * zf * zf - 4.f * zf + 1.f);
+    }
+    mom[1] += cx * wx + cy * wy + cz * wz;
+    mom[2] += dx * wx + dy * wy + dz * wz;
+    mom[3] += ex * wx + ey * wy + ez * wz;
 }
+
+// Footprint moments of the coarse lattice (one thread per footprint node).
+template <class S>
+__global__ void __launch_bounds__(256) k_extract(const S* fc, long long qs, int nxp, int ny, RefineGeom g, float* M)
+{
+    const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
+    const long long NC = (long long)ncx * ncy * ncz;
+    const long long o = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (o >= NC) return;
+    const int i = int(o % ncx), j = int((o / ncx) % ncy), k = int(o / ((long long)ncx * ncy));
+    const int cell = (g.a[0] + i) + nxp * ((g.a[1] + j) + ny * (g.a[2] + k));
+    float f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = float(Store<S>::to_real(fc[q * qs + cell], q));
+    float mom[9];
+    phys_moments(f, mom);
+#pragma unroll
+    for (int m = 0; m < 9; ++m) M[m * NC + o] = mom[m];
+}
+
+// Coarse -> fine frame.  One workgroup per 8x8x8 fine block that meets the
+// frame (origins listed by the host); the 5x5x5 coarse tile it needs is
+// staged in LDS with its strain rates, blended in time by alpha.
+template <class S>
+__global__ void __launch_bounds__(512) k_interp_cf(S* ff, long long fqs, int fnxp, const unsigned char* fmask,
+                                                   const int* blocks, const float* Mold, const float* Mnew,
+                                                   float alpha, RefineGeom g)
+{
+    __shared__ float tm[125][9];
+    __shared__ float ts[125][6];
+    const int ox = blocks[3 * blockIdx.x], oy = blocks[3 * blockIdx.x + 1], oz = blocks[3 * blockIdx.x + 2];
+    const int t = threadIdx.x + 8 * (threadIdx.y + 8 * threadIdx.z);
+    const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
+    const long long NC = (long long)ncx * ncy * ncz;
+    if (t < 125) {
+        const int cx = ox / 2 + t % 5, cy = oy / 2 + (t / 5) % 5, cz = oz / 2 + t / 25;
+        float m[9];
+        if (cx < ncx && cy < ncy && cz < ncz) {
+            const long long o = cx + (long long)ncx * (cy + (long long)ncy * cz);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const float a0 = Mold[k * NC + o];
+                m[k] = alpha == 0.f ? a0 : (1.f - alpha) * a0 + alpha * Mnew[k * NC + o];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) m[k] = 0.f;
+        }
+        float Sv[6];
+        strain_rate(m, Sv);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) tm[t][k] = m[k];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) ts[t][k] = Sv[k];
+    }
+    __syncthreads();
+    const int i = ox + threadIdx.x, j = oy + threadIdx.y, k = oz + threadIdx.z;
+    const int nfx = g.nf[0], nfy = g.nf[1], nfz = g.nf[2];
+    if (i >= nfx || j >= nfy || k >= nfz) return;
+    if (i >= 2 && i < nfx - 2 && j >= 2 && j < nfy - 2 && k >= 2 && k < nfz - 2) return;
+    const int fcell = i + fnxp * (j + nfy * k);
+    if (fmask && is_solid(fmask[fcell])) return;
+    const int lx = threadIdx.x / 2, ly = threadIdx.y / 2, lz = threadIdx.z / 2;
+    const float xf = (i & 1) ? 0.75f : 0.25f, yf = (j & 1) ? 0.75f : 0.25f, zf = (k & 1) ? 0.75f : 0.25f;
+    float Sc[8][6], uc[8][3], mom[9];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) mom[m] = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
+        const int l = (lx + dx) + 5 * ((ly + dy) + 5 * (lz + dz));
+        const float w = (dx ? xf : 1.f - xf) * (dy ? yf : 1.f - yf) * (dz ? zf : 1.f - zf);
+#pragma unroll
+        for (int m = 0; m < 9; ++m) mom[m] += tm[l][m] * w;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) Sc[c][m] = ts[l][m];
+        uc[c][0] = tm[l][1]; uc[c][1] = tm[l][2]; uc[c][2] = tm[l][3];
+    }
+    order2(Sc, uc, mom, xf, yf, zf, false);
+    // InvertPhysicalMoments_LES_cf (:540-595)
+    float Sv[6];
+    strain_rate(mom, Sv);
+    const float Smag_c = smag_of(Sv);
+    float tau_c = 1.f / g.omega_c;
+    const float tau_f = 2.f * tau_c - 0.5f + 1.5f * Smag_c * g.cs;
+    const float omega_f = 1.f / tau_f;
+    tau_c = tau_c + Smag_c * g.cs;
+    const float omega_c = 1.f / tau_c;
+    const float SF = (0.5f * omega_c * (1.f - omega_f)) / ((1.f - omega_c) * omega_f);
+    float f[Q];
+    invert_phys(f, mom, SF, g.eq_kind);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) ff[q * fqs + fcell] = Store<S>::from_real(typename Arith<S>::R(f[q]), q);
+}
+
+// Fine -> coarse.  One workgroup per 4x4x4 coarse nodes: its 512 threads load
+// the 8x8x8 fine nodes below them (coalesced along x) and reduce in LDS.
+template <class S>
+__global__ void __launch_bounds__(512) k_interp_fc(const S* ff, long long fqs, int fnxp, S* fc, long long cqs,
+                                                   int cnxp, int cny, const unsigned char* cmask, RefineGeom g,
+                                                   int nbx, int nby)
+{
+    __shared__ float fm[512][9];
+    const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
+    // coarse nodes a+2+4b .. a+2+4b+3; fine nodes 2(cx-a)-1 .. (8 per axis)
+    const int fx0 = 2 * (2 + 4 * bx) - 1, fy0 = 2 * (2 + 4 * by) - 1, fz0 = 2 * (2 + 4 * bz) - 1;
+    const int t = threadIdx.x + 8 * (threadIdx.y + 8 * threadIdx.z);
+    const int i = fx0 + threadIdx.x, j = fy0 + threadIdx.y, k = fz0 + threadIdx.z;
+    float mom[9];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) mom[m] = 0.f;
+    if (i < g.nf[0] && j < g.nf[1] && k < g.nf[2]) {
+        const int fcell = i + fnxp * (j + g.nf[1] * k);
+        float f[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) f[q] = float(Store<S>::to_real(ff[q * fqs + fcell], q));
+        phys_moments(f, mom);
+    }
+#pragma unroll
+    for (int m = 0; m < 9; ++m) fm[t][m] = mom[m];
+    __syncthreads();
+    if (t >= 64) return;
+    const int lx = t % 4, ly = (t / 4) % 4, lz = t / 16;
+    const int cx = g.a[0] + 2 + 4 * bx + lx, cy = g.a[1] + 2 + 4 * by + ly, cz = g.a[2] + 2 + 4 * bz + lz;
+    if (cx > g.a[0] + g.nf[0] / 2 - 2 || cy > g.a[1] + g.nf[1] / 2 - 2 || cz > g.a[2] + g.nf[2] / 2 - 2) return;
+    const int ccell = cx + cnxp * (cy + cny * cz);
+    if (cmask && is_solid(cmask[ccell])) return;
+    float Sc[8][6], uc[8][3], avg[9];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) avg[m] = 0.f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
+        const int l = (2 * lx + dx) + 8 * ((2 * ly + dy) + 8 * (2 * lz + dz));
+        float m[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) m[q] = fm[l][q];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) avg[q] += m[q];
+        strain_rate(m, Sc[c]);
+        uc[c][0] = m[1]; uc[c][1] = m[2]; uc[c][2] = m[3];
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) avg[q] *= 0.125f;
+    order2(Sc, uc, avg, 0.f, 0.f, 0.f, true);
+    // InvertPhysicalMoments_LES_fc (:483-538)
+    float Sv[6];
+    strain_rate(avg, Sv);
+    const float Smag_f = smag_of(Sv);
+    float tau_f = 1.f / g.omega_f;
+    float tau_c = tau_f + 0.5f + 12.f * Smag_f * g.cs;
+    tau_c *= 0.5f;
+    const float omega_c = 1.f / tau_c;
+    tau_f = tau_f + Smag_f * g.cs;
+    const float omega_f = 1.f / tau_f;
+    const float SF = (1.f - omega_c) * omega_f / (0.5f * omega_c * (1.f - omega_f));
+    float f[Q];
+    invert_phys(f, avg, SF, g.eq_kind);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) fc[q * cqs + ccell] = Store<S>::from_real(typename Arith<S>::R(f[q]), q);
+}
+
+} // namespace lbm

@@ -100,6 +100,49 @@ void orc_macro_f64(const orc_params* p, const double* f, double* rho, double* u,
 uint32_t orc_hash_u32(uint64_t seed, uint64_t counter);
 void orc_inlet_profile(int ny, int nz, float u0, float amp, uint64_t seed, float* u_in);
 
+/* ---- 2:1 local refinement (MRTtest/DynInlet3.cu:2202-2560, :3180-3245) ----
+ * Fine node (i,j,k) sits at coarse coordinates (a_x + 0.25 + 0.5 i, ...); the
+ * coarse footprint is nodes a .. a + n_f/2 per axis (n_f/2 + 1 of them).
+ * Physical moments (PhysicalMoments, :396-407): rho, jx, jy, jz, 3pxx-row,
+ * pww-row, pxy, pyz, pxz; stored as 9 planes over the footprint. */
+typedef struct {
+    int a[3];           /* footprint origin (coarse integer coordinates) */
+    int nf[3];          /* fine nodes per axis (even) */
+    double omega_c, omega_f;
+    double cs;          /* reference CS, used by the LES-aware rescale */
+    int temporal;       /* 1: frame at t+1/2 for the second sub-step */
+    int eq_kind;        /* higher-moment equilibrium of the rebuilt populations:
+                           ORC_EQ_BGK_MEQ, ORC_EQ_BGK_PROJ or ORC_EQ_MRT_MEQ */
+} orc_refine;
+
+enum {
+    ORC_EQ_BGK_MEQ = 0,  /* bgk_meq (:347-365): the LES-family collide's own equilibrium */
+    ORC_EQ_BGK_PROJ = 1, /* the BGK equilibrium's projection (3pixx = -3pxx/2): BGK's own */
+    ORC_EQ_MRT_MEQ = 2   /* mrt_meq (:329-345), as InvertPhysicalMoments codes it (:437-481) */
+};
+
+void orc_phys_moments_f32(const float f[19], float mom[9]);
+void orc_strain_rate_f32(const float mom[9], float S[6]);
+void orc_invert_phys_les_cf_f32(float f[19], const float mom[9], float omega_c, float cs, int eq_kind);
+void orc_invert_phys_les_fc_f32(float f[19], const float mom[9], float omega_f, float cs, int eq_kind);
+/* order-2 correction (:2312-2357): adds to mom[1..3]; fx etc. in {0.25, 0.75}
+ * for CF, or cf_weights=0 (the FC form, weights 1) */
+void orc_order2_correction_f32(const float corner_mom[8][9], float mom[9], float xf, float yf, float zf,
+                               int fc_form);
+/* extract the footprint moments of a coarse lattice (dense) */
+void orc_extract_f32(const orc_params* pc, const orc_refine* r, const float* fc, float* M);
+/* coarse -> fine frame (2 fine nodes thick on all six faces) */
+void orc_interp_cf_f32(const orc_refine* r, const float* Mold, const float* Mnew, float alpha,
+                       const uint8_t* fmask, float* ff);
+/* fine -> coarse nodes a+2 .. a+n_f/2-2 */
+void orc_interp_fc_f32(const orc_params* pc, const orc_refine* r, const float* ff, const uint8_t* cmask,
+                       float* fc);
+/* one coarse step with two fine sub-steps; Mold holds the coarse moments at
+ * the step's start and is replaced by those at its end */
+void orc_refine_step_f32(const orc_params* pc, const orc_params* pf, const orc_refine* r,
+                         const uint8_t* cmask, const uint8_t* fmask, const float* u_in,
+                         float* ca, float* cb, float* fa, float* fb, float* Mold, float* Mnew);
+
 #ifdef __cplusplus
 }
 #endif

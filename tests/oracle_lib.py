@@ -156,3 +156,77 @@ def inlet_profile(ny, nz, u0, amp, seed):
     out = np.empty(ny * nz, np.float32)
     lib().orc_inlet_profile(ny, nz, u0, amp, seed, out)
     return out
+
+
+# ---- refinement ----
+class OrcRefine(C.Structure):
+    _fields_ = [("a", C.c_int * 3), ("nf", C.c_int * 3), ("omega_c", C.c_double), ("omega_f", C.c_double),
+                ("cs", C.c_double), ("temporal", C.c_int), ("eq_kind", C.c_int)]
+
+
+EQ_BGK_MEQ, EQ_BGK_PROJ, EQ_MRT_MEQ = 0, 1, 2
+
+
+def _bind_refine():
+    L = lib()
+    if getattr(L, "_refine_bound", False):
+        return L
+    f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    L.orc_phys_moments_f32.argtypes = [f32p, f32p]
+    L.orc_strain_rate_f32.argtypes = [f32p, f32p]
+    L.orc_invert_phys_les_cf_f32.argtypes = [f32p, f32p, C.c_float, C.c_float, C.c_int]
+    L.orc_invert_phys_les_fc_f32.argtypes = [f32p, f32p, C.c_float, C.c_float, C.c_int]
+    L.orc_order2_correction_f32.argtypes = [f32p, f32p, C.c_float, C.c_float, C.c_float, C.c_int]
+    L.orc_extract_f32.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcRefine), f32p, f32p]
+    L.orc_interp_cf_f32.argtypes = [C.POINTER(OrcRefine), f32p, f32p, C.c_float, C.c_void_p, f32p]
+    L.orc_interp_fc_f32.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcRefine), f32p, C.c_void_p, f32p]
+    L.orc_refine_step_f32.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcParams), C.POINTER(OrcRefine),
+                                      C.c_void_p, C.c_void_p, C.c_void_p, f32p, f32p, f32p, f32p, f32p, f32p]
+    L._refine_bound = True
+    return L
+
+
+def refine(a, nf, omega_c, omega_f=None, cs=0.0, temporal=1, eq_kind=0):
+    r = OrcRefine()
+    for k in range(3):
+        r.a[k], r.nf[k] = a[k], nf[k]
+    r.omega_c = omega_c
+    r.omega_f = omega_f if omega_f is not None else float(np.float32(2) / (np.float32(1) + np.float32(2) * (np.float32(2) / np.float32(omega_c) - np.float32(1))))
+    r.cs, r.temporal, r.eq_kind = cs, temporal, eq_kind
+    return r
+
+
+def footprint_size(r):
+    return (r.nf[2] // 2 + 1) * (r.nf[1] // 2 + 1) * (r.nf[0] // 2 + 1)
+
+
+def extract(pc, r, fc):
+    M = np.zeros(9 * footprint_size(r), np.float32)
+    _bind_refine().orc_extract_f32(C.byref(pc), C.byref(r), np.ascontiguousarray(fc, np.float32), M)
+    return M
+
+
+def interp_cf(r, Mold, Mnew, alpha, ff, fmask=None):
+    ff = np.ascontiguousarray(ff, np.float32).copy()
+    _bind_refine().orc_interp_cf_f32(C.byref(r), Mold, Mnew, alpha, _ptr(fmask), ff)
+    return ff
+
+
+def interp_fc(pc, r, ff, fc, cmask=None):
+    fc = np.ascontiguousarray(fc, np.float32).copy()
+    _bind_refine().orc_interp_fc_f32(C.byref(pc), C.byref(r), np.ascontiguousarray(ff, np.float32), _ptr(cmask), fc)
+    return fc
+
+
+def refine_run(pc, pf, r, ca, fa, nsteps, cmask=None, fmask=None, u_in=None):
+    """Advance coarse (ca) and fine (fa) lattices nsteps coarse steps."""
+    L = _bind_refine()
+    ca = np.ascontiguousarray(ca, np.float32).copy()
+    fa = np.ascontiguousarray(fa, np.float32).copy()
+    cb, fb = np.empty_like(ca), np.empty_like(fa)
+    Mold = extract(pc, r, ca)
+    Mnew = np.empty_like(Mold)
+    for _ in range(nsteps):
+        L.orc_refine_step_f32(C.byref(pc), C.byref(pf), C.byref(r), _ptr(cmask), _ptr(fmask), _ptr(u_in),
+                              ca, cb, fa, fb, Mold, Mnew)
+    return ca, fa

@@ -1,0 +1,169 @@
+"""Pins the oracle's coarse-fine interface restatement (oracle/refine_impl.c,
+following MRTtest/DynInlet3.cu:367-595 and :2202-2560) with closed forms:
+  * a uniform equilibrium state is reproduced exactly by CF and FC;
+  * a linear velocity field is reproduced exactly (trilinear weights, and the
+    ORDER-2 correction vanishes for linear fields: every term is a second
+    difference over the 8 corners);
+  * moments linear in time give the exact midpoint at alpha = 1/2;
+  * with CS = 0 the LES-aware rescale reduces to the plain
+    SF_cf = LRFACTOR omega_c (1 - omega_f) / ((1 - omega_c) omega_f) of main()
+    (MRTtest/DynInlet3.cu:2878), and FC's SF to its inverse;
+  * StrainRate(m) = -3/2 (Pi_neq) (omega = 1 as coded) against the definition.
+"""
+import numpy as np
+
+import oracle_lib as O
+
+C = np.array([(0, 0, 0), (1, 0, 0), (0, 1, 0), (-1, 0, 0), (0, -1, 0), (1, 1, 0), (-1, 1, 0), (-1, -1, 0),
+              (1, -1, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1), (-1, 0, 1), (0, -1, 1), (0, 0, -1), (1, 0, -1),
+              (0, 1, -1), (-1, 0, -1), (0, -1, -1)])
+
+
+def mrt_eq(rho, u, v, w):
+    """The fixed point of the LES-family collide: M^-1 bgk_meq (BGK feq with
+    3pixx = -meq9/6), which CF/FC rebuild with eq_kind = EQ_BGK_MEQ."""
+    f = O.feq_bgk(rho, u, v, w, dtype=np.float64)
+    m9 = 2 * u * u - (v * v + w * w)
+    col10 = O.moment_matrix()[10] / 72.0
+    return (f + (-m9 / 6 + m9 / 2) * col10).astype(np.float32)
+
+
+def coarse_field(n, fn):
+    nx, ny, nz = n
+    f = np.empty((19, nz, ny, nx), np.float32)
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                f[:, z, y, x] = mrt_eq(*fn(x, y, z))
+    return f
+
+
+def test_phys_moments_and_strain_rate():
+    rng = np.random.default_rng(1)
+    L = O._bind_refine()
+    M = O.moment_matrix()
+    for _ in range(10):
+        f = (O.feq_bgk(1.0, *rng.uniform(-0.05, 0.05, 3)) + rng.uniform(-1e-3, 1e-3, 19)).astype(np.float32)
+        mom = np.zeros(9, np.float32)
+        L.orc_phys_moments_f32(f, mom)
+        m = M @ f.astype(np.float64)
+        assert np.allclose(mom, m[[0, 3, 5, 7, 9, 11, 13, 14, 15]], atol=2e-7)
+        S = np.zeros(6, np.float32)
+        L.orc_strain_rate_f32(mom, S)
+        j = C.T @ f.astype(np.float64)
+        # Pi_neq deviatoric parts from the definition, with the MRT (mrt_meq) equilibrium
+        P = np.einsum("ia,ib,i->ab", C, C, f.astype(np.float64))
+        Pneq = P - np.outer(j, j)
+        tr = np.trace(Pneq)
+        dev = Pneq - tr / 3 * np.eye(3)
+        # S = -1.5 * (deviatoric Pi_neq) in the reference's scaling
+        assert np.allclose(S[[0, 1, 2]], -1.5 * np.diag(dev), atol=3e-7)
+        assert np.allclose(S[[3, 4, 5]], -1.5 * np.array([Pneq[0, 1], Pneq[1, 2], Pneq[0, 2]]), atol=3e-7)
+
+
+def setup(n=(20, 16, 16), a=(3, 2, 2), nf=(16, 12, 12), omega=1.6, fn=None):
+    pc = O.params(*n, bc=(O.CLAMP,) * 3, collision=O.MRT_LES, omega=omega)
+    r = O.refine(a, nf, omega)
+    fc = coarse_field(n, fn)
+    return pc, r, fc
+
+
+def test_cf_fc_reproduce_uniform_and_linear_fields():
+    def lin(x, y, z):
+        return (1.0 + 1e-4 * x, 0.04 + 1e-3 * y, -0.01 + 5e-4 * z, 0.002 * x - 0.001 * z)
+
+    def lin_rho1(x, y, z):
+        rho, u, v, w = lin(x, y, z)
+        return (1.0, u, v, w)
+
+    for fn in (lambda x, y, z: (1.01, 0.03, -0.01, 0.02), lin_rho1):
+        pc, r, fc = setup(fn=fn)
+        M = O.extract(pc, r, fc)
+        nf = tuple(r.nf)
+        ff = np.zeros((19, nf[2], nf[1], nf[0]), np.float32)
+        out = O.interp_cf(r, M, M, 0.0, ff)
+        for k in range(nf[2]):
+            for j in range(nf[1]):
+                for i in range(nf[0]):
+                    frame = not (2 <= i < nf[0] - 2 and 2 <= j < nf[1] - 2 and 2 <= k < nf[2] - 2)
+                    if not frame:
+                        assert np.all(out[:, k, j, i] == 0)
+                        continue
+                    X, Y, Z = r.a[0] + 0.25 + 0.5 * i, r.a[1] + 0.25 + 0.5 * j, r.a[2] + 0.25 + 0.5 * k
+                    exp = mrt_eq(*fn(X, Y, Z))
+                    assert np.abs(out[:, k, j, i] - exp).max() < 3e-7
+        # FC of the exact fine field gives the coarse equilibrium back
+        fine = np.empty_like(ff)
+        for k in range(nf[2]):
+            for j in range(nf[1]):
+                for i in range(nf[0]):
+                    fine[:, k, j, i] = mrt_eq(*fn(r.a[0] + 0.25 + 0.5 * i, r.a[1] + 0.25 + 0.5 * j,
+                                                  r.a[2] + 0.25 + 0.5 * k))
+        back = O.interp_fc(pc, r, fine, np.zeros_like(fc))
+        for z in range(r.a[2] + 2, r.a[2] + nf[2] // 2 - 1):
+            for y in range(r.a[1] + 2, r.a[1] + nf[1] // 2 - 1):
+                for x in range(r.a[0] + 2, r.a[0] + nf[0] // 2 - 1):
+                    assert np.abs(back[:, z, y, x] - fc[:, z, y, x]).max() < 3e-7
+        assert np.all(back[:, :, :, : r.a[0] + 2] == 0)
+
+
+def test_temporal_midpoint_is_exact():
+    """alpha = 1/2 blends the stored coarse moments (linear in time) before
+    the nonlinear rescale: equal to CF of the averaged moments, and different
+    from averaging the two CF results."""
+    pc, r, fc0 = setup(fn=lambda x, y, z: (1.0, 0.02, 0.0, 0.0))
+    fc1 = coarse_field((20, 16, 16), lambda x, y, z: (1.0, 0.04 + 1e-3 * x, 0.01, 0.0))
+    M0, M1 = O.extract(pc, r, fc0), O.extract(pc, r, fc1)
+    Mm = (0.5 * M0 + 0.5 * M1).astype(np.float32)
+    nf = tuple(r.nf)
+    ff = np.zeros((19, nf[2], nf[1], nf[0]), np.float32)
+    a = O.interp_cf(r, M0, M1, 0.5, ff)
+    b = O.interp_cf(r, Mm, Mm, 0.0, ff)
+    assert np.abs(a - b).max() < 1e-7
+    c = 0.5 * O.interp_cf(r, M0, M0, 0.0, ff) + 0.5 * O.interp_cf(r, M1, M1, 0.0, ff)
+    assert np.abs(a - c).max() > 1e-6
+
+
+def test_les_rescale_reduces_to_reference_sf():
+    """With CS = 0, CF's rescale multiplies the non-equilibrium second moments
+    by SF_cf and FC's by 1/SF_cf (MRTtest/DynInlet3.cu:2878-2879)."""
+    L = O._bind_refine()
+    om = np.float32(1.7)
+    om_f = np.float32(2) / (np.float32(1) + np.float32(2) * (np.float32(2) / om - np.float32(1)))
+    sf = 0.5 * om * (1 - om_f) / ((1 - om) * om_f)
+    rng = np.random.default_rng(3)
+    M = O.moment_matrix()
+    mom = np.array([1.0, 0.03, -0.01, 0.02, 0, 0, 0, 0, 0], np.float32)
+    u, v, w = mom[1:4]
+    eq = np.array([2 * u * u - (v * v + w * w), v * v - w * w, u * v, v * w, u * w])
+    neq = rng.uniform(-1e-3, 1e-3, 5).astype(np.float32)
+    mom[4:] = eq + neq
+    for fn, s in ((L.orc_invert_phys_les_cf_f32, sf), (L.orc_invert_phys_les_fc_f32, 1 / sf)):
+        f = np.zeros(19, np.float32)
+        fn(f, mom, om if fn is L.orc_invert_phys_les_cf_f32 else om_f, 0.0, O.EQ_MRT_MEQ)
+        m = M @ f.astype(np.float64)
+        assert np.allclose(m[[0, 3, 5, 7]], mom[:4], atol=1e-7)
+        assert np.allclose(m[[9, 11, 13, 14, 15]] - eq, s * neq, rtol=2e-4, atol=2e-8)
+        # as coded: eps = +475/63 j^2 and 3pixx = piww = 0 (mrt_meq)
+        assert abs(m[2] - 475 / 63 * (u * u + v * v + w * w)) < 1e-6 and abs(m[10]) < 1e-7 and abs(m[12]) < 1e-7
+
+
+def test_refined_run_conserves_uniform_flow():
+    """A uniform stream at the collide's fixed point stays exactly uniform
+    through the refined region when the interface rebuilds populations with
+    the collide's own equilibrium (eq_kind = EQ_BGK_MEQ); with mrt_meq as
+    InvertPhysicalMoments codes it, the interface injects an eps jump of
+    ~3 rho that streaming turns into density errors next to the frame."""
+    n = (24, 16, 16)
+    pc = O.params(*n, bc=(O.PERIODIC,) * 3, collision=O.MRT_LES, omega=1.5, cs=0.01)
+    r = O.refine((4, 3, 3), (20, 16, 16), 1.5, cs=0.01)
+    pf = O.params(*r.nf, bc=(O.CLAMP,) * 3, collision=O.MRT_LES, omega=r.omega_f, cs=0.01)
+    fc = coarse_field(n, lambda x, y, z: (1.0, 0.03, 0.01, -0.005))
+    ff = coarse_field(tuple(r.nf), lambda x, y, z: (1.0, 0.03, 0.01, -0.005))
+    ca, fa = O.refine_run(pc, pf, r, fc, ff, 5)
+    assert np.abs(ca - fc).max() < 2e-6 and np.abs(fa - ff).max() < 2e-6
+    r.eq_kind = O.EQ_MRT_MEQ
+    ca, fa = O.refine_run(pc, pf, r, fc, ff, 5)
+    pfine = O.params(*r.nf)
+    rho = O.macro(pfine, fa)[0]
+    assert np.abs(rho - 1.0).max() > 1e-3
