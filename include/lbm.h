@@ -193,6 +193,63 @@ const char* lbm_last_error(lbm_ctx* ctx);
 
 void lbm_destroy(lbm_ctx* ctx);
 
+/*
+ * Running velocity statistics, fused into the stream-collide (replaces
+ * vel_av / vel_fluc and their LR forms, MRTtest/DynInlet3.cu:1139-1184 and
+ * :1773-1799, :2153-2170, switched by VELAV / START_VELAV / START_VELFLUC,
+ * :58-60).  From step start_av each collided node keeps the running mean of
+ * its post-collision velocity sum c_i f_i; from step start_fluc the mean is
+ * frozen and the running mean of (u - mean)^2 is kept.  The fine level
+ * weighs each sub-step by LRFACTOR = 1/2 as the reference does.
+ */
+int lbm_stats_enable(lbm_ctx* ctx, long long start_av, long long start_fluc);
+/* mean[3][nz][ny][nx] (u, v, w), dev[3][nz][ny][nx]; either may be NULL */
+int lbm_stats_get(lbm_ctx* ctx, int level, float* mean, float* dev);
+
+/*
+ * Learned eddy-viscosity closure (no reference counterpart: the reference's
+ * closure is constant-Cs Smagorinsky, MRTtest/DynInlet3.cu:628-735).  A per
+ * node MLP 7 -> 64 -> 64 -> 64 -> 1 (GELU, softplus output) with BF16 weights
+ * and FP32 accumulation on the matrix cores; trained by its own HIP forward,
+ * backward and Adam kernels; inference fused into the collide.
+ * Parameter vector (FP32, 8897): W1[7][64] b1[64] W2[64][64] b2[64]
+ * W3[64][64] b3[64] w4[64] b4.
+ */
+typedef struct lbm_mlp lbm_mlp;
+
+int lbm_mlp_create(uint64_t seed, lbm_mlp** out);
+void lbm_mlp_destroy(lbm_mlp* m);
+const char* lbm_mlp_last_error(lbm_mlp* m);
+int lbm_mlp_num_params(void);
+int lbm_mlp_set_params(lbm_mlp* m, const float* params);
+int lbm_mlp_get_params(lbm_mlp* m, float* params);
+/* input standardisation (x - mu)/sd, the training range [lo, hi] of each raw
+ * input (outside it the collide uses the Smagorinsky rate) and the output
+ * scale: nu_t = nu_scale * softplus(z) */
+int lbm_mlp_set_norm(lbm_mlp* m, const float* mu, const float* sd, const float* lo, const float* hi,
+                     float nu_scale);
+/* training data on the device: X[n][7] raw inputs, y[n] targets (nu_t) */
+int lbm_mlp_set_data(lbm_mlp* m, const float* X, const float* y, long long n);
+/* gradient of the loss mean((softplus(z) - y/nu_scale)^2) (the squared
+ * error of nu_t in units of nu_scale) over samples [offset, offset+count),
+ * FP64 loss reduction; grad (host, 8897) may be NULL */
+int lbm_mlp_gradient(lbm_mlp* m, long long offset, int count, float* grad, double* loss);
+/* Adam over consecutive batches of the data set in its stored order
+ * (cycling; shuffle before lbm_mlp_set_data); loss_hist[steps] (may be NULL) */
+int lbm_mlp_train(lbm_mlp* m, int steps, int batch, float lr, double* loss_hist);
+/* inference on raw inputs X[n][7] -> nu[n] */
+int lbm_mlp_forward(lbm_mlp* m, const float* X, long long n, float* nu);
+
+/* attach (or, with m = NULL, detach) the closure to an MRT-LES context: its
+ * collide uses tau = tau0 + 3 nu_t, nu_t capped at nu_cap */
+int lbm_set_closure(lbm_ctx* ctx, lbm_mlp* m, float nu_cap);
+/* training samples from a context's current state, box-filtered by 2 (the
+ * solver's own finer run): features X[n][7] and the SGS eddy viscosity
+ * y[n] = max(0, -tau_dev:S/(2 S:S)) of a hashed fraction `keep` of the
+ * filtered cells, in cell order */
+int lbm_closure_samples(lbm_ctx* ctx, uint64_t seed, float keep, float* X, float* y, int max_samples,
+                        int* count);
+
 #ifdef __cplusplus
 }
 #endif

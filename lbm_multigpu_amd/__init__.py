@@ -62,7 +62,15 @@ EXPORTS = [
     "lbm_set_populations", "lbm_set_populations_f64", "lbm_get_forces",
     "lbm_get_mass_momentum", "lbm_write_tecplot", "lbm_query_sizes", "lbm_get_time",
     "lbm_last_error", "lbm_destroy",
+    "lbm_mlp_create", "lbm_mlp_destroy", "lbm_mlp_last_error", "lbm_mlp_num_params",
+    "lbm_mlp_set_params", "lbm_mlp_get_params", "lbm_mlp_set_norm", "lbm_mlp_set_data",
+    "lbm_mlp_gradient", "lbm_mlp_train", "lbm_mlp_forward", "lbm_set_closure",
+    "lbm_closure_samples", "lbm_stats_enable", "lbm_stats_get",
 ]
+
+# closure network: 7 -> 64 -> 64 -> 64 -> 1, flat FP32 parameter vector
+MLP_IN, MLP_H = 7, 64
+MLP_NPARAM = MLP_IN * MLP_H + MLP_H + 2 * (MLP_H * MLP_H + MLP_H) + MLP_H + 1
 
 _lib = None
 
@@ -106,6 +114,23 @@ def load_library(path=LIB_PATH):
     lib.lbm_hash_u32.restype = C.c_uint32
     lib.lbm_inlet_profile.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, f32]
     lib.lbm_inlet_profile.restype = None
+    lib.lbm_stats_enable.argtypes = [P, C.c_longlong, C.c_longlong]
+    lib.lbm_stats_get.argtypes = [P, C.c_int, P, P]
+    lib.lbm_mlp_create.argtypes = [C.c_uint64, C.POINTER(P)]
+    lib.lbm_mlp_destroy.argtypes = [P]
+    lib.lbm_mlp_destroy.restype = None
+    lib.lbm_mlp_last_error.argtypes = [P]
+    lib.lbm_mlp_last_error.restype = C.c_char_p
+    lib.lbm_mlp_num_params.argtypes = []
+    lib.lbm_mlp_set_params.argtypes = [P, f32]
+    lib.lbm_mlp_get_params.argtypes = [P, f32]
+    lib.lbm_mlp_set_norm.argtypes = [P, f32, f32, f32, f32, C.c_float]
+    lib.lbm_mlp_set_data.argtypes = [P, f32, f32, C.c_longlong]
+    lib.lbm_mlp_gradient.argtypes = [P, C.c_longlong, C.c_int, P, C.POINTER(C.c_double)]
+    lib.lbm_mlp_train.argtypes = [P, C.c_int, C.c_int, C.c_float, P]
+    lib.lbm_mlp_forward.argtypes = [P, f32, C.c_longlong, f32]
+    lib.lbm_set_closure.argtypes = [P, P, C.c_float]
+    lib.lbm_closure_samples.argtypes = [P, C.c_uint64, C.c_float, f32, f32, C.c_int, C.POINTER(C.c_int)]
     _lib = lib
     return lib
 
@@ -127,7 +152,10 @@ class Solver:
     """One liblbm context.  Methods mirror the C ABI one to one."""
 
     def __init__(self, cfg):
+        stats = None
         if isinstance(cfg, Config):
+            if cfg.velav:
+                stats = (cfg.start_velav, max(cfg.start_velfluc, cfg.start_velav))
             cfg = cfg.to_c()
         self.lib = load_library()
         self.cfg = cfg
@@ -136,6 +164,8 @@ class Solver:
         if rc != 0:
             raise LBMError(rc, self.lib.lbm_last_error(None).decode())
         self.h = h
+        if stats:
+            self.stats_enable(*stats)
 
     def _check(self, rc):
         if rc != 0:
@@ -235,3 +265,99 @@ class Solver:
         t = C.c_longlong()
         self._check(self.lib.lbm_get_time(self.h, C.byref(t)))
         return t.value
+
+    def stats_enable(self, start_av, start_fluc):
+        """Running velocity mean from step start_av, mean squared deviation
+        from step start_fluc (the reference's VELAV statistics)."""
+        self._check(self.lib.lbm_stats_enable(self.h, start_av, start_fluc))
+
+    def stats(self, level=0):
+        """(mean[3, nz, ny, nx], dev[3, nz, ny, nx]) of u, v, w."""
+        shp = (3,) + self.shape(level)
+        mean = np.empty(shp, np.float32)
+        dev = np.empty(shp, np.float32)
+        self._check(self.lib.lbm_stats_get(self.h, level, mean.ctypes.data, dev.ctypes.data))
+        return mean, dev
+
+    def set_closure(self, mlp, nu_cap=0.05):
+        """Attach a trained Mlp (None detaches): the collide then takes its
+        eddy viscosity from the network, Smagorinsky outside its range."""
+        self._closure = mlp  # keep the network alive while attached
+        self._check(self.lib.lbm_set_closure(self.h, None if mlp is None else mlp.h, nu_cap))
+
+    def closure_samples(self, seed, keep, max_samples):
+        """Features X[n, 7] and eddy-viscosity targets y[n] from this run,
+        box-filtered by 2, at a hashed fraction `keep` of the filtered cells."""
+        X = np.empty((max_samples, MLP_IN), np.float32)
+        y = np.empty(max_samples, np.float32)
+        n = C.c_int()
+        self._check(self.lib.lbm_closure_samples(self.h, seed, keep, X.reshape(-1), y, max_samples, C.byref(n)))
+        return X[: n.value].copy(), y[: n.value].copy()
+
+
+class Mlp:
+    """The closure network on the GPU (include/lbm.h lbm_mlp_*)."""
+
+    def __init__(self, seed=0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.lbm_mlp_create(seed, C.byref(h))
+        if rc != 0:
+            raise LBMError(rc, self.lib.lbm_mlp_last_error(None).decode())
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise LBMError(rc, self.lib.lbm_mlp_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.lbm_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        out = np.empty(MLP_NPARAM, np.float32)
+        self._check(self.lib.lbm_mlp_get_params(self.h, out))
+        return out
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32).reshape(-1)
+        if p.size != MLP_NPARAM:
+            raise ValueError(f"expected {MLP_NPARAM} parameters, got {p.size}")
+        self._check(self.lib.lbm_mlp_set_params(self.h, p))
+
+    def set_norm(self, mu, sd, lo, hi, nu_scale):
+        a = [np.ascontiguousarray(v, dtype=np.float32).reshape(MLP_IN) for v in (mu, sd, lo, hi)]
+        self._check(self.lib.lbm_mlp_set_norm(self.h, *a, nu_scale))
+
+    def set_data(self, X, y):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32).reshape(-1)
+        if X.shape != (y.size, MLP_IN):
+            raise ValueError("X must be [n, 7] and y [n]")
+        self._check(self.lib.lbm_mlp_set_data(self.h, X.reshape(-1), y, y.size))
+
+    def gradient(self, offset, count):
+        g = np.empty(MLP_NPARAM, np.float32)
+        loss = C.c_double()
+        self._check(self.lib.lbm_mlp_gradient(self.h, offset, count, g.ctypes.data, C.byref(loss)))
+        return g, loss.value
+
+    def train(self, steps, batch, lr):
+        hist = np.empty(steps, np.float64)
+        self._check(self.lib.lbm_mlp_train(self.h, steps, batch, lr, hist.ctypes.data if steps else None))
+        return hist
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.empty(X.shape[0], np.float32)
+        self._check(self.lib.lbm_mlp_forward(self.h, X.reshape(-1), X.shape[0], out))
+        return out

@@ -24,6 +24,7 @@ REFERENCE_KEYS = [
 EXTENSION_KEYS = [
     "MRT rates:", "Boundary x:", "Boundary y:", "Boundary z:", "Halfway bounce-back:", "Storage:",
     "Geometry:", "Inlet perturbation:", "Seed:", "Temporal interpolation:", "Force steps:",
+    "Statistics:",
 ]
 
 _BC = {"periodic": 0, "clamp": 1, "wall": 2}
@@ -50,6 +51,7 @@ class Config:
     startf: int = 0             # "STARTF"
     start_velav: int = 0
     start_velfluc: int = 0
+    velav: bool = False         # VELAV: running velocity statistics (MRTtest/DynInlet3.cu:58)
     halfway: bool = False
     storage: str = "f32"
     geometry: str = "none"
@@ -195,10 +197,12 @@ def read_inputs(path_or_text):
            "Storage:": ("storage", str), "Geometry:": ("geometry", str),
            "Inlet perturbation:": ("inlet_amp", float), "Seed:": ("seed", int),
            "Temporal interpolation:": ("temporal_interp", lambda s: bool(int(s))),
-           "Force steps:": ("force_steps", int)}
+           "Force steps:": ("force_steps", int), "Statistics:": ("velav", lambda s: bool(int(s)))}
     for k, (attr, typ) in ext.items():
         if k in seen:
             setattr(cfg, attr, typ(seen[k].strip()))
+    if "START_VELAV:" in seen and "Statistics:" not in seen:
+        cfg.velav = True  # the reference programs that print START_VELAV build with VELAV 1
     bc = list(cfg.bc)
     for a, k in enumerate(("Boundary x:", "Boundary y:", "Boundary z:")):
         if k in seen:
@@ -255,6 +259,7 @@ def write_inputs(cfg, path=None):
         f"Seed: \t{cfg.seed}",
         f"Temporal interpolation: \t{int(cfg.temporal_interp)}",
         f"Force steps: \t{cfg.force_steps}",
+        f"Statistics: \t{int(cfg.velav)}",
     ]
     text = "\n".join(lines) + "\n"
     if path:

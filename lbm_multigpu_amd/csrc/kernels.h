@@ -44,6 +44,11 @@ struct StepParams {
     int halfway;
     float omega, cs, dpdy, u_lid;
     float s1, s2, s4, s10, s16;
+    // running velocity statistics: 6 planes of qs floats (mean u v w, then
+    // mean squared deviations); mode 0 off, 1 mean, 2 deviations
+    float* stat;
+    int stat_mode;
+    float stat_n, stat_w; // samples so far (t - start) and this sample's weight
 };
 
 // XCD-aware renumbering: the hardware id hw goes to XCD hw % 8; give XCD j
@@ -93,26 +98,24 @@ __device__ __forceinline__ R wave_sum(R v)
     return v;
 }
 
-template <int MODEL, class S>
-__device__ __forceinline__ void node_boundary_collide(const StepParams& p, int x, int y, int z, int cell,
-                                                      typename Arith<S>::R f[Q]);
+enum NodeAction : int { ACT_SKIP = 0, ACT_STORE = 1, ACT_COLLIDE = 2 };
 
-// Fused pull stream + boundary dispatch + collide + bounce-back for one node
-// (the work of update_inn, MRTtest/DynInlet3.cu:1612-1840, and of the CPU
-// sweep CPU_3D/CPU_3D.cpp:1320-1443).
-template <int MODEL, class S>
-__device__ __forceinline__ void node_update(const StepParams& p, int x, int y, int z,
-                                            typename Arith<S>::R& fx, typename Arith<S>::R& fy,
-                                            typename Arith<S>::R& fz)
+// Pull stream + bounce-back + boundary conditions for one node (the work of
+// update_inn before mrt_collide, MRTtest/DynInlet3.cu:1612-1771, and of the
+// CPU sweep CPU_3D/CPU_3D.cpp:1320-1420).  Leaves in f the populations to
+// collide (ACT_COLLIDE), or the reversed ones of a node bounce-back solid
+// (ACT_STORE), or nothing (ACT_SKIP: a solid under link bounce-back).
+template <class S>
+__device__ __forceinline__ int node_pull(const StepParams& p, int x, int y, int z, int cell,
+                                         typename Arith<S>::R f[Q], typename Arith<S>::R& fx,
+                                         typename Arith<S>::R& fy, typename Arith<S>::R& fz)
 {
     using R = typename Arith<S>::R;
     const S* __restrict__ fin = static_cast<const S*>(p.fin);
-    S* __restrict__ fout = static_cast<S*>(p.fout);
     const long long qs = p.qs;
-    const int cell = x + p.nxp * (y + p.ny * z);
     const unsigned char code = p.mask ? *at(p.mask, cell) : (unsigned char)0;
     const unsigned char base = code & 0x7f;
-    if (p.halfway && is_solid(base)) return; // inert under link bounce-back
+    if (p.halfway && is_solid(base)) return ACT_SKIP; // inert under link bounce-back
 
     int xm, xp, ym, yp, zm, zp;
     bool oxm, oxp, oym, oyp, ozm, ozp;
@@ -128,7 +131,6 @@ __device__ __forceinline__ void node_update(const StepParams& p, int x, int y, i
 #pragma unroll
         for (int b = 0; b < 3; ++b) row[a][b] = p.nxp * (Y[a] + p.ny * Z[b]);
 
-    R f[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const int src = row[1 - CY[i]][1 - CZ[i]] + X[1 - CX[i]];
@@ -160,8 +162,8 @@ __device__ __forceinline__ void node_update(const StepParams& p, int x, int y, i
 
     if (is_solid(base)) {
         // node bounce-back (MRTtest/DynInlet3.cu:1673-1705): the pulled values
-        // are stored reversed (a register permutation, so every store below
-        // still writes plane i from a wave-uniform base); momentum exchange
+        // are stored reversed (a register permutation, so every store still
+        // writes plane i from a wave-uniform base); momentum exchange
         // 2 sum c_i f_i on im 10 nodes (:1675-1687)
         if (p.fpart && base == NODE_SOLID_FORCE) {
 #pragma unroll
@@ -176,23 +178,9 @@ __device__ __forceinline__ void node_update(const StepParams& p, int x, int y, i
         for (int i = 0; i < Q; ++i) g[i] = f[OPP[i]];
 #pragma unroll
         for (int i = 0; i < Q; ++i) f[i] = g[i];
-    } else {
-        node_boundary_collide<MODEL, S>(p, x, y, z, cell, f);
+        return ACT_STORE;
     }
 
-#pragma unroll
-    for (int i = 0; i < Q; ++i) Store<S>::store(at(fout + i * qs, cell), f[i], i);
-}
-
-// Boundary conditions of the non-solid node codes, then the collide.
-template <int MODEL, class S>
-__device__ __forceinline__ void node_boundary_collide(const StepParams& p, int x, int y, int z, int cell,
-                                                      typename Arith<S>::R f[Q])
-{
-    using R = typename Arith<S>::R;
-    const S* __restrict__ fin = static_cast<const S*>(p.fin);
-    const long long qs = p.qs;
-    const unsigned char base = p.mask ? (*at(p.mask, cell) & 0x7f) : (unsigned char)0;
     if (base == NODE_INLET_WEST || base == NODE_OUTLET_EAST) {
         // Extrap conditions start from the unstreamed neighbour (:1725-1746)
         const int nb = base == NODE_INLET_WEST ? (x + 1 < p.nx ? cell + 1 : cell)
@@ -204,13 +192,73 @@ __device__ __forceinline__ void node_boundary_collide(const StepParams& p, int x
     } else if (base == NODE_LID_WEST_REG) {
         lid_west_reg(f, R(p.u_lid));
     }
+    return ACT_COLLIDE;
+}
 
+template <class R>
+__device__ __forceinline__ CollideParams<R> collide_params(const StepParams& p)
+{
     CollideParams<R> cp;
     cp.omega = R(p.omega);
     cp.cs = R(p.cs);
     cp.dpdy = R(p.dpdy);
     cp.s1 = R(p.s1); cp.s2 = R(p.s2); cp.s4 = R(p.s4); cp.s10 = R(p.s10); cp.s16 = R(p.s16);
-    collide<MODEL>(f, cp);
+    return cp;
+}
+
+template <class S>
+__device__ __forceinline__ void node_store(const StepParams& p, int cell, const typename Arith<S>::R f[Q])
+{
+    S* __restrict__ fout = static_cast<S*>(p.fout);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) Store<S>::store(at(fout + i * p.qs, cell), f[i], i);
+}
+
+// Running statistics of the post-collision velocity sum c_i f_i at one node:
+// vel_av / vel_fluc (MRTtest/DynInlet3.cu:1139-1148, :1159-1172) on the
+// coarse level, weight 1; vel_avLR / vel_flucLR (:1150-1157, :1174-1184) on
+// the fine level, weight LRFACTOR.  mean <- (mean n + u w) / (n + w); from the
+// deviation phase on the mean is frozen and dev <- (dev n + (u - mean)^2 w) /
+// (n + w).  Sums in the reference's order, no contraction.
+template <class R>
+__device__ __forceinline__ void stats_update(const StepParams& p, int cell, const R f[Q])
+{
+#pragma clang fp contract(off)
+    const float u[3] = {float(f[1] - f[3] + f[5] - f[6] - f[7] + f[8] + f[10] - f[12] + f[15] - f[17]),
+                        float(f[2] - f[4] + f[5] + f[6] - f[7] - f[8] + f[11] - f[13] + f[16] - f[18]),
+                        float(f[9] + f[10] + f[11] + f[12] + f[13] - f[14] - f[15] - f[16] - f[17] - f[18])};
+    const float n = p.stat_n, w = p.stat_w;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float* mean = at(p.stat + a * p.qs, cell);
+        if (p.stat_mode == 1) {
+            *mean = (*mean * n + u[a] * w) / (n + w);
+        } else {
+            float* dev = at(p.stat + (3 + a) * p.qs, cell);
+            const float m = *mean;
+            const float d = (u[a] - m) * (u[a] - m);
+            *dev = (*dev * n + d * w) / (n + w);
+        }
+    }
+}
+
+// Fused pull stream + boundary dispatch + collide + bounce-back for one node.
+template <int MODEL, class S>
+__device__ __forceinline__ void node_update(const StepParams& p, int x, int y, int z,
+                                            typename Arith<S>::R& fx, typename Arith<S>::R& fy,
+                                            typename Arith<S>::R& fz)
+{
+    using R = typename Arith<S>::R;
+    const int cell = x + p.nxp * (y + p.ny * z);
+    R f[Q];
+    const int act = node_pull<S>(p, x, y, z, cell, f, fx, fy, fz);
+    if (act == ACT_SKIP) return;
+    if (act == ACT_COLLIDE) {
+        const CollideParams<R> cp = collide_params<R>(p);
+        collide<MODEL>(f, cp);
+        if (p.stat_mode) stats_update(p, cell, f);
+    }
+    node_store<S>(p, cell, f);
 }
 
 template <int MODEL, class S>

@@ -17,6 +17,7 @@
 #include "../../include/lbm.h"
 #include "kernels.h"
 #include "refine.h"
+#include "closure.h"
 
 using namespace lbm;
 
@@ -34,6 +35,7 @@ struct Level {
     void* fb = nullptr; // next state
     unsigned char* mask = nullptr; // device, padded
     std::vector<unsigned char> host_mask; // dense codes
+    float* stat = nullptr;         // running statistics, 6 planes of qs
     int blocks = 0;
     size_t bytes = 0;
 };
@@ -54,6 +56,7 @@ struct lbm_ctx {
     double* mpart = nullptr;   // mass partials
     long long t = 0;
     int force_recorded = 0;
+    long long stat_av = -1, stat_fluc = -1; // statistics phases (off while stat_av < 0)
     bool geometry_set = false;
     bool initialised = false;
     // local refinement
@@ -63,6 +66,29 @@ struct lbm_ctx {
     int* cf_blocks = nullptr;  // origins of the 8^3 fine blocks that meet the frame
     int n_cf = 0;
     int fc_nb[3] = {0, 0, 0};  // FC workgroups per axis (4^3 coarse nodes each)
+    // learned closure
+    lbm_mlp* mlp = nullptr;    // not owned
+    float* dwall = nullptr;    // wall distance per node (padded layout)
+    float nu_cap = 0.05f;
+    std::string err;
+};
+
+struct lbm_mlp {
+    float* params = nullptr;   // FP32 master parameters
+    float* m = nullptr;        // Adam moments
+    float* v = nullptr;
+    float* grad = nullptr;
+    short* wbf = nullptr;      // BF16 copies for the matrix cores
+    float* X = nullptr;        // device training data
+    float* y = nullptr;
+    long long n = 0;
+    float* partials = nullptr; // per-tile gradients
+    double* lossp = nullptr;
+    double* dloss = nullptr;
+    int max_tiles = 0;
+    lbm::MlpNorm nrm{};
+    long long t = 0;           // Adam step
+    hipStream_t stream = nullptr;
     std::string err;
 };
 
@@ -219,8 +245,10 @@ static void free_level(Level& L)
     if (L.fa) (void)hipFree(L.fa);
     if (L.fb) (void)hipFree(L.fb);
     if (L.mask) (void)hipFree(L.mask);
+    if (L.stat) (void)hipFree(L.stat);
     L.fa = L.fb = nullptr;
     L.mask = nullptr;
+    L.stat = nullptr;
 }
 
 extern "C" int lbm_create(const lbm_config* cfg, lbm_ctx** out)
@@ -295,7 +323,7 @@ extern "C" void lbm_destroy(lbm_ctx* ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (int l = 0; l < 2; ++l) free_level(ctx->lv[l]);
     for (void* p : {(void*)ctx->u_in, (void*)ctx->fpart, (void*)ctx->fhist, (void*)ctx->red, (void*)ctx->mpart,
-                    (void*)ctx->Mold, (void*)ctx->Mnew, (void*)ctx->cf_blocks})
+                    (void*)ctx->Mold, (void*)ctx->Mnew, (void*)ctx->cf_blocks, (void*)ctx->dwall})
         if (p) (void)hipFree(p);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -516,6 +544,16 @@ static StepParams step_params(lbm_ctx* ctx, const Level& L, float omega)
     p.dpdy = c.dpdy;
     p.u_lid = c.umax;
     p.s1 = c.s_dhum[1]; p.s2 = c.s_dhum[2]; p.s4 = c.s_dhum[4]; p.s10 = c.s_dhum[10]; p.s16 = c.s_dhum[16];
+    p.stat = L.stat;
+    p.stat_mode = 0;
+    p.stat_n = 0.f;
+    p.stat_w = 1.f;
+    if (L.stat && ctx->stat_av >= 0 && ctx->t >= ctx->stat_av) {
+        const bool dev = ctx->t >= ctx->stat_fluc;
+        p.stat_mode = dev ? 2 : 1;
+        p.stat_n = (float)(ctx->t - (dev ? ctx->stat_fluc : ctx->stat_av));
+        p.stat_w = (&L == &ctx->lv[0]) ? 1.f : 0.5f; // LRFACTOR on the fine level
+    }
     return p;
 }
 
@@ -531,6 +569,22 @@ static void launch_sc(int model, const StepParams& p, int blocks, hipStream_t s)
     }
 }
 
+template <class S>
+static void launch_closure(lbm_ctx* ctx, const StepParams& p, int blocks)
+{
+    ClosureArgs ca;
+    ca.params = ctx->mlp->params;
+    ca.wbf = ctx->mlp->wbf;
+    ca.dwall = ctx->dwall;
+    ca.nrm = ctx->mlp->nrm;
+    ca.nu_cap = ctx->nu_cap;
+    const dim3 blk(TX, TY), grd(blocks);
+    if (ctx->cfg.model == LBM_MODEL_MRT_DHUMIERES)
+        hipLaunchKernelGGL((k_stream_collide_closure<MODEL_MRT_DHUM, S>), grd, blk, 0, ctx->stream, p, ca);
+    else
+        hipLaunchKernelGGL((k_stream_collide_closure<MODEL_MRT_LES, S>), grd, blk, 0, ctx->stream, p, ca);
+}
+
 // One coarse-level stream-collide launch (update_inn/top/bot on one GPU),
 // plus the deterministic force reduction when this step is recorded.
 static int coarse_step(lbm_ctx* ctx)
@@ -542,7 +596,8 @@ static int coarse_step(lbm_ctx* ctx)
     if (rec) p.fpart = ctx->fpart;
     with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        launch_sc<S>(c.model, p, L.blocks, ctx->stream);
+        if (ctx->mlp) launch_closure<S>(ctx, p, L.blocks);
+        else launch_sc<S>(c.model, p, L.blocks, ctx->stream);
         return 0;
     });
     if (rec) {
@@ -906,5 +961,405 @@ extern "C" int lbm_write_tecplot(lbm_ctx* ctx, const char* casename)
                 }
         if (!out) return fail(ctx, LBM_EIO, "lbm_write_tecplot: write failed for " + name);
     }
+    return LBM_OK;
+}
+
+// ---------------------------------------------------------------- statistics
+
+// Running velocity statistics (VELAV, START_VELAV, START_VELFLUC,
+// MRTtest/DynInlet3.cu:58-60): enabling allocates and zeroes the six
+// per-node arrays of every level.
+extern "C" int lbm_stats_enable(lbm_ctx* ctx, long long start_av, long long start_fluc)
+{
+    if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_stats_enable: null context");
+    if (start_av < 0 || start_fluc < start_av)
+        return fail(ctx, LBM_EINVAL, "lbm_stats_enable: need 0 <= start_av <= start_fluc");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    for (int l = 0; l < ctx->nlevels; ++l) {
+        Level& L = ctx->lv[l];
+        const size_t bytes = (size_t)6 * L.qs * sizeof(float);
+        if (!L.stat) HIP_TRY(ctx, hipMalloc(&L.stat, bytes));
+        HIP_TRY(ctx, hipMemsetAsync(L.stat, 0, bytes, ctx->stream));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->stat_av = start_av;
+    ctx->stat_fluc = start_fluc;
+    return LBM_OK;
+}
+
+// mean[3][nz][ny][nx] (u v w) and dev[3][nz][ny][nx] (mean squared
+// deviations); either may be NULL
+extern "C" int lbm_stats_get(lbm_ctx* ctx, int level, float* mean, float* dev)
+{
+    if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_stats_get: null context");
+    if (level < 0 || level >= ctx->nlevels) return fail(ctx, LBM_EINVAL, "lbm_stats_get: no such level");
+    Level& L = ctx->lv[level];
+    if (!L.stat) return fail(ctx, LBM_ESTATE, "lbm_stats_get: statistics not enabled");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<float> plane((size_t)L.qs);
+    for (int k = 0; k < 6; ++k) {
+        float* out = k < 3 ? mean : dev;
+        if (!out) continue;
+        HIP_TRY(ctx, hipMemcpy(plane.data(), L.stat + (size_t)k * L.qs, plane.size() * 4, hipMemcpyDeviceToHost));
+        float* o = out + (size_t)(k % 3) * L.cells;
+        for (int z = 0; z < L.nz; ++z)
+            for (int y = 0; y < L.ny; ++y)
+                std::memcpy(o + (size_t)L.nx * (y + (size_t)L.ny * z), plane.data() + (size_t)L.nxp * (y + (size_t)L.ny * z),
+                            (size_t)L.nx * 4);
+    }
+    return LBM_OK;
+}
+
+// ---------------------------------------------------------------- closure
+
+// Distance (cells) from each node to the nearest solid node or wall face:
+// solid nodes 0, faces of LBM_BC_WALL axes half a cell outside the domain,
+// then a two-pass 3-D chamfer transform (steps 1, sqrt2, sqrt3).
+static std::vector<float> wall_distance(const Level& L, const int bc[3])
+{
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    std::vector<float> d(L.cells);
+    const float big = 1e9f;
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const size_t c = (size_t)x + (size_t)nx * (y + (size_t)ny * z);
+                float v = big;
+                if (is_solid(L.host_mask[c])) v = 0.f;
+                if (bc[0] == LBM_BC_WALL) v = std::min(v, std::min(x + 0.5f, nx - 0.5f - x));
+                if (bc[1] == LBM_BC_WALL) v = std::min(v, std::min(y + 0.5f, ny - 0.5f - y));
+                if (bc[2] == LBM_BC_WALL) v = std::min(v, std::min(z + 0.5f, nz - 0.5f - z));
+                d[c] = v;
+            }
+    const float w[4] = {0.f, 1.f, 1.41421356f, 1.73205081f};
+    for (int pass = 0; pass < 2; ++pass) {
+        const int s = pass == 0 ? 1 : -1;
+        for (int zi = 0; zi < nz; ++zi)
+            for (int yi = 0; yi < ny; ++yi)
+                for (int xi = 0; xi < nx; ++xi) {
+                    const int x = pass == 0 ? xi : nx - 1 - xi;
+                    const int y = pass == 0 ? yi : ny - 1 - yi;
+                    const int z = pass == 0 ? zi : nz - 1 - zi;
+                    const size_t c = (size_t)x + (size_t)nx * (y + (size_t)ny * z);
+                    float v = d[c];
+                    for (int dz = -1; dz <= 0; ++dz)
+                        for (int dy = -1; dy <= 1; ++dy)
+                            for (int dx = -1; dx <= 1; ++dx) {
+                                if (dz == 0 && (dy > 0 || (dy == 0 && dx >= 0))) continue; // causal half
+                                const int X = x + s * dx, Y = y + s * dy, Z = z + s * dz;
+                                if (X < 0 || Y < 0 || Z < 0 || X >= nx || Y >= ny || Z >= nz) continue;
+                                const float nv = d[(size_t)X + (size_t)nx * (Y + (size_t)ny * Z)] +
+                                                 w[std::abs(dx) + std::abs(dy) + std::abs(dz)];
+                                v = std::min(v, nv);
+                            }
+                    d[c] = v;
+                }
+    }
+    return d;
+}
+
+static int mfail(lbm_mlp* m, int code, const std::string& msg)
+{
+    if (m) m->err = msg;
+    g_last_error = msg;
+    return code;
+}
+
+#define MLP_TRY(m, expr)                                                                   \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return mfail(m, LBM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" int lbm_mlp_create(uint64_t seed, lbm_mlp** out)
+{
+    if (!out) return mfail(nullptr, LBM_EINVAL, "lbm_mlp_create: null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return mfail(nullptr, LBM_EDEVICE, "lbm_mlp_create: no HIP device visible");
+    lbm_mlp* m = new lbm_mlp();
+    auto bail = [&](int rc) { lbm_mlp_destroy(m); return rc; };
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(mfail(nullptr, LBM_EDEVICE, "lbm_mlp_create: stream creation failed"));
+    const size_t P = MLP_NPARAM;
+    const size_t nwbf = MLP_INP * MLP_H + 2 * MLP_H * MLP_H;
+    if (hipMalloc(&m->params, P * 4) || hipMalloc(&m->m, P * 4) || hipMalloc(&m->v, P * 4) ||
+        hipMalloc(&m->grad, P * 4) || hipMalloc(&m->wbf, nwbf * 2) || hipMalloc(&m->dloss, 8))
+        return bail(mfail(nullptr, LBM_ENOMEM, "lbm_mlp_create: device allocation failed"));
+    // Glorot-uniform weights from the counter hash, zero biases
+    std::vector<float> p(P, 0.f);
+    auto fill = [&](int off, int fin, int fout, uint64_t salt) {
+        const float a = std::sqrt(6.f / (fin + fout));
+        for (int i = 0; i < fin * fout; ++i) {
+            const float r = (float)(hash_u32(seed, salt * 1000003ULL + i) >> 8) * (1.0f / 16777216.0f);
+            p[off + i] = a * (2.f * r - 1.f);
+        }
+    };
+    fill(OFF_W1, MLP_IN, MLP_H, 1);
+    fill(OFF_W2, MLP_H, MLP_H, 2);
+    fill(OFF_W3, MLP_H, MLP_H, 3);
+    fill(OFF_W4, MLP_H, 1, 4);
+    MlpNorm n{};
+    for (int k = 0; k < MLP_IN; ++k) { n.mu[k] = 0.f; n.inv_sd[k] = 1.f; n.lo[k] = -1e30f; n.hi[k] = 1e30f; }
+    n.nu_scale = 1.f;
+    m->nrm = n;
+    *out = m;
+    int rc = lbm_mlp_set_params(m, p.data());
+    if (rc) { *out = nullptr; return bail(rc); }
+    return LBM_OK;
+}
+
+extern "C" void lbm_mlp_destroy(lbm_mlp* m)
+{
+    if (!m) return;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    for (void* q : {(void*)m->params, (void*)m->m, (void*)m->v, (void*)m->grad, (void*)m->wbf, (void*)m->X,
+                    (void*)m->y, (void*)m->partials, (void*)m->lossp, (void*)m->dloss})
+        if (q) (void)hipFree(q);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+extern "C" const char* lbm_mlp_last_error(lbm_mlp* m) { return m ? m->err.c_str() : g_last_error.c_str(); }
+
+extern "C" int lbm_mlp_num_params(void) { return MLP_NPARAM; }
+
+extern "C" int lbm_mlp_set_params(lbm_mlp* m, const float* params)
+{
+    if (!m || !params) return mfail(m, LBM_EINVAL, "lbm_mlp_set_params: null argument");
+    const size_t P = MLP_NPARAM;
+    MLP_TRY(m, hipMemcpyAsync(m->params, params, P * 4, hipMemcpyHostToDevice, m->stream));
+    MLP_TRY(m, hipMemsetAsync(m->m, 0, P * 4, m->stream));
+    MLP_TRY(m, hipMemsetAsync(m->v, 0, P * 4, m->stream));
+    const int nwbf = MLP_INP * MLP_H + 2 * MLP_H * MLP_H;
+    hipLaunchKernelGGL(k_mlp_sync_bf16, dim3((nwbf + 255) / 256), dim3(256), 0, m->stream, m->params, m->wbf);
+    MLP_TRY(m, hipGetLastError());
+    MLP_TRY(m, hipStreamSynchronize(m->stream));
+    m->t = 0;
+    return LBM_OK;
+}
+
+extern "C" int lbm_mlp_get_params(lbm_mlp* m, float* params)
+{
+    if (!m || !params) return mfail(m, LBM_EINVAL, "lbm_mlp_get_params: null argument");
+    MLP_TRY(m, hipMemcpyAsync(params, m->params, MLP_NPARAM * 4, hipMemcpyDeviceToHost, m->stream));
+    MLP_TRY(m, hipStreamSynchronize(m->stream));
+    return LBM_OK;
+}
+
+extern "C" int lbm_mlp_set_norm(lbm_mlp* m, const float* mu, const float* sd, const float* lo, const float* hi,
+                                float nu_scale)
+{
+    if (!m || !mu || !sd || !lo || !hi) return mfail(m, LBM_EINVAL, "lbm_mlp_set_norm: null argument");
+    if (!(nu_scale > 0.f)) return mfail(m, LBM_EINVAL, "lbm_mlp_set_norm: nu_scale must be > 0");
+    for (int k = 0; k < MLP_IN; ++k) {
+        if (!(sd[k] > 0.f)) return mfail(m, LBM_EINVAL, "lbm_mlp_set_norm: sd must be > 0");
+        m->nrm.mu[k] = mu[k];
+        m->nrm.inv_sd[k] = 1.f / sd[k];
+        m->nrm.lo[k] = lo[k];
+        m->nrm.hi[k] = hi[k];
+    }
+    m->nrm.nu_scale = nu_scale;
+    return LBM_OK;
+}
+
+extern "C" int lbm_mlp_set_data(lbm_mlp* m, const float* X, const float* y, long long n)
+{
+    if (!m || !X || !y || n <= 0) return mfail(m, LBM_EINVAL, "lbm_mlp_set_data: bad argument");
+    if (m->X) (void)hipFree(m->X);
+    if (m->y) (void)hipFree(m->y);
+    m->X = m->y = nullptr;
+    MLP_TRY(m, hipMalloc(&m->X, (size_t)n * MLP_IN * 4));
+    MLP_TRY(m, hipMalloc(&m->y, (size_t)n * 4));
+    MLP_TRY(m, hipMemcpyAsync(m->X, X, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice, m->stream));
+    MLP_TRY(m, hipMemcpyAsync(m->y, y, (size_t)n * 4, hipMemcpyHostToDevice, m->stream));
+    MLP_TRY(m, hipStreamSynchronize(m->stream));
+    m->n = n;
+    return LBM_OK;
+}
+
+static int mlp_grad_launch(lbm_mlp* m, long long offset, int count)
+{
+    const int tiles = (count + 63) / 64;
+    if (tiles > m->max_tiles) {
+        if (m->partials) (void)hipFree(m->partials);
+        if (m->lossp) (void)hipFree(m->lossp);
+        m->partials = nullptr;
+        m->lossp = nullptr;
+        MLP_TRY(m, hipMalloc(&m->partials, (size_t)tiles * MLP_NPARAM * 4));
+        MLP_TRY(m, hipMalloc(&m->lossp, (size_t)tiles * 8));
+        m->max_tiles = tiles;
+    }
+    hipLaunchKernelGGL(k_mlp_grad, dim3(tiles), dim3(64), 0, m->stream, m->X, m->y, (long long)count, offset,
+                       m->params, m->wbf, m->nrm, 1.f / (float)count, m->partials, m->lossp);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, m->stream, m->partials, tiles,
+                       m->grad, m->lossp, m->dloss);
+    MLP_TRY(m, hipGetLastError());
+    return LBM_OK;
+}
+
+extern "C" int lbm_mlp_gradient(lbm_mlp* m, long long offset, int count, float* grad, double* loss)
+{
+    if (!m || !m->X) return mfail(m, LBM_ESTATE, "lbm_mlp_gradient: no data set");
+    if (offset < 0 || count <= 0 || offset + count > m->n) return mfail(m, LBM_EINVAL, "lbm_mlp_gradient: bad range");
+    int rc = mlp_grad_launch(m, offset, count);
+    if (rc) return rc;
+    if (grad) MLP_TRY(m, hipMemcpyAsync(grad, m->grad, MLP_NPARAM * 4, hipMemcpyDeviceToHost, m->stream));
+    double l = 0;
+    MLP_TRY(m, hipMemcpyAsync(&l, m->dloss, 8, hipMemcpyDeviceToHost, m->stream));
+    MLP_TRY(m, hipStreamSynchronize(m->stream));
+    if (loss) *loss = l / count;
+    return LBM_OK;
+}
+
+// Adam on mean-squared error over consecutive batches of the data set
+// (cycling); loss_hist[i] receives the batch loss of step i (FP64 sums).
+extern "C" int lbm_mlp_train(lbm_mlp* m, int steps, int batch, float lr, double* loss_hist)
+{
+    if (!m || !m->X) return mfail(m, LBM_ESTATE, "lbm_mlp_train: no data set");
+    if (steps < 0 || batch <= 0 || batch > m->n || !(lr > 0.f)) return mfail(m, LBM_EINVAL, "lbm_mlp_train: bad argument");
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const long long nb = m->n / batch;
+    for (int s = 0; s < steps; ++s) {
+        const long long offset = (m->t % nb) * batch;
+        int rc = mlp_grad_launch(m, offset, batch);
+        if (rc) return rc;
+        m->t += 1;
+        const float bc1 = 1.f - std::pow(b1, (float)m->t), bc2 = 1.f - std::pow(b2, (float)m->t);
+        hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, m->stream, m->params, m->m, m->v,
+                           m->grad, lr, b1, b2, eps, bc1, bc2, m->wbf);
+        if (loss_hist) {
+            double l = 0;
+            MLP_TRY(m, hipMemcpyAsync(&l, m->dloss, 8, hipMemcpyDeviceToHost, m->stream));
+            MLP_TRY(m, hipStreamSynchronize(m->stream));
+            loss_hist[s] = l / batch;
+        }
+    }
+    MLP_TRY(m, hipGetLastError());
+    MLP_TRY(m, hipStreamSynchronize(m->stream));
+    return LBM_OK;
+}
+
+extern "C" int lbm_mlp_forward(lbm_mlp* m, const float* X, long long n, float* nu)
+{
+    if (!m || !X || !nu || n <= 0) return mfail(m, LBM_EINVAL, "lbm_mlp_forward: bad argument");
+    float *dx = nullptr, *dn = nullptr;
+    MLP_TRY(m, hipMalloc(&dx, (size_t)n * MLP_IN * 4));
+    MLP_TRY(m, hipMalloc(&dn, (size_t)n * 4));
+    hipError_t e = hipMemcpyAsync(dx, X, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice, m->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, dx, n,
+                           m->params, m->wbf, m->nrm, dn);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(nu, dn, (size_t)n * 4, hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    (void)hipFree(dx);
+    (void)hipFree(dn);
+    if (e != hipSuccess) return mfail(m, LBM_EDEVICE, std::string("lbm_mlp_forward: ") + hipGetErrorString(e));
+    return LBM_OK;
+}
+
+extern "C" int lbm_set_closure(lbm_ctx* ctx, lbm_mlp* m, float nu_cap)
+{
+    if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_set_closure: null context");
+    if (!m) { ctx->mlp = nullptr; return LBM_OK; }
+    if (ctx->cfg.model != LBM_MODEL_MRT_LES && ctx->cfg.model != LBM_MODEL_MRT_DHUMIERES)
+        return fail(ctx, LBM_EINVAL, "lbm_set_closure: the closure replaces the LES rate of an MRT-LES model");
+    if (ctx->nlevels > 1) return fail(ctx, LBM_EINVAL, "lbm_set_closure: not combined with refinement");
+    if (!ctx->geometry_set) {
+        int rc = lbm_set_geometry(ctx, nullptr);
+        if (rc) return rc;
+    }
+    Level& L = ctx->lv[0];
+    const std::vector<float> d = wall_distance(L, ctx->cfg.bc);
+    std::vector<float> pad((size_t)L.qs, 1.f);
+    for (int z = 0; z < L.nz; ++z)
+        for (int y = 0; y < L.ny; ++y)
+            for (int x = 0; x < L.nx; ++x)
+                pad[(size_t)x + (size_t)L.nxp * (y + (size_t)L.ny * z)] = d[(size_t)x + (size_t)L.nx * (y + (size_t)L.ny * z)];
+    if (!ctx->dwall) HIP_TRY(ctx, hipMalloc(&ctx->dwall, pad.size() * 4));
+    HIP_TRY(ctx, hipMemcpy(ctx->dwall, pad.data(), pad.size() * 4, hipMemcpyHostToDevice));
+    ctx->mlp = m;
+    ctx->nu_cap = nu_cap > 0.f ? nu_cap : 0.05f;
+    return LBM_OK;
+}
+
+// Training samples from this context's current state: box filter by 2,
+// features and SGS eddy viscosity at a hashed selection (fraction `keep`) of
+// the coarse cells, written in cell order.
+extern "C" int lbm_closure_samples(lbm_ctx* ctx, uint64_t seed, float keep, float* X, float* y, int max_samples,
+                                   int* count)
+{
+    if (!ctx || !X || !y || !count || max_samples <= 0) return fail(ctx, LBM_EINVAL, "lbm_closure_samples: bad argument");
+    if (ctx->nlevels > 1) return fail(ctx, LBM_EINVAL, "lbm_closure_samples: single-level runs only");
+    if (!ctx->geometry_set) {
+        int rc = lbm_set_geometry(ctx, nullptr);
+        if (rc) return rc;
+    }
+    Level& L = ctx->lv[0];
+    if (L.nx % 2 || L.ny % 2 || L.nz % 2) return fail(ctx, LBM_EINVAL, "lbm_closure_samples: even domain needed");
+    const int cx = L.nx / 2, cy = L.ny / 2, cz = L.nz / 2;
+    const size_t NC = (size_t)cx * cy * cz;
+    // coarse wall distance: the fine distance at the coarse cell centre, in coarse cells
+    const std::vector<float> dfine = wall_distance(L, ctx->cfg.bc);
+    std::vector<float> dc(NC);
+    for (int k = 0; k < cz; ++k)
+        for (int j = 0; j < cy; ++j)
+            for (int i = 0; i < cx; ++i) {
+                float s = 0.f;
+                for (int d = 0; d < 8; ++d)
+                    s += dfine[(size_t)(2 * i + (d & 1)) + (size_t)L.nx * ((2 * j + ((d >> 1) & 1)) + (size_t)L.ny * (2 * k + ((d >> 2) & 1)))];
+                dc[(size_t)i + (size_t)cx * (j + (size_t)cy * k)] = 0.125f * s * 0.5f;
+            }
+    float *bx = nullptr, *dd = nullptr, *dX = nullptr, *dy = nullptr;
+    unsigned int* di = nullptr;
+    unsigned long long* dcount = nullptr;
+    auto cleanup = [&]() {
+        for (void* q : {(void*)bx, (void*)dd, (void*)dX, (void*)dy, (void*)di, (void*)dcount})
+            if (q) (void)hipFree(q);
+    };
+    if (hipMalloc(&bx, NC * 10 * 4) || hipMalloc(&dd, NC * 4) || hipMalloc(&dX, (size_t)max_samples * MLP_IN * 4) ||
+        hipMalloc(&dy, (size_t)max_samples * 4) || hipMalloc(&di, (size_t)max_samples * 4) || hipMalloc(&dcount, 8)) {
+        cleanup();
+        return fail(ctx, LBM_ENOMEM, "lbm_closure_samples: device allocation failed");
+    }
+    hipError_t e = hipMemcpyAsync(dd, dc.data(), NC * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(dcount, 0, 8, ctx->stream);
+    if (e == hipSuccess) {
+        with_storage(L.storage, [&](auto* tag) {
+            using S = std::remove_pointer_t<decltype(tag)>;
+            hipLaunchKernelGGL(k_box_filter<S>, dim3((unsigned)((NC + 255) / 256)), dim3(256), 0, ctx->stream,
+                               (const S*)L.fa, L.qs, L.nxp, L.nx, L.ny, L.nz, L.mask, bx);
+            return 0;
+        });
+        const uint32_t kp = (uint32_t)std::min(4294967295.0, std::max(0.0, (double)keep * 4294967296.0));
+        hipLaunchKernelGGL(k_closure_features, dim3((unsigned)((NC + 255) / 256)), dim3(256), 0, ctx->stream, bx, cx, cy,
+                           cz, dd, seed, kp, dX, dy, di, dcount, (unsigned long long)max_samples);
+        e = hipGetLastError();
+    }
+    unsigned long long n = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&n, dcount, 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    const int got = (int)std::min<unsigned long long>(n, (unsigned long long)max_samples);
+    std::vector<float> hx((size_t)got * MLP_IN), hy(got);
+    std::vector<unsigned int> hi(got);
+    if (e == hipSuccess && got > 0) {
+        e = hipMemcpy(hx.data(), dX, hx.size() * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(hy.data(), dy, hy.size() * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(hi.data(), di, hi.size() * 4, hipMemcpyDeviceToHost);
+    }
+    cleanup();
+    if (e != hipSuccess) return fail(ctx, LBM_EDEVICE, std::string("lbm_closure_samples: ") + hipGetErrorString(e));
+    std::vector<int> order(got);
+    for (int i = 0; i < got; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return hi[a] < hi[b]; });
+    for (int i = 0; i < got; ++i) {
+        for (int k = 0; k < MLP_IN; ++k) X[(size_t)i * MLP_IN + k] = hx[(size_t)order[i] * MLP_IN + k];
+        y[i] = hy[order[i]];
+    }
+    *count = got;
     return LBM_OK;
 }

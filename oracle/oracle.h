@@ -80,6 +80,8 @@ void orc_step_f32(const orc_params* p, const uint8_t* mask, const float* u_in,
 void orc_run_f32(const orc_params* p, const uint8_t* mask, const float* u_in,
                  float* fa, float* fb, int nsteps, double* forces);
 void orc_macro_f32(const orc_params* p, const float* f, float* rho, float* u, float* v, float* w);
+void orc_stats_update_f32(const orc_params* p, const uint8_t* mask, const float* f, float* stat, int mode,
+                          float n, float w);
 
 /* ---- double restatement ---- */
 void orc_moments_f64(const double f[19], double m[19]);
@@ -95,6 +97,8 @@ void orc_step_f64(const orc_params* p, const uint8_t* mask, const double* u_in,
 void orc_run_f64(const orc_params* p, const uint8_t* mask, const double* u_in,
                  double* fa, double* fb, int nsteps, double* forces);
 void orc_macro_f64(const orc_params* p, const double* f, double* rho, double* u, double* v, double* w);
+void orc_stats_update_f64(const orc_params* p, const uint8_t* mask, const double* f, float* stat, int mode,
+                          float n, float w);
 
 /* seeded inlet perturbation: counter-based hash of (seed, cell), no state */
 uint32_t orc_hash_u32(uint64_t seed, uint64_t counter);

@@ -388,3 +388,36 @@ void FN(orc_macro)(const orc_params* p, const R* f, R* rho, R* u, R* v, R* w)
         if (w) w[c] = d;
     }
 }
+
+/* Running velocity statistics after one step (vel_av / vel_fluc,
+ * MRTtest/DynInlet3.cu:1139-1148, :1159-1172, applied after mrt_collide in
+ * update_inn, :1773-1799; LR forms :1150-1157, :1174-1184 with weight w =
+ * LRFACTOR).  f holds the step's output: at a non-solid node these are its
+ * post-collision populations.  stat: six planes of nx*ny*nz floats (mean u,
+ * v, w; mean squared deviations); mode 1 updates the means, mode 2 the
+ * deviations against the frozen means; n = t - START of the phase. */
+void FN(orc_stats_update)(const orc_params* p, const uint8_t* mask, const R* f, float* stat, int mode,
+                          float n, float w)
+{
+    const size_t N = (size_t)p->nx * p->ny * p->nz;
+    for (size_t c = 0; c < N; ++c) {
+        if (mask && FN(is_solid)(mask[c])) continue;
+        const R* g = f + c;
+#define F_(i) g[(size_t)(i) * N]
+        const float u[3] = {
+            (float)(F_(1) - F_(3) + F_(5) - F_(6) - F_(7) + F_(8) + F_(10) - F_(12) + F_(15) - F_(17)),
+            (float)(F_(2) - F_(4) + F_(5) + F_(6) - F_(7) - F_(8) + F_(11) - F_(13) + F_(16) - F_(18)),
+            (float)(F_(9) + F_(10) + F_(11) + F_(12) + F_(13) - F_(14) - F_(15) - F_(16) - F_(17) - F_(18))};
+#undef F_
+        for (int a = 0; a < 3; ++a) {
+            float* mean = stat + (size_t)a * N + c;
+            if (mode == 1) {
+                *mean = (*mean * n + u[a] * w) / (n + w);
+            } else {
+                float* dev = stat + (size_t)(3 + a) * N + c;
+                const float d = (u[a] - *mean) * (u[a] - *mean);
+                *dev = (*dev * n + d * w) / (n + w);
+            }
+        }
+    }
+}

@@ -42,6 +42,8 @@ def lib():
             getattr(_lib, "orc_step_" + suf).argtypes = [C.POINTER(OrcParams), C.c_void_p, C.c_void_p, fp, fp, C.c_void_p]
             getattr(_lib, "orc_run_" + suf).argtypes = [C.POINTER(OrcParams), C.c_void_p, C.c_void_p, fp, fp, C.c_int, C.c_void_p]
             getattr(_lib, "orc_macro_" + suf).argtypes = [C.POINTER(OrcParams), fp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+            getattr(_lib, "orc_stats_update_" + suf).argtypes = [C.POINTER(OrcParams), C.c_void_p, fp, f32p, C.c_int,
+                                                                 C.c_float, C.c_float]
             getattr(_lib, "orc_west_extrap_" + suf).argtypes = [fp, C.c_float if suf == "f32" else C.c_double]
             getattr(_lib, "orc_east_extrap_" + suf).argtypes = [fp, C.c_float if suf == "f32" else C.c_double]
             getattr(_lib, "orc_lid_west_reg_" + suf).argtypes = [fp, C.c_float if suf == "f32" else C.c_double]
@@ -230,3 +232,12 @@ def refine_run(pc, pf, r, ca, fa, nsteps, cmask=None, fmask=None, u_in=None):
         L.orc_refine_step_f32(C.byref(pc), C.byref(pf), C.byref(r), _ptr(cmask), _ptr(fmask), _ptr(u_in),
                               ca, cb, fa, fb, Mold, Mnew)
     return ca, fa
+
+
+def stats_update(p, f, stat, mode, n, w=1.0, mask=None):
+    """One running-statistics update of stat (float32 [6, nz, ny, nx], in
+    place) from a step's output populations f."""
+    assert stat.dtype == np.float32 and stat.flags.c_contiguous
+    f = np.ascontiguousarray(f)
+    suf = "f32" if f.dtype == np.float32 else "f64"
+    getattr(lib(), "orc_stats_update_" + suf)(C.byref(p), _ptr(mask), f, stat.reshape(-1), mode, n, w)

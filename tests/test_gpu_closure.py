@@ -1,0 +1,368 @@
+"""Learned eddy-viscosity closure on the GPU against the NumPy reference
+(tests/mlp_ref.py, itself pinned by finite differences and closed forms in
+tests/test_mlp_ref.py) and the oracle's collide.
+
+Tolerances, from the BF16 arithmetic the kernels use (8-bit mantissa):
+* forward: against the same BF16 rounding points in NumPy, 5e-3 relative
+  (L2 over the batch); against plain FP64, 3e-2;
+* gradient: against the BF16 NumPy backward, 2e-2 relative L2 per layer;
+  against FP64, 8e-2 per layer and cosine similarity >= 0.995 overall;
+* fused collide: one step against the oracle's collide at the per-node rate
+  the NumPy network gives, 1e-5 relative to max|f| (a BF16 rounding flip in
+  the network moves nu_t by ~1e-3 relative); the closure's own effect on the
+  same step is checked to be far above that bar.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import lbm_multigpu_amd as L
+import mlp_ref as M
+import oracle_lib as O
+from lbm_multigpu_amd import config as cfgmod
+from test_gpu_parity import orc_params, random_state
+
+pytestmark = pytest.mark.gpu
+INF = np.float32(3e38)
+# lattice velocities in the solver's order (MRTtest/DynInlet3.cu:603-605)
+CV = np.array([(0, 0, 0), (1, 0, 0), (0, 1, 0), (-1, 0, 0), (0, -1, 0), (1, 1, 0), (-1, 1, 0), (-1, -1, 0),
+               (1, -1, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1), (-1, 0, 1), (0, -1, 1), (0, 0, -1), (1, 0, -1),
+               (0, 1, -1), (-1, 0, -1), (0, -1, -1)], np.float64)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def synthetic_inputs(seed, n):
+    """Raw closure inputs spread like a wake flow's."""
+    rng = np.random.default_rng(seed)
+    X = np.empty((n, 7), np.float32)
+    X[:, 0] = rng.normal(-6.0, 1.0, n)
+    X[:, 1] = rng.uniform(-1, 1, n)
+    X[:, 2:5] = rng.uniform(-0.7, 0.7, (n, 3))
+    X[:, 5] = np.log(rng.uniform(0.5, 30.0, n))
+    X[:, 6] = rng.uniform(0.0, 0.08, n)
+    return X
+
+
+def norm_of(X):
+    mu = X.mean(0).astype(np.float32)
+    sd = X.std(0).astype(np.float32) + np.float32(1e-6)
+    return mu, sd
+
+
+def rl2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_params_roundtrip_and_device_init():
+    m = L.Mlp(seed=9)
+    p0 = m.params
+    W1 = M.unpack(p0)["W1"]
+    a = math.sqrt(6.0 / (7 + 64))
+    assert np.abs(W1).max() <= a and W1.std() > 0.45 * a     # uniform on [-a, a]: std a/sqrt(3)
+    assert np.all(M.unpack(p0)["b2"] == 0)
+    assert np.array_equal(L.Mlp(seed=9).params, p0)          # seeded
+    assert not np.array_equal(L.Mlp(seed=10).params, p0)
+    p = M.glorot(3)
+    m.params = p
+    assert np.array_equal(m.params, p)
+
+
+@pytest.mark.parametrize("n", [1, 1000])
+def test_forward_matches_reference(n):
+    say("mlp forward n", n)
+    X = synthetic_inputs(1, max(n, 64))[:n]
+    mu, sd = norm_of(synthetic_inputs(1, 4096))
+    p = M.glorot(7)
+    nu_scale = 2e-3
+    m = L.Mlp(seed=0)
+    m.params = p
+    m.set_norm(mu, sd, np.full(7, -INF), np.full(7, INF), nu_scale)
+    got = m.forward(X)
+    xn = M.normalise(X, mu, sd)
+    zb, _ = M.forward_bf16(p, xn)
+    z64, _ = M.forward64(p, xn)
+    nb = nu_scale * M.softplus(zb.astype(np.float64))
+    n64 = nu_scale * M.softplus(z64)
+    say("  rel vs bf16 ref", rl2(got, nb), "vs fp64", rl2(got, n64))
+    assert rl2(got, nb) < 5e-3
+    assert rl2(got, n64) < 3e-2
+
+
+def _layers(g):
+    return {k: v.reshape(-1) for k, v in M.unpack(g).items()}
+
+
+@pytest.mark.parametrize("offset,count", [(0, 300), (100, 150), (0, 2048)])
+def test_gradient_matches_reference(offset, count):
+    say("mlp gradient", offset, count)
+    n = 2048
+    X = synthetic_inputs(2, n)
+    mu, sd = norm_of(X)
+    p = M.glorot(11)
+    teacher = M.glorot(12)
+    nu_scale = 1e-3
+    y = (nu_scale * M.softplus(M.forward64(teacher, M.normalise(X, mu, sd))[0])).astype(np.float32)
+    m = L.Mlp()
+    m.params = p
+    m.set_norm(mu, sd, np.full(7, -INF), np.full(7, INF), nu_scale)
+    m.set_data(X, y)
+    g, loss = m.gradient(offset, count)
+    g2, loss2 = m.gradient(offset, count)
+    assert np.array_equal(g, g2) and loss == loss2          # deterministic reduction
+    xs, ys = M.normalise(X[offset:offset + count], mu, sd), y[offset:offset + count]
+    gb, lb = M.grad_bf16(p, xs, ys, nu_scale)
+    g64, l64 = M.grad64(p, xs, ys, nu_scale)
+    say("  loss", loss, lb, l64)
+    assert abs(loss - lb) < 1e-2 * lb and abs(loss - l64) < 5e-2 * l64
+    G, GB, G64 = _layers(g), _layers(gb), _layers(g64)
+    for k in G:
+        say("  ", k, "vs bf16", rl2(G[k], GB[k]), "vs fp64", rl2(G[k], G64[k]))
+        assert rl2(G[k], GB[k]) < 2e-2, k
+        assert rl2(G[k], G64[k]) < 8e-2, k
+    cos = float(g @ g64 / (np.linalg.norm(g) * np.linalg.norm(g64)))
+    assert cos >= 0.995
+
+
+def test_adam_first_step_and_training():
+    say("mlp adam + training")
+    n = 8192
+    X = synthetic_inputs(3, n)
+    mu, sd = norm_of(X)
+    nu_scale = 1e-3
+    teacher = M.glorot(21)
+    y = (nu_scale * M.softplus(M.forward64(teacher, M.normalise(X, mu, sd))[0])).astype(np.float32)
+    m = L.Mlp(seed=4)
+    p0 = m.params
+    m.set_norm(mu, sd, np.full(7, -INF), np.full(7, INF), nu_scale)
+    m.set_data(X, y)
+    lr = 1e-3
+    g0, _ = m.gradient(0, 512)
+    hist = m.train(1, 512, lr)
+    p1 = m.params
+    d = p1.astype(np.float64) - p0
+    # the first Adam step: m/(1-b1) = g, v/(1-b2) = g^2, so each parameter
+    # moves by -lr g / (|g| + eps)
+    g64 = g0.astype(np.float64)
+    want = -lr * g64 / (np.abs(g64) + 1e-8)
+    say("  adam step max dev / lr", np.abs(d - want).max() / lr)
+    assert np.abs(d - want).max() < 2e-3 * lr
+    # training reaches a small fraction of the initial loss
+    hist = np.concatenate([hist, m.train(600, 512, 3e-3)])
+    say("  loss", hist[0], "->", hist[-16:].mean())
+    assert np.all(np.isfinite(hist))
+    assert hist[-16:].mean() < 0.05 * hist[0]
+    # held-out samples from the same distribution
+    Xt = synthetic_inputs(30, 2048)
+    yt = nu_scale * M.softplus(M.forward64(teacher, M.normalise(Xt, mu, sd))[0])
+    pt = m.forward(Xt)
+    assert np.mean((pt - yt) ** 2) < 0.1 * np.mean((yt - yt.mean()) ** 2)
+
+
+def _closure_case():
+    c = L.default_config()
+    c.nx, c.ny, c.nz = 40, 16, 12
+    c.bc[0], c.bc[1], c.bc[2] = L.BC_PERIODIC, L.BC_WALL, L.BC_WALL
+    c.halfway = 1
+    c.model = L.MODEL_MRT_DHUMIERES
+    c.omega = 1 / 0.51
+    c.cs = 0.01
+    return c
+
+
+def _pre_collision(c, f):
+    """Streaming and boundaries of one step with an identity collide (BGK
+    at rate 0 leaves f unchanged)."""
+    p = orc_params(c)
+    p.collision, p.omega, p.cs, p.dpdy = O.BGK, 0.0, 0.0, 0.0
+    fs, _ = O.step(p, f)
+    return fs
+
+
+def _node_features(c, fs):
+    """Closure inputs from the pre-collision populations: S = -3 omega0 /
+    (2 rho) (P - rho/3 I - j j), wall distance, |j|."""
+    Cv = CV
+    f = fs.astype(np.float64)
+    rho = f.sum(0)
+    j = np.einsum("ia,i...->a...", Cv, f)
+    P = np.einsum("ia,ib,i...->ab...", Cv, Cv, f)
+    k = -1.5 * float(np.float32(c.omega)) / rho
+    S = k * (P - rho / 3.0 * np.eye(3)[:, :, None, None, None] - j[:, None] * j[None, :])
+    S = np.moveaxis(S, (0, 1), (-2, -1))
+    nz, ny, nx = rho.shape
+    zz, yy, _ = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    dw = np.minimum(np.minimum(yy + 0.5, ny - 0.5 - yy), np.minimum(zz + 0.5, nz - 0.5 - zz))
+    speed = np.sqrt((j ** 2).sum(0))
+    return M.features(S, dw, speed).astype(np.float32)
+
+
+def _gap_threshold(v, q):
+    """A threshold near quantile q lying in a wide gap of the sorted values."""
+    s = np.sort(v.reshape(-1))
+    i0, i1 = int(len(s) * (q - 0.05)), int(len(s) * (q + 0.05))
+    gaps = np.diff(s[i0:i1])
+    i = i0 + int(np.argmax(gaps))
+    return np.float32(0.5 * (s[i] + s[i + 1])), float(gaps.max())
+
+
+def test_fused_closure_collide_matches_reference():
+    c = _closure_case()
+    say("fused closure collide", c.nx, c.ny, c.nz)
+    rng = np.random.default_rng(5)
+    f = random_state(rng, (c.nz, c.ny, c.nx), amp=2e-3, umax=0.03)
+    fs = _pre_collision(c, f)
+    X = _node_features(c, fs)
+    Xf = X.reshape(-1, 7)
+    mu, sd = norm_of(Xf)
+    lo = np.full(7, -INF, np.float32)
+    hi = np.full(7, INF, np.float32)
+    lo[0], gap = _gap_threshold(Xf[:, 0], 0.3)          # ~30% of the nodes fall back to Smagorinsky
+    assert gap > 1e-4
+    nu_scale = np.float32(0.01)
+    p = M.glorot(31)
+    m = L.Mlp()
+    m.params = p
+    m.set_norm(mu, sd, lo, hi, nu_scale)
+    with L.Solver(c) as s:
+        s.set_populations(f)
+        s.set_closure(m, nu_cap=0.05)
+        s.step(1)
+        g = s.populations()
+    with L.Solver(c) as s:
+        s.set_populations(f)
+        s.step(1)
+        g_smag = s.populations()
+    # reference: the NumPy network's rate at each node, the oracle's collide
+    zb, _ = M.forward_bf16(p, M.normalise(Xf, mu, sd))
+    nut = np.minimum(nu_scale * M.softplus(zb.astype(np.float64)), 0.05).astype(np.float32)
+    inr = np.all((Xf >= lo) & (Xf <= hi), axis=1)
+    om0 = np.float32(c.omega)
+    omega = (np.float32(1) / (np.float32(1) / om0 + np.float32(3) * nut)).astype(np.float32)
+    say("  in range", inr.mean(), "nu_t", nut[inr].min(), nut[inr].max())
+    assert 0.5 < inr.mean() < 0.9
+    pin = orc_params(c)
+    pin.cs = 0.0
+    pout = orc_params(c)
+    ref = np.empty_like(fs)
+    fr = fs.reshape(19, -1)
+    rr = ref.reshape(19, -1)
+    for n in range(fr.shape[1]):
+        if inr[n]:
+            pin.omega = float(omega[n])
+            rr[:, n] = O.collide(pin, fr[:, n].copy())
+        else:
+            rr[:, n] = O.collide(pout, fr[:, n].copy())
+    scale = np.abs(ref).max()
+    err = np.abs(g - ref).max() / scale
+    effect = np.abs(g_smag - ref).max() / scale
+    say("  err", err, "closure effect", effect)
+    assert err < 1e-5
+    assert effect > 50 * 1e-5
+    # nodes outside the range take the Smagorinsky rate: identical to the plain solver there
+    out = ~inr.reshape(fs.shape[1:])
+    assert np.abs(g[:, out] - g_smag[:, out]).max() < 1e-6 * scale
+
+
+def test_closure_out_of_range_is_smagorinsky():
+    c = _closure_case()
+    say("closure fallback")
+    rng = np.random.default_rng(6)
+    f = random_state(rng, (c.nz, c.ny, c.nx), amp=2e-3, umax=0.03)
+    m = L.Mlp(seed=1)
+    m.set_norm(np.zeros(7), np.ones(7), np.full(7, 1e9, np.float32), np.full(7, 2e9, np.float32), 0.01)
+    with L.Solver(c) as s:
+        s.set_populations(f)
+        s.set_closure(m)
+        s.step(3)
+        g = s.populations()
+    with L.Solver(c) as s:
+        s.set_populations(f)
+        s.step(3)
+        h = s.populations()
+    assert np.abs(g - h).max() <= 1e-6 * np.abs(h).max()
+
+
+def test_closure_rejects_unsupported_contexts():
+    m = L.Mlp()
+    c = L.default_config()
+    c.nx, c.ny, c.nz = 16, 8, 8
+    c.model = L.MODEL_BGK
+    with L.Solver(c) as s:
+        with pytest.raises(L.LBMError):
+            s.set_closure(m)
+    with pytest.raises(L.LBMError):
+        m.gradient(0, 10)                # no data set
+    X = synthetic_inputs(0, 100)
+    m.set_data(X, np.zeros(100, np.float32))
+    with pytest.raises(L.LBMError):
+        m.gradient(50, 60)               # range beyond the data
+
+
+def _sampler_reference(f, mask, seed, keep):
+    """Box filter by 2, central differences and the eddy-viscosity target,
+    restated in FP64 from the populations and the node mask."""
+    Cv = CV
+    solid = np.isin(mask, [L.NODE_SOLID, L.NODE_SOLID_FORCE])
+    j = np.einsum("ia,i...->a...", Cv, f.astype(np.float64))
+    j[:, solid] = 0
+    nz, ny, nx = mask.shape
+    cz, cy, cx = nz // 2, ny // 2, nx // 2
+
+    def box(a):
+        return a.reshape(cz, 2, cy, 2, cx, 2).mean(axis=(1, 3, 5))
+
+    ub = np.stack([box(j[a]) for a in range(3)])
+    uu = {(a, b): box(j[a] * j[b]) for a in range(3) for b in range(a, 3)}
+    sf = box(solid.astype(np.float64))
+    dfine = M.wall_distance(solid, (False, True, True)).astype(np.float64)
+    dc = box(dfine) * 0.5
+    X, y, cells = [], [], []
+    for k in range(1, cz - 1):
+        for jj in range(1, cy - 1):
+            for i in range(1, cx - 1):
+                c = i + cx * (jj + cy * k)
+                if M.hash_u32(seed, c) >= keep:
+                    continue
+                if sf[k - 1:k + 2, jj - 1:jj + 2, i - 1:i + 2].max() > 0:
+                    continue
+                g = np.empty((3, 3))
+                for q in range(3):
+                    g[q, 0] = 0.5 * (ub[q, k, jj, i + 1] - ub[q, k, jj, i - 1])
+                    g[q, 1] = 0.5 * (ub[q, k, jj + 1, i] - ub[q, k, jj - 1, i])
+                    g[q, 2] = 0.5 * (ub[q, k + 1, jj, i] - ub[q, k - 1, jj, i])
+                S = 0.5 * (g + g.T)
+                u = ub[:, k, jj, i]
+                t = np.array([[uu[(min(a, b), max(a, b))][k, jj, i] - u[a] * u[b] for b in range(3)] for a in range(3)])
+                t -= np.trace(t) / 3 * np.eye(3)
+                ss = (S * S).sum()
+                nut = max(0.0, -(t * S).sum() / (2 * ss)) if ss > 1e-20 else 0.0
+                X.append(M.features(S, dc[k, jj, i], np.linalg.norm(u)))
+                y.append(nut)
+                cells.append(c)
+    return np.array(X), np.array(y), np.array(cells)
+
+
+def test_sampler_matches_reference():
+    cfg = cfgmod.channel(nx=32, ny=16, nz=16, side=4, x_center=10, tau=0.52, u0=0.05, seed=3)
+    say("closure sampler")
+    rng = np.random.default_rng(8)
+    with L.Solver(cfg) as s:
+        s.set_geometry()
+        mask = s.get_mask()
+        f = random_state(rng, mask.shape, amp=2e-3, umax=0.04)
+        s.set_populations(f)
+        keep = 0.6
+        X, y = s.closure_samples(seed=17, keep=keep, max_samples=4096)
+    Xr, yr, cells = _sampler_reference(f, mask, 17, int(keep * 2 ** 32))
+    say("  samples", len(y), len(yr))
+    assert len(y) == len(yr) and len(y) > 50
+    assert np.abs(X[:, 0] - Xr[:, 0]).max() < 1e-4
+    assert np.abs(X[:, 1:5] - Xr[:, 1:5]).max() < 2e-3
+    assert np.abs(X[:, 5:] - Xr[:, 5:]).max() < 1e-5
+    assert np.abs(y - yr).max() < 1e-3 * np.abs(yr).max() + 1e-9

@@ -284,3 +284,36 @@ def test_inlet_profile_hash_properties():
     r = (h >> np.uint64(8)).astype(np.float32) * np.float32(1 / 16777216)
     t = np.float32(2) * r - np.float32(1)
     assert np.array_equal(prof, np.float32(0.05) + np.float32(1e-3) * t)
+
+
+def test_stats_update_closed_forms():
+    """Running statistics restatement: unit weight gives the arithmetic mean
+    of the sampled momenta, then the mean squared deviation from that frozen
+    mean; a constant sequence stays constant under the fine-level weight 1/2;
+    solid nodes are left alone."""
+    p = O.params(2, 1, 1)
+    rng = np.random.default_rng(4)
+    us = rng.normal(size=(10, 3)).astype(np.float32)
+    stat = np.zeros((6, 1, 1, 2), np.float32)
+    mask = np.array([O.FLUID, O.SOLID], np.uint8).reshape(1, 1, 2)
+
+    def pops(u):
+        f = np.zeros((19, 1, 1, 2), np.float32)
+        f[1], f[2], f[9] = u[0], u[1], u[2]  # c_1 = +x, c_2 = +y, c_9 = +z
+        return f
+
+    for t in range(6):
+        O.stats_update(p, pops(us[t]), stat, 1, float(t), 1.0, mask=mask)
+    mean = us[:6].astype(np.float64).mean(0)
+    assert np.allclose(stat[:3, 0, 0, 0], mean, rtol=1e-5, atol=1e-6)
+    for t in range(6, 10):
+        O.stats_update(p, pops(us[t]), stat, 2, float(t - 6), 1.0, mask=mask)
+    dev = ((us[6:].astype(np.float64) - stat[:3, 0, 0, 0].astype(np.float64)) ** 2).mean(0)
+    assert np.allclose(stat[3:, 0, 0, 0], dev, rtol=1e-5)
+    assert np.all(stat[:, 0, 0, 1] == 0)
+    st2 = np.zeros_like(stat)
+    c = np.array([0.25, -0.5, 0.125], np.float32)
+    for n in range(5):
+        for _ in range(2):
+            O.stats_update(p, pops(c), st2, 1, float(n), 0.5)
+    assert np.array_equal(st2[:3, 0, 0, 0], c)
