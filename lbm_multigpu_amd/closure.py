@@ -1,0 +1,239 @@
+"""Learned eddy-viscosity closure: data generation, training and evaluation.
+
+The closure network (include/lbm.h, lbm_mlp_*) replaces the constant-Cs
+Smagorinsky rate of the reference's MRT-LES collide
+(MRTtest/DynInlet3.cu:628-735).  This module drives it end to end on the
+solver's own data:
+
+1. a fine channel run (2x the target resolution) is advanced past its
+   start-up; every `every` steps the library box-filters it by 2 and returns
+   the closure inputs and the eddy viscosity nu_t = max(0, -tau_dev:S /
+   (2 S:S)) of a hashed fraction of the filtered cells (lbm_closure_samples);
+2. the network is trained on the GPU by the library's own forward, backward
+   and Adam kernels (lbm_mlp_train), with the inputs standardised and their
+   training range recorded (outside it the collide falls back to
+   Smagorinsky);
+3. coarse runs with the learned closure and with Smagorinsky, and Smagorinsky
+   runs at intermediate resolutions, collect the time-averaged velocity with
+   the fused running statistics; each is compared with the fine run's mean
+   at the coarse cell centres, and the GPU time of each run is measured with
+   HIP events.
+
+Every resolution uses the same physical set-up: channel length 4 H, square
+cylinder of side H/8 at x = H, halfway walls, seeded inlet perturbation,
+U = 0.05 (acoustic scaling) and the fine run's viscosity, so
+tau_r = 1/2 + 3 nu_fine r for a grid r times the fine one.
+
+Run: python -m lbm_multigpu_amd.closure [--scale S] [--out FILE]
+"""
+import argparse
+import json
+import math
+import time
+
+import numpy as np
+
+from . import Mlp, Solver, STORE_F32
+from . import config as cfgmod
+
+FEATURES = ("log|S|", "3sqrt6 det(S^)", "S^xy", "S^xz", "S^yz", "log d_wall", "|u|")
+
+
+def channel_at(r, base=(512, 128, 128), tau_fine=0.52, cs_smag=0.1, seed=0):
+    """The channel case at r times the base grid (r = 1: the fine run)."""
+    nx, ny, nz = (int(round(b * r)) for b in base)
+    nu = (tau_fine - 0.5) / 3.0 * r
+    return cfgmod.channel(nx=nx, ny=ny, nz=nz, side=int(round(16 * r * base[1] / 128)),
+                          x_center=int(round(128 * r * base[1] / 128)), tau=0.5 + 3.0 * nu,
+                          cs_smag=cs_smag, seed=seed)
+
+
+def generate_samples(cfg, spinup, snapshots, every, keep=0.02, seed=1, max_per_snapshot=200000,
+                     stats_from=None, log=print):
+    """Advance the fine run and collect box-filtered training samples.
+
+    Returns (X, y, snap_id, solver-state dict with the fine mean velocity
+    when stats_from is given, GPU ms)."""
+    s = Solver(cfg)
+    s.set_geometry()
+    s.init_equilibrium()
+    if stats_from is not None:
+        s.stats_enable(stats_from, 1 << 62)
+    ms = s.step_timed(spinup)
+    Xs, ys, ids = [], [], []
+    for k in range(snapshots):
+        ms += s.step_timed(every)
+        X, y = s.closure_samples(seed + k, keep, max_per_snapshot)
+        Xs.append(X)
+        ys.append(y)
+        ids.append(np.full(len(y), k, np.int32))
+        if (k + 1) % 20 == 0:
+            log(f"  fine snapshot {k + 1}/{snapshots}: {len(y)} samples, t={s.time()}")
+    out = {"mask": s.get_mask()}
+    if stats_from is not None:
+        out["mean"] = s.stats()[0]
+    s.close()
+    return np.concatenate(Xs), np.concatenate(ys), np.concatenate(ids), out, ms
+
+
+def smagorinsky_nu(X, c):
+    """A-priori Smagorinsky viscosity c sqrt(2 S:S) from the inputs
+    (|S_dev| = exp(x0); the solver's CS = Cs^2 plays the role of c)."""
+    return c * math.sqrt(2.0) * np.exp(X[:, 0].astype(np.float64))
+
+
+def fit(X, y, seed=0, steps=3000, batch=4096, lr=2e-3, lr_final=2e-4, log=print):
+    """Train a network on (X, y); returns (mlp, norm dict, loss history)."""
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(len(y))
+    X, y = X[perm], y[perm]
+    mu = X.mean(0).astype(np.float32)
+    sd = (X.std(0) + 1e-6).astype(np.float32)
+    lo, hi = X.min(0).astype(np.float32), X.max(0).astype(np.float32)
+    nu_scale = float(max(y.mean(), 1e-12))
+    m = Mlp(seed)
+    m.set_norm(mu, sd, lo, hi, nu_scale)
+    m.set_data(X, y)
+    hist = []
+    chunks = 10
+    for c in range(chunks):
+        # geometric learning-rate decay in ten stages
+        lr_c = lr * (lr_final / lr) ** (c / max(chunks - 1, 1))
+        h = m.train(steps // chunks, min(batch, len(y)), lr_c)
+        hist.append(h)
+        log(f"  train stage {c + 1}/{chunks}: lr {lr_c:.2e} loss {h[-50:].mean():.4f}")
+    norm = {"mu": mu.tolist(), "sd": sd.tolist(), "lo": lo.tolist(), "hi": hi.tolist(), "nu_scale": nu_scale}
+    return m, norm, np.concatenate(hist)
+
+
+def mean_velocity(cfg, spinup, avg_steps, mlp=None):
+    """Time-averaged velocity of a run after its start-up; returns (mean
+    u[3, nz, ny, nx], mask, GPU ms of the whole run, ms per step)."""
+    s = Solver(cfg)
+    s.set_geometry()
+    s.init_equilibrium()
+    if mlp is not None:
+        s.set_closure(mlp)
+    s.stats_enable(spinup, 1 << 62)
+    ms = s.step_timed(spinup)
+    ms2 = s.step_timed(avg_steps)
+    mean = s.stats()[0]
+    mask = s.get_mask()
+    s.close()
+    return mean, mask, ms + ms2, ms2 / avg_steps
+
+
+def at_coarse_centres(field, r_src, r_dst):
+    """Trilinear samples of a node field of the r_src grid at the node
+    positions of the r_dst grid (node i of grid r sits at (i + 1/2)/r in
+    fine-grid units); r_src = 2 r_dst gives the 2x box filter."""
+    from scipy.ndimage import map_coordinates
+
+    nz, ny, nx = (int(round(n * r_dst / r_src)) for n in field.shape)
+    zz, yy, xx = np.meshgrid(*((np.arange(n) + 0.5) / r_dst * r_src - 0.5 for n in (nz, ny, nx)), indexing="ij")
+    return map_coordinates(field.astype(np.float64), [zz, yy, xx], order=1, mode="nearest")
+
+
+def eval_region(mask_c):
+    """Coarse cells used for the error: fluid cells at least two cells from
+    any solid node (the obstacle's staircase differs between grids)."""
+    from scipy.ndimage import binary_dilation
+
+    solid = np.isin(mask_c, (1, 2))
+    return ~binary_dilation(solid, iterations=2)
+
+
+def l2_errors(u, u_ref, region, u0):
+    d = (u - u_ref)[region]
+    return {"rel_l2": float(np.linalg.norm(d) / np.linalg.norm(u_ref[region])),
+            "wake_l2": float(np.linalg.norm(d) / np.linalg.norm((u_ref - u0)[region]))}
+
+
+def run(scale=1.0, spinup_fine=16000, snapshots=200, every=100, keep=0.02, train_steps=3000,
+        resolutions=(0.625, 0.75), seed=0, log=print):
+    """The full experiment at `scale` times the 512x128x128 fine grid."""
+    t0 = time.time()
+    base = tuple(int(round(b * scale)) for b in (512, 128, 128))
+    fine = channel_at(1.0, base, seed=seed)
+    u0 = fine.umax
+    fine_avg = snapshots * every
+    log(f"fine run {base} tau {1 / fine.omega:.4f}: spin-up {spinup_fine}, {snapshots} snapshots every {every}")
+    X, y, ids, fout, ms_fine = generate_samples(fine, spinup_fine, snapshots, every, keep, seed + 1,
+                                                stats_from=spinup_fine, log=log)
+    log(f"  {len(y)} samples, nu_t mean {y.mean():.3e}, zero fraction {np.mean(y == 0):.3f}")
+    # hold out the last 20% of the snapshots
+    split = int(snapshots * 0.8)
+    tr, te = ids < split, ids >= split
+    mlp, norm, hist = fit(X[tr], y[tr], seed, train_steps, log=log)
+    pred = mlp.forward(X[te])
+    c_fit = float(np.dot(smagorinsky_nu(X[tr], 1.0), y[tr]) / np.dot(smagorinsky_nu(X[tr], 1.0),
+                                                                      smagorinsky_nu(X[tr], 1.0)))
+    var = float(np.mean((y[te] - y[te].mean()) ** 2))
+    apriori = {"mse_mlp": float(np.mean((pred - y[te]) ** 2)),
+               "mse_smag_bestfit": float(np.mean((smagorinsky_nu(X[te], c_fit) - y[te]) ** 2)),
+               "mse_smag_cs0.1": float(np.mean((smagorinsky_nu(X[te], 0.01) - y[te]) ** 2)),
+               "var_target": var, "c_bestfit": c_fit,
+               "corr_mlp": float(np.corrcoef(pred, y[te])[0, 1]),
+               "corr_smag": float(np.corrcoef(smagorinsky_nu(X[te], 1.0), y[te])[0, 1])}
+    log(f"  a-priori test MSE: mlp {apriori['mse_mlp']:.3e}, best-fit Smagorinsky {apriori['mse_smag_bestfit']:.3e}")
+    u_ref = at_coarse_centres(fout["mean"][0], 1.0, 0.5)
+    coarse = channel_at(0.5, base, seed=seed)
+    # the same physical time as the fine run: half the steps
+    sp_c, av_c = spinup_fine // 2, fine_avg // 2
+    runs = {}
+    for name, r, use_mlp in [("coarse_learned", 0.5, True), ("coarse_smagorinsky", 0.5, False)] + \
+            [(f"smagorinsky_r{r}", r, False) for r in resolutions]:
+        cfg = channel_at(r, base, seed=seed)
+        sp, av = int(round(spinup_fine * r)), int(round(fine_avg * r))
+        mean, mask, ms, ms_step = mean_velocity(cfg, sp, av, mlp if use_mlp else None)
+        u_c = at_coarse_centres(mean[0], r, 0.5) if r != 0.5 else mean[0].astype(np.float64)
+        if r == 0.5:
+            region = eval_region(mask)
+        runs[name] = {"grid": [cfg.nx, cfg.ny, cfg.nz], "tau": 1 / cfg.omega, "steps": sp + av,
+                      "gpu_ms": ms, "ms_per_step": ms_step, "_u": u_c}
+        log(f"  {name}: grid {cfg.nx}x{cfg.ny}x{cfg.nz}, {sp + av} steps, {ms / 1e3:.2f} s GPU")
+    assert sp_c + av_c == runs["coarse_learned"]["steps"]
+    for name, rr in runs.items():
+        rr.update(l2_errors(rr.pop("_u"), u_ref, region, u0))
+        log(f"  {name}: rel L2 {rr['rel_l2']:.4f}, wake L2 {rr['wake_l2']:.4f}")
+    res = {"fine": {"grid": list(base), "tau": 1 / fine.omega, "steps": spinup_fine + fine_avg, "gpu_ms": ms_fine,
+                    "samples": int(len(y))},
+           "apriori": apriori, "training": {"steps": train_steps, "loss_first": float(hist[:50].mean()),
+                                            "loss_last": float(hist[-50:].mean())},
+           "norm": norm, "runs": runs, "wall_s": time.time() - t0, "eval_cells": int(region.sum())}
+    # GPU time Smagorinsky needs for the learned run's error (log-log
+    # interpolation of its error against its GPU time)
+    pts = sorted((v["gpu_ms"], v["wake_l2"]) for k, v in runs.items() if k.startswith("smag") or k == "coarse_smagorinsky")
+    target = runs["coarse_learned"]["wake_l2"]
+    tms = [p[0] for p in pts]
+    errs = [p[1] for p in pts]
+    res["smagorinsky_ms_at_learned_error"] = None
+    for (t1, e1), (t2, e2) in zip(pts, pts[1:]):
+        if (e1 - target) * (e2 - target) <= 0 and e1 != e2:
+            a = (math.log(target) - math.log(e1)) / (math.log(e2) - math.log(e1))
+            res["smagorinsky_ms_at_learned_error"] = math.exp(math.log(t1) + a * (math.log(t2) - math.log(t1)))
+    res["smagorinsky_curve"] = {"gpu_ms": tms, "wake_l2": errs}
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--scale", type=float, default=1.0, help="grid scale (1: the 512x128x128 fine run)")
+    ap.add_argument("--spinup", type=int, default=16000, help="fine-run steps before sampling")
+    ap.add_argument("--snapshots", type=int, default=200)
+    ap.add_argument("--every", type=int, default=100)
+    ap.add_argument("--keep", type=float, default=0.02)
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    res = run(a.scale, a.spinup, a.snapshots, a.every, a.keep, a.train_steps,
+              log=lambda *m: print(*m, flush=True))
+    txt = json.dumps(res, indent=1)
+    if a.out:
+        with open(a.out, "w") as fh:
+            fh.write(txt + "\n")
+    print(txt)
+
+
+if __name__ == "__main__":
+    main()
