@@ -10,6 +10,14 @@ Prints one JSON line with GLUPS, the roofline fraction of the stream-collide
 kernel (152 algorithmic bytes per node update in FP32, 76 in 16-bit, 304 in
 FP64, over its launch time from HIP events on the library's stream) and the
 CPU baseline (the oracle, OpenMP, on a bounded sample of the same workload).
+
+Two further workloads are measured for their own profiles (not the metric's
+line): --workload refined (the same channel on a 256x128x128 coarse grid
+with a 128x96x96 fine patch at 2:1, two fine sub-steps per coarse step;
+GLUPS counts coarse plus fine node updates) and --workload closure (the
+256x64x64 channel with the learned closure fused into the collide; its
+roofline is the matrix cores: 17408 algorithmic FLOP per node for the
+7-64-64-64-1 network, against the dense BF16 peak).
 """
 import argparse
 import json
@@ -22,8 +30,20 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GLUPS and % HBM roofline (D3Q19 MRT-LES); learned vs Smagorinsky error/time"
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+MLP_FLOP_PER_NODE = 2 * (7 * 64 + 64 * 64 + 64 * 64 + 64)  # algorithmic, padding excluded
 BYTES_PER_NODE = {"f32": 152, "f16": 76, "bf16": 76, "f64": 304}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stream_collide_channel512_traffic.json")
+
+
+WORKLOADS = {
+    "channel": "channel {c.nx}x{c.ny}x{c.nz} D3Q19 MRT (d'Humieres rates, tau=0.52) + Smagorinsky Cs=0.1, "
+               "square cylinder side 16, halfway walls, seeded inlet",
+    "refined": "channel {c.nx}x{c.ny}x{c.nz} MRT-LES with a 2:1 fine patch {c.lr_size[0]}x{c.lr_size[1]}x{c.lr_size[2]} "
+               "(temporal interpolation), square cylinder side 16",
+    "closure": "channel {c.nx}x{c.ny}x{c.nz} MRT (tau=0.51) with the learned 7-64-64-64-1 closure fused into "
+               "the collide, square cylinder side 8",
+}
 
 
 def cpu_baseline(cfg, budget_s=12.0):
@@ -105,6 +125,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16", "f64"])
+    ap.add_argument("--workload", default="channel", choices=["channel", "refined", "closure"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain device copy of one lattice instead (HBM ceiling / counter scale)")
@@ -120,35 +141,54 @@ def main():
     import lbm_multigpu_amd as L
     from lbm_multigpu_amd import config as C
 
-    cfg = C.channel(storage=a.storage)
+    if a.workload == "refined":
+        cfg = C.refined_channel(storage=a.storage)
+    elif a.workload == "closure":
+        cfg = C.channel(nx=256, ny=64, nz=64, side=8, x_center=64, tau=0.51, storage=a.storage)
+    else:
+        cfg = C.channel(storage=a.storage)
+    mlp = None
     with L.Solver(cfg.to_c()) as s:
         s.set_geometry(None)
         s.init_equilibrium(L.INIT_BGK)
+        if a.workload == "closure":
+            # a seeded network over the whole input range: every fluid node
+            # runs the network (the cost does not depend on the weights)
+            import numpy as np
+            mlp = L.Mlp(seed=1)
+            big = np.full(7, 3e38, np.float32)
+            mlp.set_norm(np.zeros(7, np.float32), np.ones(7, np.float32), -big, big, 1e-4)
+            s.set_closure(mlp)
         s.step(a.warmup)
         s.step_timed(1)  # drains the warm-up
         t0 = time.perf_counter()
         ms = s.step_timed(a.steps)
         wall = time.perf_counter() - t0
         mm = s.mass_momentum()
+        fine = s.sizes(1) if a.workload == "refined" else None
     cells = cfg.nx * cfg.ny * cfg.nz
+    updates = cells + (2 * fine.nx * fine.ny * fine.nz if fine is not None else 0)
     ms_step = ms / a.steps
-    glups = cells / (ms_step * 1e-3) / 1e9
-    ach = BYTES_PER_NODE[a.storage] * cells / (ms_step * 1e-3) / 1e9
+    glups = updates / (ms_step * 1e-3) / 1e9
     traffic = None
-    if os.path.exists(TRAFFIC_FILE) and a.storage == "f32":
-        with open(TRAFFIC_FILE) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
-    roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic}
-    cpu = None if a.no_cpu else cpu_baseline(cfg)
+    if a.workload == "closure":
+        ach = MLP_FLOP_PER_NODE * cells / (ms_step * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None}
+    else:
+        ach = BYTES_PER_NODE[a.storage] * updates / (ms_step * 1e-3) / 1e9
+        if os.path.exists(TRAFFIC_FILE) and a.storage == "f32" and a.workload == "channel":
+            with open(TRAFFIC_FILE) as fh:
+                traffic = json.load(fh).get("bytes_per_launch")
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic}
+    cpu = None if (a.no_cpu or a.workload != "channel") else cpu_baseline(cfg)
     line = {
         "metric": METRIC, "value": round(glups, 3), "unit": "GLUPS", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": a.storage, "data": "synthetic",
-        "config": {"workload": f"channel {cfg.nx}x{cfg.ny}x{cfg.nz} D3Q19 MRT (d'Humieres rates, tau=0.52) "
-                               "+ Smagorinsky Cs=0.1, square cylinder side 16, halfway walls, seeded inlet",
-                   "cells": cells, "storage": a.storage, "wall_s": round(wall, 3),
-                   "mass_sum": mm[0]},
+        "config": {"workload": WORKLOADS[a.workload].format(c=cfg), "cells": cells, "node_updates_per_step": updates,
+                   "storage": a.storage, "wall_s": round(wall, 3), "mass_sum": mm[0]},
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
