@@ -126,6 +126,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16", "f64"])
     ap.add_argument("--workload", default="channel", choices=["channel", "refined", "closure"])
+    ap.add_argument("--closure-every", type=int, default=1,
+                    help="closure workload: evaluate the network every N steps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain device copy of one lattice instead (HBM ceiling / counter scale)")
@@ -158,7 +160,7 @@ def main():
             mlp = L.Mlp(seed=1)
             big = np.full(7, 3e38, np.float32)
             mlp.set_norm(np.zeros(7, np.float32), np.ones(7, np.float32), -big, big, 1e-4)
-            s.set_closure(mlp)
+            s.set_closure(mlp, every=a.closure_every)
         s.step(a.warmup)
         s.step_timed(1)  # drains the warm-up
         t0 = time.perf_counter()

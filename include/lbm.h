@@ -243,6 +243,9 @@ int lbm_mlp_forward(lbm_mlp* m, const float* X, long long n, float* nu);
 /* attach (or, with m = NULL, detach) the closure to an MRT-LES context: its
  * collide uses tau = tau0 + 3 nu_t, nu_t capped at nu_cap */
 int lbm_set_closure(lbm_ctx* ctx, lbm_mlp* m, float nu_cap);
+/* evaluate the network every `every` steps (default 1); the steps between
+ * evaluations collide at the per-node rates of the last one */
+int lbm_set_closure_interval(lbm_ctx* ctx, int every);
 /* training samples from a context's current state, box-filtered by 2 (the
  * solver's own finer run): features X[n][7] and the SGS eddy viscosity
  * y[n] = max(0, -tau_dev:S/(2 S:S)) of a hashed fraction `keep` of the

@@ -65,7 +65,7 @@ EXPORTS = [
     "lbm_mlp_create", "lbm_mlp_destroy", "lbm_mlp_last_error", "lbm_mlp_num_params",
     "lbm_mlp_set_params", "lbm_mlp_get_params", "lbm_mlp_set_norm", "lbm_mlp_set_data",
     "lbm_mlp_gradient", "lbm_mlp_train", "lbm_mlp_forward", "lbm_set_closure",
-    "lbm_closure_samples", "lbm_stats_enable", "lbm_stats_get",
+    "lbm_closure_samples", "lbm_stats_enable", "lbm_stats_get", "lbm_set_closure_interval",
 ]
 
 # closure network: 7 -> 64 -> 64 -> 64 -> 1, flat FP32 parameter vector
@@ -130,6 +130,7 @@ def load_library(path=LIB_PATH):
     lib.lbm_mlp_train.argtypes = [P, C.c_int, C.c_int, C.c_float, P]
     lib.lbm_mlp_forward.argtypes = [P, f32, C.c_longlong, f32]
     lib.lbm_set_closure.argtypes = [P, P, C.c_float]
+    lib.lbm_set_closure_interval.argtypes = [P, C.c_int]
     lib.lbm_closure_samples.argtypes = [P, C.c_uint64, C.c_float, f32, f32, C.c_int, C.POINTER(C.c_int)]
     _lib = lib
     return lib
@@ -279,11 +280,14 @@ class Solver:
         self._check(self.lib.lbm_stats_get(self.h, level, mean.ctypes.data, dev.ctypes.data))
         return mean, dev
 
-    def set_closure(self, mlp, nu_cap=0.05):
+    def set_closure(self, mlp, nu_cap=0.05, every=1):
         """Attach a trained Mlp (None detaches): the collide then takes its
-        eddy viscosity from the network, Smagorinsky outside its range."""
+        eddy viscosity from the network, Smagorinsky outside its range; the
+        network is evaluated every `every` steps."""
         self._closure = mlp  # keep the network alive while attached
         self._check(self.lib.lbm_set_closure(self.h, None if mlp is None else mlp.h, nu_cap))
+        if mlp is not None:
+            self._check(self.lib.lbm_set_closure_interval(self.h, every))
 
     def closure_samples(self, seed, keep, max_samples):
         """Features X[n, 7] and eddy-viscosity targets y[n] from this run,

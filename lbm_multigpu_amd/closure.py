@@ -13,7 +13,8 @@ solver's own data:
    and Adam kernels (lbm_mlp_train), with the inputs standardised and their
    training range recorded (outside it the collide falls back to
    Smagorinsky);
-3. coarse runs with the learned closure and with Smagorinsky, and Smagorinsky
+3. coarse runs with the learned closure (evaluated every step, and every k
+   steps with the rates held in between) and with Smagorinsky, and Smagorinsky
    runs at intermediate resolutions, collect the time-averaged velocity with
    the fused running statistics; each is compared with the fine run's mean
    at the coarse cell centres, and the GPU time of each run is measured with
@@ -33,7 +34,7 @@ import time
 
 import numpy as np
 
-from . import Mlp, Solver, STORE_F32
+from . import Mlp, Solver
 from . import config as cfgmod
 
 FEATURES = ("log|S|", "3sqrt6 det(S^)", "S^xy", "S^xz", "S^yz", "log d_wall", "|u|")
@@ -106,14 +107,15 @@ def fit(X, y, seed=0, steps=3000, batch=4096, lr=2e-3, lr_final=2e-4, log=print)
     return m, norm, np.concatenate(hist)
 
 
-def mean_velocity(cfg, spinup, avg_steps, mlp=None):
+def mean_velocity(cfg, spinup, avg_steps, mlp=None, refresh=1):
     """Time-averaged velocity of a run after its start-up; returns (mean
-    u[3, nz, ny, nx], mask, GPU ms of the whole run, ms per step)."""
+    u[3, nz, ny, nx], mask, GPU ms of the whole run, ms per step).  With a
+    network, it is evaluated every `refresh` steps."""
     s = Solver(cfg)
     s.set_geometry()
     s.init_equilibrium()
     if mlp is not None:
-        s.set_closure(mlp)
+        s.set_closure(mlp, every=refresh)
     s.stats_enable(spinup, 1 << 62)
     ms = s.step_timed(spinup)
     ms2 = s.step_timed(avg_steps)
@@ -150,7 +152,7 @@ def l2_errors(u, u_ref, region, u0):
 
 
 def run(scale=1.0, spinup_fine=16000, snapshots=200, every=100, keep=0.02, train_steps=3000,
-        resolutions=(0.625, 0.75), seed=0, log=print):
+        resolutions=(0.625, 0.75), seed=0, refresh=(1, 10), log=print):
     """The full experiment at `scale` times the 512x128x128 fine grid."""
     t0 = time.time()
     base = tuple(int(round(b * scale)) for b in (512, 128, 128))
@@ -181,18 +183,19 @@ def run(scale=1.0, spinup_fine=16000, snapshots=200, every=100, keep=0.02, train
     # the same physical time as the fine run: half the steps
     sp_c, av_c = spinup_fine // 2, fine_avg // 2
     runs = {}
-    for name, r, use_mlp in [("coarse_learned", 0.5, True), ("coarse_smagorinsky", 0.5, False)] + \
-            [(f"smagorinsky_r{r}", r, False) for r in resolutions]:
+    cases = [("coarse_learned" if k == 1 else f"coarse_learned_refresh{k}", 0.5, k) for k in refresh]
+    cases += [("coarse_smagorinsky", 0.5, 0)] + [(f"smagorinsky_r{r}", r, 0) for r in resolutions]
+    for name, r, k in cases:
         cfg = channel_at(r, base, seed=seed)
         sp, av = int(round(spinup_fine * r)), int(round(fine_avg * r))
-        mean, mask, ms, ms_step = mean_velocity(cfg, sp, av, mlp if use_mlp else None)
+        mean, mask, ms, ms_step = mean_velocity(cfg, sp, av, mlp if k else None, max(k, 1))
         u_c = at_coarse_centres(mean[0], r, 0.5) if r != 0.5 else mean[0].astype(np.float64)
         if r == 0.5:
             region = eval_region(mask)
         runs[name] = {"grid": [cfg.nx, cfg.ny, cfg.nz], "tau": 1 / cfg.omega, "steps": sp + av,
                       "gpu_ms": ms, "ms_per_step": ms_step, "_u": u_c}
         log(f"  {name}: grid {cfg.nx}x{cfg.ny}x{cfg.nz}, {sp + av} steps, {ms / 1e3:.2f} s GPU")
-    assert sp_c + av_c == runs["coarse_learned"]["steps"]
+    assert sp_c + av_c == runs["coarse_smagorinsky"]["steps"]
     for name, rr in runs.items():
         rr.update(l2_errors(rr.pop("_u"), u_ref, region, u0))
         log(f"  {name}: rel L2 {rr['rel_l2']:.4f}, wake L2 {rr['wake_l2']:.4f}")
@@ -204,7 +207,7 @@ def run(scale=1.0, spinup_fine=16000, snapshots=200, every=100, keep=0.02, train
     # GPU time Smagorinsky needs for the learned run's error (log-log
     # interpolation of its error against its GPU time)
     pts = sorted((v["gpu_ms"], v["wake_l2"]) for k, v in runs.items() if k.startswith("smag") or k == "coarse_smagorinsky")
-    target = runs["coarse_learned"]["wake_l2"]
+    target = runs[cases[0][0]]["wake_l2"]
     tms = [p[0] for p in pts]
     errs = [p[1] for p in pts]
     res["smagorinsky_ms_at_learned_error"] = None
@@ -224,9 +227,11 @@ def main(argv=None):
     ap.add_argument("--every", type=int, default=100)
     ap.add_argument("--keep", type=float, default=0.02)
     ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--refresh", type=int, nargs="+", default=[1, 10],
+                    help="network evaluation intervals of the learned coarse runs")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
-    res = run(a.scale, a.spinup, a.snapshots, a.every, a.keep, a.train_steps,
+    res = run(a.scale, a.spinup, a.snapshots, a.every, a.keep, a.train_steps, refresh=tuple(a.refresh),
               log=lambda *m: print(*m, flush=True))
     txt = json.dumps(res, indent=1)
     if a.out:

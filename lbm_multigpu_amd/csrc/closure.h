@@ -189,15 +189,15 @@ __global__ void k_mlp_sync_bf16(const float* params, short* wbf)
     }
 }
 
-// Inference on raw inputs X[n][7] -> nu[n] (256 threads: 4 wavefronts of 64).
-__global__ void __launch_bounds__(256) k_mlp_forward(const float* X, long long n, const float* params,
-                                                     const short* wbf, MlpNorm nrm, float* nu)
+// Inference on raw inputs X[n][7] -> nu[n] (256 threads: 4 wavefronts of 64),
+// the same register chain the fused collide runs.
+__global__ void __launch_bounds__(256) k_mlp_forward(const float* X, long long n, const MlpChain* chain,
+                                                     MlpNorm nrm, float* nu)
 {
-    __shared__ MlpLds s;
+    __shared__ MlpChain s;
     __shared__ short xb[4][64 * MLP_INP];
-    __shared__ short buf[4][64 * MLP_H];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    mlp_stage(s, wbf, params);
+    chain_stage(s, chain);
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
 #pragma unroll
     for (int k = 0; k < MLP_INP; ++k) {
@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float* X, long long n
         xb[w][lane * MLP_INP + k] = bf(v);
     }
     __syncthreads();
-    const float zo = mlp_infer(s, xb[w], buf[w]);
+    const float zo = mlp_chain_infer(s, xb[w]);
     if (i < n) nu[i] = nrm.nu_scale * softplus(zo);
 }
 
@@ -299,26 +299,26 @@ __global__ void k_closure_features(const float* bx, int cx, int cy, int cz, cons
 // ------------------------------------------------------------ fused closure
 
 struct ClosureArgs {
-    const float* params;
-    const short* wbf;
-    const float* dwall; // per node (padded layout)
+    const MlpChain* chain; // weights in the register-chain arrangement
+    const float* dwall;    // per node (padded layout)
     MlpNorm nrm;
     float nu_cap;       // upper bound on nu_t (stability)
+    float* omega_out;   // if set: each collided node's rate (0 = Smagorinsky) for the steps until the next refresh
 };
 
 // Stream-collide with the learned closure: pull and boundary conditions per
 // node, features from the pre-collision moments, the MLP for the 64 nodes of
-// each wavefront on the matrix cores (all lanes take part; nodes that are not
-// collided feed dummy inputs), then the MRT collide at tau = tau0 + 3 nu_t,
-// or the Smagorinsky rate where an input leaves the training range.
+// each wavefront on the matrix cores (register chain; all lanes take part,
+// nodes that are not collided feed dummy inputs), then the MRT collide at
+// tau = tau0 + 3 nu_t, or the Smagorinsky rate where an input leaves the
+// training range.
 template <int MODEL, class S>
 __global__ void __launch_bounds__(TX* TY) k_stream_collide_closure(StepParams p, ClosureArgs ca)
 {
     using R = typename Arith<S>::R;
-    __shared__ MlpLds s;
+    __shared__ MlpChain s;
     __shared__ short xb[TY][64 * MLP_INP];
-    __shared__ short buf[TY][64 * MLP_H];
-    mlp_stage(s, ca.wbf, ca.params);
+    chain_stage(s, ca.chain);
     const int lb = logical_block();
     const int seg = lb % p.nseg;
     const int t2 = lb / p.nseg;
@@ -360,15 +360,18 @@ __global__ void __launch_bounds__(TX* TY) k_stream_collide_closure(StepParams p,
     for (int q = 0; q < MLP_INP; ++q)
         xb[w][threadIdx.x * MLP_INP + q] = q < MLP_IN ? bf((feat[q] - ca.nrm.mu[q]) * ca.nrm.inv_sd[q]) : (short)0;
     __syncthreads();
-    const float zo = mlp_infer(s, xb[w], buf[w]);
+    const float zo = mlp_chain_infer(s, xb[w]);
     if (act != ACT_SKIP) {
         if (act == ACT_COLLIDE) {
             CollideParams<R> cp = collide_params<R>(p);
             if (in_range) {
                 const float nut = fminf(ca.nrm.nu_scale * softplus(zo), ca.nu_cap);
-                cp.omega = R(1.f / (1.f / p.omega + 3.f * nut));
+                const float w = 1.f / (1.f / p.omega + 3.f * nut);
+                cp.omega = R(w);
+                if (ca.omega_out) *at(ca.omega_out, cell) = w;
             } else {
                 cp.omega = omega_les;
+                if (ca.omega_out) *at(ca.omega_out, cell) = 0.f;
             }
             cp.cs = R(0);
             collide<MODEL>(f, cp);

@@ -49,6 +49,9 @@ struct StepParams {
     float* stat;
     int stat_mode;
     float stat_n, stat_w; // samples so far (t - start) and this sample's weight
+    // per-node collision rate left by the last learned-closure step (padded
+    // layout): > 0 replaces the LES rate, 0 means the Smagorinsky rate
+    const float* omega_node;
 };
 
 // XCD-aware renumbering: the hardware id hw goes to XCD hw % 8; give XCD j
@@ -254,7 +257,14 @@ __device__ __forceinline__ void node_update(const StepParams& p, int x, int y, i
     const int act = node_pull<S>(p, x, y, z, cell, f, fx, fy, fz);
     if (act == ACT_SKIP) return;
     if (act == ACT_COLLIDE) {
-        const CollideParams<R> cp = collide_params<R>(p);
+        CollideParams<R> cp = collide_params<R>(p);
+        if (p.omega_node) {
+            const float w = *at(p.omega_node, cell);
+            if (w > 0.f) {
+                cp.omega = R(w);
+                cp.cs = R(0);
+            }
+        }
         collide<MODEL>(f, cp);
         if (p.stat_mode) stats_update(p, cell, f);
     }

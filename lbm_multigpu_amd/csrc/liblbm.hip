@@ -70,6 +70,9 @@ struct lbm_ctx {
     lbm_mlp* mlp = nullptr;    // not owned
     float* dwall = nullptr;    // wall distance per node (padded layout)
     float nu_cap = 0.05f;
+    int closure_every = 1;     // network refresh interval (steps)
+    long long closure_t0 = 0;  // step of the first refresh
+    float* omega_field = nullptr; // per-node rate between refreshes (padded layout)
     std::string err;
 };
 
@@ -78,7 +81,8 @@ struct lbm_mlp {
     float* m = nullptr;        // Adam moments
     float* v = nullptr;
     float* grad = nullptr;
-    short* wbf = nullptr;      // BF16 copies for the matrix cores
+    short* wbf = nullptr;      // BF16 copies for the matrix cores (training)
+    MlpChain* chain = nullptr; // BF16 weights arranged for the inference chain
     float* X = nullptr;        // device training data
     float* y = nullptr;
     long long n = 0;
@@ -323,7 +327,8 @@ extern "C" void lbm_destroy(lbm_ctx* ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (int l = 0; l < 2; ++l) free_level(ctx->lv[l]);
     for (void* p : {(void*)ctx->u_in, (void*)ctx->fpart, (void*)ctx->fhist, (void*)ctx->red, (void*)ctx->mpart,
-                    (void*)ctx->Mold, (void*)ctx->Mnew, (void*)ctx->cf_blocks, (void*)ctx->dwall})
+                    (void*)ctx->Mold, (void*)ctx->Mnew, (void*)ctx->cf_blocks, (void*)ctx->dwall,
+                    (void*)ctx->omega_field})
         if (p) (void)hipFree(p);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -544,6 +549,7 @@ static StepParams step_params(lbm_ctx* ctx, const Level& L, float omega)
     p.dpdy = c.dpdy;
     p.u_lid = c.umax;
     p.s1 = c.s_dhum[1]; p.s2 = c.s_dhum[2]; p.s4 = c.s_dhum[4]; p.s10 = c.s_dhum[10]; p.s16 = c.s_dhum[16];
+    p.omega_node = nullptr;
     p.stat = L.stat;
     p.stat_mode = 0;
     p.stat_n = 0.f;
@@ -570,11 +576,11 @@ static void launch_sc(int model, const StepParams& p, int blocks, hipStream_t s)
 }
 
 template <class S>
-static void launch_closure(lbm_ctx* ctx, const StepParams& p, int blocks)
+static void launch_closure(lbm_ctx* ctx, const StepParams& p, int blocks, float* omega_out)
 {
     ClosureArgs ca;
-    ca.params = ctx->mlp->params;
-    ca.wbf = ctx->mlp->wbf;
+    ca.omega_out = omega_out;
+    ca.chain = ctx->mlp->chain;
     ca.dwall = ctx->dwall;
     ca.nrm = ctx->mlp->nrm;
     ca.nu_cap = ctx->nu_cap;
@@ -596,8 +602,17 @@ static int coarse_step(lbm_ctx* ctx)
     if (rec) p.fpart = ctx->fpart;
     with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        if (ctx->mlp) launch_closure<S>(ctx, p, L.blocks);
-        else launch_sc<S>(c.model, p, L.blocks, ctx->stream);
+        if (!ctx->mlp) {
+            launch_sc<S>(c.model, p, L.blocks, ctx->stream);
+        } else if (ctx->closure_every <= 1) {
+            launch_closure<S>(ctx, p, L.blocks, nullptr);
+        } else if ((ctx->t - ctx->closure_t0) % ctx->closure_every == 0) {
+            launch_closure<S>(ctx, p, L.blocks, ctx->omega_field); // refresh: network + rates for the next steps
+        } else {
+            StepParams q = p;
+            q.omega_node = ctx->omega_field;
+            launch_sc<S>(c.model, q, L.blocks, ctx->stream);
+        }
         return 0;
     });
     if (rec) {
@@ -1086,7 +1101,8 @@ extern "C" int lbm_mlp_create(uint64_t seed, lbm_mlp** out)
     const size_t P = MLP_NPARAM;
     const size_t nwbf = MLP_INP * MLP_H + 2 * MLP_H * MLP_H;
     if (hipMalloc(&m->params, P * 4) || hipMalloc(&m->m, P * 4) || hipMalloc(&m->v, P * 4) ||
-        hipMalloc(&m->grad, P * 4) || hipMalloc(&m->wbf, nwbf * 2) || hipMalloc(&m->dloss, 8))
+        hipMalloc(&m->grad, P * 4) || hipMalloc(&m->wbf, nwbf * 2) || hipMalloc(&m->dloss, 8) ||
+        hipMalloc(&m->chain, sizeof(MlpChain)))
         return bail(mfail(nullptr, LBM_ENOMEM, "lbm_mlp_create: device allocation failed"));
     // Glorot-uniform weights from the counter hash, zero biases
     std::vector<float> p(P, 0.f);
@@ -1115,7 +1131,7 @@ extern "C" void lbm_mlp_destroy(lbm_mlp* m)
 {
     if (!m) return;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
-    for (void* q : {(void*)m->params, (void*)m->m, (void*)m->v, (void*)m->grad, (void*)m->wbf, (void*)m->X,
+    for (void* q : {(void*)m->params, (void*)m->m, (void*)m->v, (void*)m->grad, (void*)m->wbf, (void*)m->chain, (void*)m->X,
                     (void*)m->y, (void*)m->partials, (void*)m->lossp, (void*)m->dloss})
         if (q) (void)hipFree(q);
     if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -1135,6 +1151,7 @@ extern "C" int lbm_mlp_set_params(lbm_mlp* m, const float* params)
     MLP_TRY(m, hipMemsetAsync(m->v, 0, P * 4, m->stream));
     const int nwbf = MLP_INP * MLP_H + 2 * MLP_H * MLP_H;
     hipLaunchKernelGGL(k_mlp_sync_bf16, dim3((nwbf + 255) / 256), dim3(256), 0, m->stream, m->params, m->wbf);
+    hipLaunchKernelGGL(k_mlp_pack_chain, dim3(1), dim3(64), 0, m->stream, m->params, m->chain);
     MLP_TRY(m, hipGetLastError());
     MLP_TRY(m, hipStreamSynchronize(m->stream));
     m->t = 0;
@@ -1237,6 +1254,7 @@ extern "C" int lbm_mlp_train(lbm_mlp* m, int steps, int batch, float lr, double*
             loss_hist[s] = l / batch;
         }
     }
+    hipLaunchKernelGGL(k_mlp_pack_chain, dim3(1), dim3(64), 0, m->stream, m->params, m->chain);
     MLP_TRY(m, hipGetLastError());
     MLP_TRY(m, hipStreamSynchronize(m->stream));
     return LBM_OK;
@@ -1251,7 +1269,7 @@ extern "C" int lbm_mlp_forward(lbm_mlp* m, const float* X, long long n, float* n
     hipError_t e = hipMemcpyAsync(dx, X, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice, m->stream);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, m->stream, dx, n,
-                           m->params, m->wbf, m->nrm, dn);
+                           m->chain, m->nrm, dn);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(nu, dn, (size_t)n * 4, hipMemcpyDeviceToHost, m->stream);
@@ -1284,6 +1302,24 @@ extern "C" int lbm_set_closure(lbm_ctx* ctx, lbm_mlp* m, float nu_cap)
     HIP_TRY(ctx, hipMemcpy(ctx->dwall, pad.data(), pad.size() * 4, hipMemcpyHostToDevice));
     ctx->mlp = m;
     ctx->nu_cap = nu_cap > 0.f ? nu_cap : 0.05f;
+    ctx->closure_t0 = ctx->t;
+    return LBM_OK;
+}
+
+// Evaluate the network every `every` steps; the steps in between collide at
+// the per-node rates of the last evaluation (Smagorinsky where the inputs
+// were outside the training range).
+extern "C" int lbm_set_closure_interval(lbm_ctx* ctx, int every)
+{
+    if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_set_closure_interval: null context");
+    if (every < 1) return fail(ctx, LBM_EINVAL, "lbm_set_closure_interval: every must be >= 1");
+    if (every > 1 && !ctx->omega_field) {
+        Level& L = ctx->lv[0];
+        HIP_TRY(ctx, hipMalloc(&ctx->omega_field, (size_t)L.qs * sizeof(float)));
+        HIP_TRY(ctx, hipMemset(ctx->omega_field, 0, (size_t)L.qs * sizeof(float)));
+    }
+    ctx->closure_every = every;
+    ctx->closure_t0 = ctx->t;
     return LBM_OK;
 }
 

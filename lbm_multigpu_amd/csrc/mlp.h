@@ -2,13 +2,24 @@
 // 7 -> 64 -> 64 -> 64 -> 1 (GELU hidden layers, softplus output), BF16
 // weights and activations, FP32 accumulation on the matrix cores.
 //
-// Every GEMM is a wavefront-level product over 64 rows (64 nodes or 64
-// training samples) built from v_mfma_f32_16x16x16_bf16: operand lanes
-// l = 16 q + r hold row/column r and the four k = 4q..4q+3 of a 16x16x16
-// step, and the 4 accumulator lanes hold rows 4q..4q+3 of column r.  The
-// operands are read through callables, so the same routine serves the
-// forward pass (H W), the data gradient (dZ W^T) and the weight gradient
-// (H^T dZ) straight from LDS in whatever layout they sit.
+// Training: every GEMM is a wavefront-level product over 64 samples built
+// from v_mfma_f32_16x16x16_bf16 (wave_gemm): operand lanes l = 16 q + r hold
+// row/column r and the four k = 4q..4q+3 of a 16x16x16 step, and the
+// accumulator lanes hold rows 4q..4q+3 of column r.  The operands are read
+// through callables, so the same routine serves the forward pass (H W), the
+// data gradient (dZ W^T) and the weight gradient (H^T dZ) straight from LDS.
+//
+// Inference (mlp_chain_infer): the transposed products Z^T = W^T H^T, 16
+// nodes per MFMA column block.  An accumulator lane then holds hidden units
+// 4q..4q+3 of node r -- exactly the B-operand slots of the next layer -- so
+// activations stay in registers through all three layers; the weights are
+// pre-arranged (MlpChain) so each A operand is one 8- or 16-byte LDS read.
+// Layers 2 and 3 use gfx950's v_mfma_f32_16x16x32_bf16, whose 32-long k
+// slots 8q..8q+7 take hidden units {4q..4q+3} of one 16-unit block and
+// {4q..4q+3} of the next (the pairing the accumulator layout produces).
+//
+// GELU is the tanh form 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))) in
+// training and inference alike.
 //
 // No reference counterpart: the reference's closure is constant-Cs
 // Smagorinsky (MRTtest/DynInlet3.cu:628-735); its Pi_neq (:722-729) and
@@ -42,10 +53,23 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ short bf(float v) { return (short)float_to_bf16_bits(v); }
 __device__ __forceinline__ float fb(short h) { return bf16_bits_to_float((uint16_t)h); }
 
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
+// GELU (tanh form) through the logistic function: 0.5 (1 + tanh u) =
+// 1 / (1 + exp(-2u)) with u = sqrt(2/pi) (x + 0.044715 x^3), so
+// gelu(x) = x s and gelu'(x) = s + 2 sqrt(2/pi) x s (1 - s) (1 + 3 0.044715 x^2).
+// s costs one v_exp_f32 and one v_rcp_f32 (the exponent -2u log2(e) is one
+// multiply-add); the limits s -> 0 (exp2 -> inf) and s -> 1 are exact.
+__device__ __forceinline__ float gelu_s(float x)
+{
+    const float K1 = -2.3022081981443250f;  // -2 sqrt(2/pi) log2(e)
+    const float K2 = -0.1029432395800235f;  // K1 * 0.044715
+    const float v = x * fmaf(x * x, K2, K1);
+    return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(v));
+}
+__device__ __forceinline__ float gelu(float x) { return x * gelu_s(x); }
 __device__ __forceinline__ float gelu_grad(float x)
 {
-    return 0.5f * (1.f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
+    const float s = gelu_s(x);
+    return s + 1.5957691216057308f * x * s * (1.f - s) * fmaf(0.134145f, x * x, 1.f);
 }
 __device__ __forceinline__ float softplus(float z) { return fmaxf(z, 0.f) + log1pf(expf(-fabsf(z))); }
 __device__ __forceinline__ float sigmoid(float z) { return 1.f / (1.f + expf(-z)); }
@@ -115,74 +139,136 @@ __device__ __forceinline__ void mlp_stage(MlpLds& s, const short* wbf, const flo
     if (threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0) s.b4 = p[OFF_B4];
 }
 
-// First layer (inference): buf[64][64] = BF16(GELU(x W1 + b1)), x BF16 [64][16].
-__device__ __forceinline__ void mlp_layer1(const MlpLds& s, const short* x, short* buf)
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// Weights arranged for the register chain (built by k_mlp_pack_chain):
+// A-operand lane l = 16 q + r of output block mt holds
+//   w1[mt][l][j]     = W1[4q + j][16 mt + r]              (inputs padded to 16)
+//   w2[mt][kt][l][j] = W2[h(kt, q, j)][16 mt + r], h = 32 kt + 4q + j (j < 4),
+//                                                      32 kt + 16 + 4q + j - 4 (j >= 4)
+// and b*[mt][q] = b[16 mt + 4q .. +3], w4[mt][q] likewise.
+struct MlpChain {
+    bf16x4 w1[4][64];
+    bf16x8 w2[4][2][64];
+    bf16x8 w3[4][2][64];
+    f32x4 b1[4][4], b2[4][4], b3[4][4], w4[4][4];
+    float b4;
+    float pad[3];
+};
+
+__device__ __forceinline__ int chain_h(int kt, int q, int j) { return 32 * kt + (j < 4 ? 4 * q + j : 12 + 4 * q + j); }
+
+__global__ void k_mlp_pack_chain(const float* p, MlpChain* c)
 {
-    wave_gemm<64, 64, 16>([&](int m, int k) { return x[m * MLP_INP + k]; },
-                          [&](int k, int n) { return s.w1[k * MLP_H + n]; },
-                          [&](int m, int n, float v) { buf[m * MLP_H + n] = bf(gelu(fb(bf(v + s.b1[n])))); });
+    const int l = threadIdx.x; // 64 lanes
+    const int q = l >> 4, r = l & 15;
+    for (int mt = 0; mt < 4; ++mt) {
+        const int col = 16 * mt + r;
+        for (int j = 0; j < 4; ++j) {
+            const int in = 4 * q + j;
+            c->w1[mt][l][j] = in < MLP_IN ? bf(p[OFF_W1 + in * MLP_H + col]) : (short)0;
+        }
+        for (int kt = 0; kt < 2; ++kt)
+            for (int j = 0; j < 8; ++j) {
+                const int h = chain_h(kt, q, j);
+                c->w2[mt][kt][l][j] = (__bf16)p[OFF_W2 + h * MLP_H + col];
+                c->w3[mt][kt][l][j] = (__bf16)p[OFF_W3 + h * MLP_H + col];
+            }
+    }
+    if (l < 16) {
+        const int mt = l >> 2, qq = l & 3;
+        for (int j = 0; j < 4; ++j) {
+            const int h = 16 * mt + 4 * qq + j;
+            c->b1[mt][qq][j] = p[OFF_B1 + h];
+            c->b2[mt][qq][j] = p[OFF_B2 + h];
+            c->b3[mt][qq][j] = p[OFF_B3 + h];
+            c->w4[mt][qq][j] = p[OFF_W4 + h];
+        }
+    }
+    if (l == 0) c->b4 = p[OFF_B4];
 }
 
-// Hidden layer in place (inference): buf holds activations h = BF16(GELU(z))
-// and becomes BF16(GELU(BF16(buf W + b))) -- the same roundings as the
-// training forward pass, which keeps z in BF16.  The 64x64 results are
-// held in registers until every lane of the wavefront has read its operands.
-// Contains block barriers: every wavefront of the workgroup must call it.
-__device__ __forceinline__ void mlp_layer_inplace(short* buf, const short* w, const float* b)
+// cooperative copy of the chain weights into LDS
+__device__ __forceinline__ void chain_stage(MlpChain& s, const MlpChain* g)
+{
+    const int n = (int)(sizeof(MlpChain) / 16);
+    const int t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nt = blockDim.x * blockDim.y * blockDim.z;
+    const f32x4* src = reinterpret_cast<const f32x4*>(g);
+    f32x4* dst = reinterpret_cast<f32x4*>(&s);
+    for (int i = t; i < n; i += nt) dst[i] = src[i];
+}
+
+// z = BF16(acc + b), h = BF16(GELU(z)) for the four units this lane holds
+__device__ __forceinline__ void chain_act(const f32x4& acc, const f32x4& b, __bf16 h[4])
+{
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float z = (float)(__bf16)(acc[j] + b[j]);
+        h[j] = (__bf16)gelu(z);
+    }
+}
+
+// The network for the 64 nodes of one wavefront.  xb: LDS [64][16] BF16 bits
+// of the standardised inputs, one row per node (lane l's node is row l).
+// Returns the output pre-activation of this lane's node.  All 64 lanes must
+// take part.
+__device__ __forceinline__ float mlp_chain_infer(const MlpChain& c, const short* xb)
 {
     const int lane = threadIdx.x & 63;
-    const int r = lane & 15, q = lane >> 4;
-    float res[16][4];
+    const int q = lane >> 4, r = lane & 15;
+    float mine = 0.f;
+#pragma unroll 1
+    for (int nt = 0; nt < 4; ++nt) {
+        // keeps the weight reads inside the loop (re-read from LDS per node
+        // block) instead of 128 VGPRs of hoisted weights
+        asm volatile("" ::: "memory");
+        const bf16x4 xin = *reinterpret_cast<const bf16x4*>(xb + (16 * nt + r) * MLP_INP + 4 * q);
+        f32x4 acc[4];
+        bf16x8 hb[2];
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
+        for (int mt = 0; mt < 4; ++mt) {
+            const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+            acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(c.w1[mt][lane], xin, zero, 0, 0, 0);
+        }
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int mt = 0; mt < 4; ++mt) {
+            __bf16 h[4];
+            chain_act(acc[mt], c.b1[mt][q], h);
 #pragma unroll
-            for (int kt = 0; kt < 4; ++kt) {
-                bf16x4 av, bv;
+            for (int j = 0; j < 4; ++j) hb[mt >> 1][(mt & 1) * 4 + j] = h[j];
+        }
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    av[j] = buf[(mt * 16 + r) * MLP_H + kt * 16 + 4 * q + j];
-                    bv[j] = w[(kt * 16 + 4 * q + j) * MLP_H + nt * 16 + r];
-                }
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+        for (int layer = 0; layer < 2; ++layer) {
+            const bf16x8(*w)[2][64] = layer == 0 ? c.w2 : c.w3;
+            const f32x4(*b)[4] = layer == 0 ? c.b2 : c.b3;
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                f32x4 a = {0.f, 0.f, 0.f, 0.f};
+                a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[mt][0][lane], hb[0], a, 0, 0, 0);
+                a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[mt][1][lane], hb[1], a, 0, 0, 0);
+                acc[mt] = a;
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) res[mt * 4 + nt][j] = acc[j];
+            for (int mt = 0; mt < 4; ++mt) {
+                __bf16 h[4];
+                chain_act(acc[mt], b[mt][q], h);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) hb[mt >> 1][(mt & 1) * 4 + j] = h[j];
+            }
         }
-    __syncthreads();
+        float part = 0.f;
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
+        for (int mt = 0; mt < 4; ++mt) {
+            const f32x4 w4 = c.w4[mt][q];
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                buf[(mt * 16 + 4 * q + j) * MLP_H + nt * 16 + r] =
-                    bf(gelu(fb(bf(res[mt * 4 + nt][j] + b[nt * 16 + r]))));
-    __syncthreads();
-}
-
-// Output layer (64 -> 1) on the vector ALUs: this lane's row of the
-// activations buf.
-__device__ __forceinline__ float mlp_head(const MlpLds& s, const short* buf)
-{
-    const int lane = threadIdx.x & 63;
-    float zo = s.b4;
-#pragma unroll 8
-    for (int i = 0; i < MLP_H; ++i) zo += fb(buf[lane * MLP_H + i]) * s.w4[i];
-    return zo;
-}
-
-// Inference for the 64 rows of one wavefront (x: BF16 [64][16], buf: BF16
-// [64][64] scratch in LDS); returns this lane's output pre-activation.
-__device__ __forceinline__ float mlp_infer(const MlpLds& s, const short* x, short* buf)
-{
-    mlp_layer1(s, x, buf);
-    __syncthreads();
-    mlp_layer_inplace(buf, s.w2, s.b2);
-    mlp_layer_inplace(buf, s.w3, s.b3);
-    return mlp_head(s, buf);
+            for (int j = 0; j < 4; ++j) part += (float)hb[mt >> 1][(mt & 1) * 4 + j] * w4[j];
+        }
+        part += __shfl_xor(part, 16, 64);
+        part += __shfl_xor(part, 32, 64);
+        if (nt == q) mine = part + c.b4;
+    }
+    return mine;
 }
 
 } // namespace lbm

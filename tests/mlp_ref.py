@@ -21,7 +21,6 @@ W3[64][64] b3[64] w4[64] b4.
 import math
 
 import numpy as np
-from scipy.special import erf
 
 NIN, NH = 7, 64
 SIZES = [("W1", (NIN, NH)), ("b1", (NH,)), ("W2", (NH, NH)), ("b2", (NH,)),
@@ -63,12 +62,17 @@ def bf16(x):
     return u.astype(np.uint32).view(np.float32)
 
 
+_C = math.sqrt(2.0 / math.pi)
+
+
 def gelu(x):
-    return 0.5 * x * (1.0 + erf(x / math.sqrt(2.0)))
+    """GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))."""
+    return 0.5 * x * (1.0 + np.tanh(_C * (x + 0.044715 * x ** 3)))
 
 
 def gelu_grad(x):
-    return 0.5 * (1.0 + erf(x / math.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / math.sqrt(2.0 * math.pi)
+    t = np.tanh(_C * (x + 0.044715 * x ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * _C * (1.0 + 3 * 0.044715 * x * x)
 
 
 def softplus(z):

@@ -211,41 +211,40 @@ def _gap_threshold(v, q):
     return np.float32(0.5 * (s[i] + s[i + 1])), float(gaps.max())
 
 
-def test_fused_closure_collide_matches_reference():
+def _closure_setup(seed):
+    """A random near-equilibrium state of the closure case and a network
+    whose training range leaves ~30% of the nodes to Smagorinsky."""
     c = _closure_case()
-    say("fused closure collide", c.nx, c.ny, c.nz)
-    rng = np.random.default_rng(5)
+    rng = np.random.default_rng(seed)
     f = random_state(rng, (c.nz, c.ny, c.nx), amp=2e-3, umax=0.03)
-    fs = _pre_collision(c, f)
-    X = _node_features(c, fs)
-    Xf = X.reshape(-1, 7)
+    Xf = _node_features(c, _pre_collision(c, f)).reshape(-1, 7)
     mu, sd = norm_of(Xf)
     lo = np.full(7, -INF, np.float32)
     hi = np.full(7, INF, np.float32)
-    lo[0], gap = _gap_threshold(Xf[:, 0], 0.3)          # ~30% of the nodes fall back to Smagorinsky
+    lo[0], gap = _gap_threshold(Xf[:, 0], 0.3)
     assert gap > 1e-4
-    nu_scale = np.float32(0.01)
-    p = M.glorot(31)
+    net = dict(p=M.glorot(31), mu=mu, sd=sd, lo=lo, hi=hi, nu_scale=np.float32(0.01))
     m = L.Mlp()
-    m.params = p
-    m.set_norm(mu, sd, lo, hi, nu_scale)
-    with L.Solver(c) as s:
-        s.set_populations(f)
-        s.set_closure(m, nu_cap=0.05)
-        s.step(1)
-        g = s.populations()
-    with L.Solver(c) as s:
-        s.set_populations(f)
-        s.step(1)
-        g_smag = s.populations()
-    # reference: the NumPy network's rate at each node, the oracle's collide
-    zb, _ = M.forward_bf16(p, M.normalise(Xf, mu, sd))
-    nut = np.minimum(nu_scale * M.softplus(zb.astype(np.float64)), 0.05).astype(np.float32)
-    inr = np.all((Xf >= lo) & (Xf <= hi), axis=1)
+    m.params = net["p"]
+    m.set_norm(mu, sd, lo, hi, net["nu_scale"])
+    return c, f, m, net
+
+
+def _reference_rates(c, fs, net):
+    """(in-range flags, per-node rate) the NumPy network gives for the
+    pre-collision populations fs."""
+    Xf = _node_features(c, fs).reshape(-1, 7)
+    zb, _ = M.forward_bf16(net["p"], M.normalise(Xf, net["mu"], net["sd"]))
+    nut = np.minimum(net["nu_scale"] * M.softplus(zb.astype(np.float64)), 0.05).astype(np.float32)
+    inr = np.all((Xf >= net["lo"]) & (Xf <= net["hi"]), axis=1)
     om0 = np.float32(c.omega)
     omega = (np.float32(1) / (np.float32(1) / om0 + np.float32(3) * nut)).astype(np.float32)
-    say("  in range", inr.mean(), "nu_t", nut[inr].min(), nut[inr].max())
-    assert 0.5 < inr.mean() < 0.9
+    return inr, omega, nut
+
+
+def _reference_collide(c, fs, inr, omega):
+    """The oracle's collide node by node: at the given rate (no LES) where
+    inr, at the Smagorinsky rate elsewhere."""
     pin = orc_params(c)
     pin.cs = 0.0
     pout = orc_params(c)
@@ -258,6 +257,26 @@ def test_fused_closure_collide_matches_reference():
             rr[:, n] = O.collide(pin, fr[:, n].copy())
         else:
             rr[:, n] = O.collide(pout, fr[:, n].copy())
+    return ref
+
+
+def test_fused_closure_collide_matches_reference():
+    c, f, m, net = _closure_setup(5)
+    say("fused closure collide", c.nx, c.ny, c.nz)
+    with L.Solver(c) as s:
+        s.set_populations(f)
+        s.set_closure(m, nu_cap=0.05)
+        s.step(1)
+        g = s.populations()
+    with L.Solver(c) as s:
+        s.set_populations(f)
+        s.step(1)
+        g_smag = s.populations()
+    fs = _pre_collision(c, f)
+    inr, omega, nut = _reference_rates(c, fs, net)
+    say("  in range", inr.mean(), "nu_t", nut[inr].min(), nut[inr].max())
+    assert 0.5 < inr.mean() < 0.9
+    ref = _reference_collide(c, fs, inr, omega)
     scale = np.abs(ref).max()
     err = np.abs(g - ref).max() / scale
     effect = np.abs(g_smag - ref).max() / scale
@@ -267,6 +286,35 @@ def test_fused_closure_collide_matches_reference():
     # nodes outside the range take the Smagorinsky rate: identical to the plain solver there
     out = ~inr.reshape(fs.shape[1:])
     assert np.abs(g[:, out] - g_smag[:, out]).max() < 1e-6 * scale
+
+
+def test_closure_refresh_interval_matches_reference():
+    """every = 3: the network runs on step 0 only; steps 1 and 2 collide at
+    the rates of step 0 (Smagorinsky, recomputed each step, where step 0's
+    inputs were out of range)."""
+    c, f, m, net = _closure_setup(7)
+    say("closure refresh interval")
+    with L.Solver(c) as s:
+        s.set_populations(f)
+        s.set_closure(m, every=3)
+        s.step(3)
+        g = s.populations()
+    with L.Solver(c) as s:
+        s.set_populations(f)
+        s.set_closure(m)
+        s.step(3)
+        g_every = s.populations()
+    fs = _pre_collision(c, f)
+    inr, omega, _ = _reference_rates(c, fs, net)
+    ref = _reference_collide(c, fs, inr, omega)
+    for _ in range(2):
+        ref = _reference_collide(c, _pre_collision(c, ref), inr, omega)
+    scale = np.abs(ref).max()
+    err = np.abs(g - ref).max() / scale
+    stale = np.abs(g_every - ref).max() / scale
+    say("  err", err, "vs refreshing every step", stale)
+    assert err < 1e-5
+    assert stale > 10 * 1e-5
 
 
 def test_closure_out_of_range_is_smagorinsky():
