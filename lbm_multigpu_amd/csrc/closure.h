@@ -317,7 +317,7 @@ __global__ void __launch_bounds__(TX* TY) k_stream_collide_closure(StepParams p,
 {
     using R = typename Arith<S>::R;
     __shared__ MlpChain s;
-    __shared__ short xb[TY][64 * MLP_INP];
+    __shared__ short xb[NWAVE][64 * MLP_INP];
     chain_stage(s, ca.chain);
     const int lb = logical_block();
     const int seg = lb % p.nseg;
@@ -355,10 +355,10 @@ __global__ void __launch_bounds__(TX* TY) k_stream_collide_closure(StepParams p,
         for (int q = 0; q < MLP_IN; ++q) in_range = in_range && feat[q] >= ca.nrm.lo[q] && feat[q] <= ca.nrm.hi[q];
         if (p.cs > 0.f) omega_les = les_omega(m, R(p.omega), R(p.cs));
     }
-    const int w = threadIdx.y;
+    const int w = wave_id();
 #pragma unroll
     for (int q = 0; q < MLP_INP; ++q)
-        xb[w][threadIdx.x * MLP_INP + q] = q < MLP_IN ? bf((feat[q] - ca.nrm.mu[q]) * ca.nrm.inv_sd[q]) : (short)0;
+        xb[w][lane_id() * MLP_INP + q] = q < MLP_IN ? bf((feat[q] - ca.nrm.mu[q]) * ca.nrm.inv_sd[q]) : (short)0;
     __syncthreads();
     const float zo = mlp_chain_infer(s, xb[w]);
     if (act != ACT_SKIP) {
@@ -366,9 +366,9 @@ __global__ void __launch_bounds__(TX* TY) k_stream_collide_closure(StepParams p,
             CollideParams<R> cp = collide_params<R>(p);
             if (in_range) {
                 const float nut = fminf(ca.nrm.nu_scale * softplus(zo), ca.nu_cap);
-                const float w = 1.f / (1.f / p.omega + 3.f * nut);
-                cp.omega = R(w);
-                if (ca.omega_out) *at(ca.omega_out, cell) = w;
+                const float wr = 1.f / (1.f / p.omega + 3.f * nut);
+                cp.omega = R(wr);
+                if (ca.omega_out) *at(ca.omega_out, cell) = wr;
             } else {
                 cp.omega = omega_les;
                 if (ca.omega_out) *at(ca.omega_out, cell) = 0.f;
@@ -379,23 +379,7 @@ __global__ void __launch_bounds__(TX* TY) k_stream_collide_closure(StepParams p,
         }
         node_store<S>(p, cell, f);
     }
-    if (p.fpart) {
-        __shared__ double red[3][TY];
-        fx = wave_sum(fx);
-        fy = wave_sum(fy);
-        fz = wave_sum(fz);
-        if (threadIdx.x == 0) {
-            red[0][threadIdx.y] = double(fx);
-            red[1][threadIdx.y] = double(fy);
-            red[2][threadIdx.y] = double(fz);
-        }
-        __syncthreads();
-        if (threadIdx.x < 3 && threadIdx.y == 0) {
-            double sum = 0;
-            for (int q = 0; q < TY; ++q) sum += red[threadIdx.x][q];
-            p.fpart[(long long)lb * 3 + threadIdx.x] = sum;
-        }
-    }
+    if (p.fpart) block_force_partial(p, lb, fx, fy, fz);
 }
 
 } // namespace lbm

@@ -224,8 +224,8 @@ static int validate(const lbm_config* c, std::string& why)
 static int alloc_level(lbm_ctx* ctx, Level& L, int nx, int ny, int nz, int storage)
 {
     L.nx = nx; L.ny = ny; L.nz = nz;
-    L.nxp = ((nx + TX - 1) / TX) * TX;
-    L.nseg = L.nxp / TX;
+    L.nxp = ((nx + 63) / 64) * 64;     // rows padded to whole wavefronts
+    L.nseg = (nx + TX - 1) / TX;       // workgroups along x
     L.nyb = (ny + TY - 1) / TY;
     L.qs = (long long)L.nxp * ny * nz;
     L.cells = (size_t)nx * ny * nz;
