@@ -313,20 +313,17 @@ struct ClosureArgs {
 // tau = tau0 + 3 nu_t, or the Smagorinsky rate where an input leaves the
 // training range.
 template <int MODEL, class S>
-__global__ void __launch_bounds__(TX* TY) k_stream_collide_closure(StepParams p, ClosureArgs ca)
+__global__ void __launch_bounds__(Shape<S>::THREADS) k_stream_collide_closure(StepParams p, ClosureArgs ca)
 {
     using R = typename Arith<S>::R;
     __shared__ MlpChain s;
-    __shared__ short xb[NWAVE][64 * MLP_INP];
+    constexpr int NW = Shape<S>::NW;
+    __shared__ short xb[NW][64 * MLP_INP];
     chain_stage(s, ca.chain);
     const int lb = logical_block();
-    const int seg = lb % p.nseg;
-    const int t2 = lb / p.nseg;
-    const int yb = t2 % p.nyb;
-    const int z = t2 / p.nyb;
-    const int x = seg * TX + threadIdx.x;
-    const int y = yb * TY + threadIdx.y;
-    const bool active = x < p.nx && y < p.ny;
+    int x, y, z;
+    const bool row_ok = wave_node<NW>(lb, p.nsegw, p.ny, p.nz, x, y, z);
+    const bool active = row_ok && x < p.nx;
     const int cell = active ? x + p.nxp * (y + p.ny * z) : 0;
     R fx = 0, fy = 0, fz = 0;
     R f[Q];
@@ -379,7 +376,7 @@ __global__ void __launch_bounds__(TX* TY) k_stream_collide_closure(StepParams p,
         }
         node_store<S>(p, cell, f);
     }
-    if (p.fpart) block_force_partial(p, lb, fx, fy, fz);
+    if (p.fpart) block_force_partial<NW>(p, lb, fx, fy, fz);
 }
 
 } // namespace lbm

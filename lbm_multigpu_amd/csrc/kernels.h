@@ -1,13 +1,13 @@
 // kernels.h -- HIP kernels of the D3Q19 stream-collide path for gfx950.
 //
-// One thread per node, workgroups of 256x1 threads: each workgroup owns 256
-// consecutive x of one (y, z) row (four wavefronts of 64), so each of the 19
-// population loads and stores of a workgroup is one contiguous 1 KB (FP32)
-// row segment; only the 10 directions with c_x != 0 are offset by one
-// element.  Consecutive workgroup ids walk the rows in memory order.  (A
-// 64x4 shape -- four rows of 256 B per plane -- and an XCD-slab renumbering
-// of the workgroups both measured 8-10 % below this on a 19-plane copy with
-// the same pull pattern; the renumbering stays available as a knob.)
+// One thread per node; each 64-lane wavefront owns 64 consecutive x of one
+// (y, z) row and a workgroup takes consecutive row segments in memory order
+// (wave_node; four for FP32, so each of the 19 population loads and stores
+// of a workgroup is one contiguous 1 KB stretch of a plane); only the 10
+// directions with c_x != 0 are offset by one element.  (A 64x4 shape -- four rows of 256 B per plane -- and an XCD-slab
+// renumbering of the workgroups both measured 8-10 % below this on a
+// 19-plane copy with the same pull pattern; the renumbering stays available
+// as a knob.)
 #pragma once
 
 #include "collide.h"
@@ -18,11 +18,11 @@ namespace lbm {
 enum Bc : int { BC_PERIODIC = 0, BC_CLAMP = 1, BC_WALL = 2 };
 
 // Tuning knobs (compile-time; the defaults are the measured best on MI355X)
-#ifndef LBM_WX
-#define LBM_WX 4 // wavefronts along x per workgroup (64 consecutive x each)
+#ifndef LBM_NWAVE
+#define LBM_NWAVE 4 // wavefronts per workgroup, FP32 and FP64 populations
 #endif
-#ifndef LBM_TY
-#define LBM_TY 1 // rows along y per workgroup
+#ifndef LBM_NWAVE16
+#define LBM_NWAVE16 1 // wavefronts per workgroup, 16-bit populations
 #endif
 #ifndef LBM_XCD_REMAP
 #define LBM_XCD_REMAP 0
@@ -31,14 +31,37 @@ enum Bc : int { BC_PERIODIC = 0, BC_CLAMP = 1, BC_WALL = 2 };
 #define LBM_MIN_BLOCKS 1 // __launch_bounds__ minimum workgroups per CU
 #endif
 
-constexpr int WX = LBM_WX;
-constexpr int TX = 64 * WX;     // threads along x per workgroup
-constexpr int TY = LBM_TY;      // rows per workgroup
-constexpr int NWAVE = WX * TY;  // wavefronts per workgroup
+// Workgroup shape per population type: NW wavefronts of 64 lanes on NW
+// consecutive row segments.  FP32 streams best with four (one contiguous
+// 1 KB stretch per plane and workgroup); the VALU-bound 16-bit kernels run
+// best with one (measured on the 512x128x128 channel: 35.1 vs 32.2 GLUPS
+// FP32, 46.2 vs 41.7 GLUPS FP16).
+template <class S>
+struct Shape {
+    static constexpr int NW = sizeof(S) == 2 ? LBM_NWAVE16 : LBM_NWAVE;
+    static constexpr int THREADS = 64 * NW;
+};
+__host__ inline int nwave_of_storage(int storage) { return (storage == 1 || storage == 2) ? LBM_NWAVE16 : LBM_NWAVE; }
 
 // wavefront index within the workgroup and lane within the wavefront
-__device__ __forceinline__ int wave_id() { return (int)(threadIdx.x >> 6) + WX * (int)threadIdx.y; }
+__device__ __forceinline__ int wave_id() { return (int)(threadIdx.x >> 6); }
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+// Node of this lane: wavefronts enumerate the 64-node row segments in memory
+// order (segment fastest, then y, then z) and workgroup lb takes NW
+// consecutive ones, so every workgroup streams one contiguous stretch of
+// each plane whatever nx is.  Returns false past the last row.
+template <int NW>
+__device__ __forceinline__ bool wave_node(int lb, int nsegw, int ny, int nz, int& x, int& y, int& z)
+{
+    // wave-uniform: the divisions run on the scalar unit, not per lane
+    const int g = __builtin_amdgcn_readfirstlane(lb * NW + wave_id());
+    const int row = g / nsegw;
+    x = (g - row * nsegw) * 64 + lane_id();
+    y = row % ny;
+    z = row / ny;
+    return z < nz;
+}
 
 struct StepParams {
     const void* fin;
@@ -48,7 +71,7 @@ struct StepParams {
     double* fpart;             // momentum-exchange partials [blocks*3], or nullptr
     long long qs;              // elements per direction plane
     int nx, ny, nz, nxp;
-    int nseg, nyb;             // x segments of 64 and y groups of 4
+    int nsegw;                 // 64-node wavefront segments per padded row (nxp / 64)
     int bcx, bcy, bcz;
     int halfway;
     float omega, cs, dpdy, u_lid;
@@ -282,10 +305,10 @@ __device__ __forceinline__ void node_update(const StepParams& p, int x, int y, i
 
 // Momentum-exchange partial of one workgroup: wavefront sums, then the
 // wavefronts in a fixed order (deterministic), in FP64.
-template <class R>
+template <int NW, class R>
 __device__ __forceinline__ void block_force_partial(const StepParams& p, int lb, R fx, R fy, R fz)
 {
-    __shared__ double red[3][NWAVE];
+    __shared__ double red[3][NW];
     fx = wave_sum(fx);
     fy = wave_sum(fy);
     fz = wave_sum(fz);
@@ -298,25 +321,22 @@ __device__ __forceinline__ void block_force_partial(const StepParams& p, int lb,
     if (threadIdx.x < 3 && threadIdx.y == 0) {
         double s = 0;
 #pragma unroll
-        for (int w = 0; w < NWAVE; ++w) s += red[threadIdx.x][w];
+        for (int w = 0; w < NW; ++w) s += red[threadIdx.x][w];
         p.fpart[(long long)lb * 3 + threadIdx.x] = s;
     }
 }
 
 template <int MODEL, class S>
-__global__ void __launch_bounds__(TX* TY, LBM_MIN_BLOCKS) k_stream_collide(StepParams p)
+__global__ void __launch_bounds__(Shape<S>::THREADS, LBM_MIN_BLOCKS) k_stream_collide(StepParams p)
 {
     using R = typename Arith<S>::R;
+    constexpr int NW = Shape<S>::NW;
     const int lb = logical_block();
-    const int seg = lb % p.nseg;
-    const int t2 = lb / p.nseg;
-    const int yb = t2 % p.nyb;
-    const int z = t2 / p.nyb;
-    const int x = seg * TX + threadIdx.x;
-    const int y = yb * TY + threadIdx.y;
+    int x, y, z;
+    const bool row_ok = wave_node<NW>(lb, p.nsegw, p.ny, p.nz, x, y, z);
     R fx = 0, fy = 0, fz = 0;
-    if (x < p.nx && y < p.ny) node_update<MODEL, S>(p, x, y, z, fx, fy, fz);
-    if (p.fpart) block_force_partial(p, lb, fx, fy, fz);
+    if (row_ok && x < p.nx) node_update<MODEL, S>(p, x, y, z, fx, fy, fz);
+    if (p.fpart) block_force_partial<NW>(p, lb, fx, fy, fz);
 }
 
 // Deterministic FP64 reduction of ncomp-component partials (fixed order:
@@ -341,20 +361,17 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const double* part, lon
 
 // Per-block FP64 sums of (rho, jx, jy, jz) over non-solid nodes.
 template <class S>
-__global__ void __launch_bounds__(TX* TY) k_mass_momentum(const S* f, const unsigned char* mask,
+__global__ void __launch_bounds__(Shape<S>::THREADS) k_mass_momentum(const S* f, const unsigned char* mask,
                                                         long long qs, int nx, int ny, int nxp,
-                                                        int nseg, int nyb, double* part)
+                                                        int nsegw, int nz, double* part)
 {
     using R = typename Arith<S>::R;
     const int lb = blockIdx.x;
-    const int seg = lb % nseg;
-    const int t2 = lb / nseg;
-    const int yb = t2 % nyb;
-    const int z = t2 / nyb;
-    const int x = seg * TX + threadIdx.x;
-    const int y = yb * TY + threadIdx.y;
+    int x, y, z;
+    constexpr int NW = Shape<S>::NW;
+    const bool row_ok = wave_node<NW>(lb, nsegw, ny, nz, x, y, z);
     double acc[4] = {0, 0, 0, 0};
-    if (x < nx && y < ny) {
+    if (row_ok && x < nx) {
         const int cell = x + nxp * (y + ny * z);
         if (!(mask && is_solid(mask[cell]))) {
             R r = 0, a = 0, b = 0, c = 0;
@@ -369,7 +386,7 @@ __global__ void __launch_bounds__(TX* TY) k_mass_momentum(const S* f, const unsi
             acc[0] = r; acc[1] = a; acc[2] = b; acc[3] = c;
         }
     }
-    __shared__ double red[4][NWAVE];
+    __shared__ double red[4][NW];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const double s = wave_sum(acc[k]);
@@ -378,7 +395,7 @@ __global__ void __launch_bounds__(TX* TY) k_mass_momentum(const S* f, const unsi
     __syncthreads();
     if (threadIdx.x < 4 && threadIdx.y == 0) {
         double s = 0;
-        for (int w = 0; w < NWAVE; ++w) s += red[threadIdx.x][w];
+        for (int w = 0; w < NW; ++w) s += red[threadIdx.x][w];
         part[(long long)lb * 4 + threadIdx.x] = s;
     }
 }

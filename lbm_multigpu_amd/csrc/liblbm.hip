@@ -26,7 +26,7 @@ namespace {
 thread_local std::string g_last_error;
 
 struct Level {
-    int nx = 0, ny = 0, nz = 0, nxp = 0, nseg = 0, nyb = 0;
+    int nx = 0, ny = 0, nz = 0, nxp = 0, nsegw = 0;
     long long qs = 0;
     size_t cells = 0;
     int storage = LBM_STORE_F32;
@@ -63,9 +63,8 @@ struct lbm_ctx {
     RefineGeom rg{};
     float* Mold = nullptr;     // footprint moments at the step's start
     float* Mnew = nullptr;     // and at its end
-    int* cf_blocks = nullptr;  // origins of the 8^3 fine blocks that meet the frame
-    int n_cf = 0;
-    int fc_nb[3] = {0, 0, 0};  // FC workgroups per axis (4^3 coarse nodes each)
+    int fc_nb[3] = {0, 0, 0};  // FC workgroups per axis (FC_TX x FC_TY x FC_TZ coarse nodes each)
+    float* Pcf = nullptr;      // footprint moments + strain rates of one CF pass (16 floats per node)
     // learned closure
     lbm_mlp* mlp = nullptr;    // not owned
     float* dwall = nullptr;    // wall distance per node (padded layout)
@@ -225,13 +224,13 @@ static int alloc_level(lbm_ctx* ctx, Level& L, int nx, int ny, int nz, int stora
 {
     L.nx = nx; L.ny = ny; L.nz = nz;
     L.nxp = ((nx + 63) / 64) * 64;     // rows padded to whole wavefronts
-    L.nseg = (nx + TX - 1) / TX;       // workgroups along x
-    L.nyb = (ny + TY - 1) / TY;
+    L.nsegw = L.nxp / 64;              // wavefront segments per row
     L.qs = (long long)L.nxp * ny * nz;
     L.cells = (size_t)nx * ny * nz;
     L.storage = storage;
     L.esize = elem_size(storage);
-    L.blocks = L.nseg * L.nyb * nz;
+    const int nw = nwave_of_storage(L.storage);
+    L.blocks = (int)(((long long)L.nsegw * ny * nz + nw - 1) / nw);
     const size_t lattice = (size_t)Q * (size_t)L.qs * L.esize;
     HIP_TRY(ctx, hipMalloc(&L.fa, lattice));
     HIP_TRY(ctx, hipMalloc(&L.fb, lattice));
@@ -327,7 +326,7 @@ extern "C" void lbm_destroy(lbm_ctx* ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (int l = 0; l < 2; ++l) free_level(ctx->lv[l]);
     for (void* p : {(void*)ctx->u_in, (void*)ctx->fpart, (void*)ctx->fhist, (void*)ctx->red, (void*)ctx->mpart,
-                    (void*)ctx->Mold, (void*)ctx->Mnew, (void*)ctx->cf_blocks, (void*)ctx->dwall,
+                    (void*)ctx->Mold, (void*)ctx->Mnew, (void*)ctx->Pcf, (void*)ctx->dwall,
                     (void*)ctx->omega_field})
         if (p) (void)hipFree(p);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -541,7 +540,7 @@ static StepParams step_params(lbm_ctx* ctx, const Level& L, float omega)
     p.fpart = nullptr;
     p.qs = L.qs;
     p.nx = L.nx; p.ny = L.ny; p.nz = L.nz; p.nxp = L.nxp;
-    p.nseg = L.nseg; p.nyb = L.nyb;
+    p.nsegw = L.nsegw;
     p.bcx = c.bc[0]; p.bcy = c.bc[1]; p.bcz = c.bc[2];
     p.halfway = c.halfway;
     p.omega = omega;
@@ -566,7 +565,7 @@ static StepParams step_params(lbm_ctx* ctx, const Level& L, float omega)
 template <class S>
 static void launch_sc(int model, const StepParams& p, int blocks, hipStream_t s)
 {
-    const dim3 blk(TX, TY), grd(blocks);
+    const dim3 blk(Shape<S>::THREADS), grd(blocks);
     switch (model) {
     case LBM_MODEL_BGK: hipLaunchKernelGGL((k_stream_collide<MODEL_BGK, S>), grd, blk, 0, s, p); break;
     case LBM_MODEL_MRT_CPU3D: hipLaunchKernelGGL((k_stream_collide<MODEL_MRT_CPU3D, S>), grd, blk, 0, s, p); break;
@@ -584,7 +583,7 @@ static void launch_closure(lbm_ctx* ctx, const StepParams& p, int blocks, float*
     ca.dwall = ctx->dwall;
     ca.nrm = ctx->mlp->nrm;
     ca.nu_cap = ctx->nu_cap;
-    const dim3 blk(TX, TY), grd(blocks);
+    const dim3 blk(Shape<S>::THREADS), grd(blocks);
     if (ctx->cfg.model == LBM_MODEL_MRT_DHUMIERES)
         hipLaunchKernelGGL((k_stream_collide_closure<MODEL_MRT_DHUM, S>), grd, blk, 0, ctx->stream, p, ca);
     else
@@ -653,8 +652,10 @@ static int refine_setup(lbm_ctx* ctx)
         g.a[a] = (int)std::lround(x0[a] - 0.25f);
         g.nf[a] = nf[a];
         g.nc[a] = nf[a] / 2 + 1;
-        ctx->fc_nb[a] = (nf[a] / 2 - 3 + 3) / 4;
     }
+    ctx->fc_nb[0] = (nf[0] / 2 - 3 + FC_TX - 1) / FC_TX;
+    ctx->fc_nb[1] = (nf[1] / 2 - 3 + FC_TY - 1) / FC_TY;
+    ctx->fc_nb[2] = (nf[2] / 2 - 3 + FC_TZ - 1) / FC_TZ;
     g.omega_c = c.omega;
     // omegaLR as main() computes it (MRTtest/DynInlet3.cu:2870)
     g.omega_f = 2.0f / (1.0f + 2.0f * (2.0f / c.omega - 1.0f));
@@ -663,17 +664,7 @@ static int refine_setup(lbm_ctx* ctx)
     const size_t NC = (size_t)g.nc[0] * g.nc[1] * g.nc[2];
     HIP_TRY(ctx, hipMalloc(&ctx->Mold, 9 * NC * sizeof(float)));
     HIP_TRY(ctx, hipMalloc(&ctx->Mnew, 9 * NC * sizeof(float)));
-    std::vector<int> blocks;
-    for (int k = 0; k < nf[2]; k += 8)
-        for (int j = 0; j < nf[1]; j += 8)
-            for (int i = 0; i < nf[0]; i += 8) {
-                const bool interior = i >= 2 && i + 8 <= nf[0] - 2 && j >= 2 && j + 8 <= nf[1] - 2 && k >= 2 &&
-                                      k + 8 <= nf[2] - 2;
-                if (!interior) { blocks.push_back(i); blocks.push_back(j); blocks.push_back(k); }
-            }
-    ctx->n_cf = (int)(blocks.size() / 3);
-    HIP_TRY(ctx, hipMalloc(&ctx->cf_blocks, blocks.size() * sizeof(int)));
-    HIP_TRY(ctx, hipMemcpy(ctx->cf_blocks, blocks.data(), blocks.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMalloc(&ctx->Pcf, 16 * NC * sizeof(float)));
     return LBM_OK;
 }
 
@@ -722,8 +713,12 @@ static int refine_cf(lbm_ctx* ctx, float alpha)
     Level& F = ctx->lv[1];
     with_storage(F.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        hipLaunchKernelGGL(k_interp_cf<S>, dim3(ctx->n_cf), dim3(8, 8, 8), 0, ctx->stream, (S*)F.fa, F.qs, F.nxp,
-                           F.mask, ctx->cf_blocks, ctx->Mold, ctx->Mnew, alpha, ctx->rg);
+        const long long NC = (long long)ctx->rg.nc[0] * ctx->rg.nc[1] * ctx->rg.nc[2];
+        hipLaunchKernelGGL(k_cf_prep, dim3((unsigned)((NC + 255) / 256)), dim3(256), 0, ctx->stream, ctx->Mold,
+                           ctx->Mnew, alpha, NC, (f32x4v*)ctx->Pcf);
+        const long long n = frame_nodes(ctx->rg.nf);
+        hipLaunchKernelGGL(k_interp_cf<S>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (S*)F.fa,
+                           F.qs, F.nxp, F.mask, (const f32x4v*)ctx->Pcf, ctx->rg);
         return 0;
     });
     HIP_TRY(ctx, hipGetLastError());
@@ -737,7 +732,7 @@ static int refine_fc(lbm_ctx* ctx)
     const int nb = ctx->fc_nb[0] * ctx->fc_nb[1] * ctx->fc_nb[2];
     with_storage(F.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        hipLaunchKernelGGL(k_interp_fc<S>, dim3(nb), dim3(8, 8, 8), 0, ctx->stream, (const S*)F.fa, F.qs, F.nxp,
+        hipLaunchKernelGGL(k_interp_fc<S>, dim3(nb), dim3(256), 0, ctx->stream, (const S*)F.fa, F.qs, F.nxp,
                            (S*)L.fa, L.qs, L.nxp, L.ny, L.mask, ctx->rg, ctx->fc_nb[0], ctx->fc_nb[1]);
         return 0;
     });
@@ -913,8 +908,8 @@ extern "C" int lbm_get_mass_momentum(lbm_ctx* ctx, int level, double out[4])
     }
     with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        hipLaunchKernelGGL(k_mass_momentum<S>, dim3(L.blocks), dim3(TX, TY), 0, ctx->stream, (const S*)L.fa,
-                           L.mask, L.qs, L.nx, L.ny, L.nxp, L.nseg, L.nyb, part);
+        hipLaunchKernelGGL(k_mass_momentum<S>, dim3(L.blocks), dim3(Shape<S>::THREADS), 0, ctx->stream, (const S*)L.fa,
+                           L.mask, L.qs, L.nx, L.ny, L.nxp, L.nsegw, L.nz, part);
         return 0;
     });
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, ctx->stream, part, (long long)L.blocks, 4,

@@ -181,63 +181,110 @@ __global__ void __launch_bounds__(256) k_extract(const S* fc, long long qs, int 
     for (int m = 0; m < 9; ++m) M[m * NC + o] = mom[m];
 }
 
-// Coarse -> fine frame.  One workgroup per 8x8x8 fine block that meets the
-// frame (origins listed by the host); the 5x5x5 coarse tile it needs is
-// staged in LDS with its strain rates, blended in time by alpha.
-template <class S>
-__global__ void __launch_bounds__(512) k_interp_cf(S* ff, long long fqs, int fnxp, const unsigned char* fmask,
-                                                   const int* blocks, const float* Mold, const float* Mnew,
-                                                   float alpha, RefineGeom g)
+// Frame node number o of the CF frame (two fine nodes thick on all six faces)
+// -> (i, j, k).  Enumerated as the z slabs (k < 2 or k >= nf_z - 2), then the
+// y slabs of the remaining planes, then the x slabs of the remaining rows; x
+// runs fastest in the first two, so consecutive threads write consecutive x.
+__device__ __forceinline__ bool frame_node(long long o, const RefineGeom& g, int& i, int& j, int& k)
 {
-    __shared__ float tm[125][9];
-    __shared__ float ts[125][6];
-    const int ox = blocks[3 * blockIdx.x], oy = blocks[3 * blockIdx.x + 1], oz = blocks[3 * blockIdx.x + 2];
-    const int t = threadIdx.x + 8 * (threadIdx.y + 8 * threadIdx.z);
-    const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
-    const long long NC = (long long)ncx * ncy * ncz;
-    if (t < 125) {
-        const int cx = ox / 2 + t % 5, cy = oy / 2 + (t / 5) % 5, cz = oz / 2 + t / 25;
-        float m[9];
-        if (cx < ncx && cy < ncy && cz < ncz) {
-            const long long o = cx + (long long)ncx * (cy + (long long)ncy * cz);
-#pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                const float a0 = Mold[k * NC + o];
-                m[k] = alpha == 0.f ? a0 : (1.f - alpha) * a0 + alpha * Mnew[k * NC + o];
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < 9; ++k) m[k] = 0.f;
-        }
-        float Sv[6];
-        strain_rate(m, Sv);
-#pragma unroll
-        for (int k = 0; k < 9; ++k) tm[t][k] = m[k];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) ts[t][k] = Sv[k];
+    const int nx = g.nf[0], ny = g.nf[1], nz = g.nf[2];
+    const long long nZ = 4LL * nx * ny, nY = 4LL * nx * (nz - 4), nX = 4LL * (ny - 4) * (nz - 4);
+    if (o < nZ) {
+        i = int(o % nx);
+        const long long r = o / nx;
+        j = int(r % ny);
+        const int s = int(r / ny);
+        k = s < 2 ? s : nz - 4 + s;
+        return true;
     }
-    __syncthreads();
-    const int i = ox + threadIdx.x, j = oy + threadIdx.y, k = oz + threadIdx.z;
-    const int nfx = g.nf[0], nfy = g.nf[1], nfz = g.nf[2];
-    if (i >= nfx || j >= nfy || k >= nfz) return;
-    if (i >= 2 && i < nfx - 2 && j >= 2 && j < nfy - 2 && k >= 2 && k < nfz - 2) return;
+    o -= nZ;
+    if (o < nY) {
+        i = int(o % nx);
+        const long long r = o / nx;
+        k = 2 + int(r % (nz - 4));
+        const int s = int(r / (nz - 4));
+        j = s < 2 ? s : ny - 4 + s;
+        return true;
+    }
+    o -= nY;
+    if (o < nX) {
+        const int s = int(o % 4);
+        const long long r = o / 4;
+        i = s < 2 ? s : nx - 4 + s;
+        j = 2 + int(r % (ny - 4));
+        k = 2 + int(r / (ny - 4));
+        return true;
+    }
+    return false;
+}
+
+__host__ __device__ inline long long frame_nodes(const int nf[3])
+{
+    return 4LL * nf[0] * nf[1] + 4LL * nf[0] * (nf[2] - 4) + 4LL * (nf[1] - 4) * (nf[2] - 4);
+}
+
+// Footprint node data for one CF pass: the 9 moments blended in time by
+// alpha and their 6 strain rates, 16 floats per node (AoS, so a CF thread
+// fetches a corner with four 16-byte loads).
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) k_cf_prep(const float* Mold, const float* Mnew, float alpha, long long NC,
+                                                 f32x4v* P)
+{
+    const long long o = blockIdx.x * 256LL + threadIdx.x;
+    if (o >= NC) return;
+    float m[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const float a0 = Mold[q * NC + o];
+        m[q] = alpha == 0.f ? a0 : (1.f - alpha) * a0 + alpha * Mnew[q * NC + o];
+    }
+    float Sv[6];
+    strain_rate(m, Sv);
+    P[4 * o + 0] = f32x4v{m[0], m[1], m[2], m[3]};
+    P[4 * o + 1] = f32x4v{m[4], m[5], m[6], m[7]};
+    P[4 * o + 2] = f32x4v{m[8], Sv[0], Sv[1], Sv[2]};
+    P[4 * o + 3] = f32x4v{Sv[3], Sv[4], Sv[5], 0.f};
+}
+
+// Coarse -> fine frame, one thread per frame node: the 8 surrounding
+// footprint nodes' blended moments and strain rates (k_cf_prep, L2-resident
+// and shared by neighbouring threads), then trilinear interpolation, ORDER-2
+// correction and the LES-aware rescale.
+template <class S>
+__global__ void __launch_bounds__(256) k_interp_cf(S* ff, long long fqs, int fnxp, const unsigned char* fmask,
+                                                   const f32x4v* P, RefineGeom g)
+{
+    int i, j, k;
+    if (!frame_node(blockIdx.x * 256LL + threadIdx.x, g, i, j, k)) return;
+    const int nfy = g.nf[1];
     const int fcell = i + fnxp * (j + nfy * k);
     if (fmask && is_solid(fmask[fcell])) return;
-    const int lx = threadIdx.x / 2, ly = threadIdx.y / 2, lz = threadIdx.z / 2;
+    const int ncx = g.nc[0], ncy = g.nc[1];
+    const int lx = i >> 1, ly = j >> 1, lz = k >> 1;
     const float xf = (i & 1) ? 0.75f : 0.25f, yf = (j & 1) ? 0.75f : 0.25f, zf = (k & 1) ? 0.75f : 0.25f;
+    f32x4v pc[8][4];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
+        const long long o = (lx + dx) + (long long)ncx * ((ly + dy) + (long long)ncy * (lz + dz));
+#pragma unroll
+        for (int v = 0; v < 4; ++v) pc[c][v] = P[4 * o + v];
+    }
     float Sc[8][6], uc[8][3], mom[9];
 #pragma unroll
     for (int m = 0; m < 9; ++m) mom[m] = 0.f;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const int dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
-        const int l = (lx + dx) + 5 * ((ly + dy) + 5 * (lz + dz));
+        const float m[9] = {pc[c][0][0], pc[c][0][1], pc[c][0][2], pc[c][0][3], pc[c][1][0],
+                            pc[c][1][1], pc[c][1][2], pc[c][1][3], pc[c][2][0]};
         const float w = (dx ? xf : 1.f - xf) * (dy ? yf : 1.f - yf) * (dz ? zf : 1.f - zf);
 #pragma unroll
-        for (int m = 0; m < 9; ++m) mom[m] += tm[l][m] * w;
-#pragma unroll
-        for (int m = 0; m < 6; ++m) Sc[c][m] = ts[l][m];
-        uc[c][0] = tm[l][1]; uc[c][1] = tm[l][2]; uc[c][2] = tm[l][3];
+        for (int q = 0; q < 9; ++q) mom[q] += m[q] * w;
+        Sc[c][0] = pc[c][2][1]; Sc[c][1] = pc[c][2][2]; Sc[c][2] = pc[c][2][3];
+        Sc[c][3] = pc[c][3][0]; Sc[c][4] = pc[c][3][1]; Sc[c][5] = pc[c][3][2];
+        uc[c][0] = m[1]; uc[c][1] = m[2]; uc[c][2] = m[3];
     }
     order2(Sc, uc, mom, xf, yf, zf, false);
     // InvertPhysicalMoments_LES_cf (:540-595)
@@ -256,35 +303,45 @@ __global__ void __launch_bounds__(512) k_interp_cf(S* ff, long long fqs, int fnx
     for (int q = 0; q < Q; ++q) ff[q * fqs + fcell] = Store<S>::from_real(typename Arith<S>::R(f[q]), q);
 }
 
-// Fine -> coarse.  One workgroup per 4x4x4 coarse nodes: its 512 threads load
-// the 8x8x8 fine nodes below them (coalesced along x) and reduce in LDS.
+// Fine -> coarse.  One workgroup per tile of 32x2x2 coarse nodes: its 256
+// threads load the 64x4x4 fine nodes below them as 16 contiguous 64-node
+// rows (one wavefront per row and plane), reduce them to the 9 physical
+// moments in LDS, and 128 threads then finish one coarse node each.
+constexpr int FC_TX = 32, FC_TY = 2, FC_TZ = 2;
+
 template <class S>
-__global__ void __launch_bounds__(512) k_interp_fc(const S* ff, long long fqs, int fnxp, S* fc, long long cqs,
+__global__ void __launch_bounds__(256) k_interp_fc(const S* ff, long long fqs, int fnxp, S* fc, long long cqs,
                                                    int cnxp, int cny, const unsigned char* cmask, RefineGeom g,
                                                    int nbx, int nby)
 {
-    __shared__ float fm[512][9];
+    __shared__ float fm[2 * FC_TZ][2 * FC_TY][2 * FC_TX][9]; // fine moments [fz][fy][fx][m]
     const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
-    // coarse nodes a+2+4b .. a+2+4b+3; fine nodes 2(cx-a)-1 .. (8 per axis)
-    const int fx0 = 2 * (2 + 4 * bx) - 1, fy0 = 2 * (2 + 4 * by) - 1, fz0 = 2 * (2 + 4 * bz) - 1;
-    const int t = threadIdx.x + 8 * (threadIdx.y + 8 * threadIdx.z);
-    const int i = fx0 + threadIdx.x, j = fy0 + threadIdx.y, k = fz0 + threadIdx.z;
-    float mom[9];
+    // coarse nodes a + 2 + (FC_T* b) + l; fine nodes 2(c - a) - 1 .. 2(c - a)
+    const int fx0 = 2 * (2 + FC_TX * bx) - 1, fy0 = 2 * (2 + FC_TY * by) - 1, fz0 = 2 * (2 + FC_TZ * bz) - 1;
+    const int lxf = threadIdx.x & 63, row0 = threadIdx.x >> 6; // 4 wavefronts
 #pragma unroll
-    for (int m = 0; m < 9; ++m) mom[m] = 0.f;
-    if (i < g.nf[0] && j < g.nf[1] && k < g.nf[2]) {
-        const int fcell = i + fnxp * (j + g.nf[1] * k);
-        float f[Q];
+    for (int rr = 0; rr < 4; ++rr) {
+        const int row = row0 + 4 * rr;          // 0..15: (fy, fz) of the tile
+        const int ly = row & 3, lz = row >> 2;
+        const int i = fx0 + lxf, j = fy0 + ly, k = fz0 + lz;
+        float mom[9];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) f[q] = float(Store<S>::to_real(ff[q * fqs + fcell], q));
-        phys_moments(f, mom);
+        for (int m = 0; m < 9; ++m) mom[m] = 0.f;
+        if (i < g.nf[0] && j < g.nf[1] && k < g.nf[2]) {
+            const int fcell = i + fnxp * (j + g.nf[1] * k);
+            float f[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) f[q] = float(Store<S>::to_real(ff[q * fqs + fcell], q));
+            phys_moments(f, mom);
+        }
+#pragma unroll
+        for (int m = 0; m < 9; ++m) fm[lz][ly][lxf][m] = mom[m];
     }
-#pragma unroll
-    for (int m = 0; m < 9; ++m) fm[t][m] = mom[m];
     __syncthreads();
-    if (t >= 64) return;
-    const int lx = t % 4, ly = (t / 4) % 4, lz = t / 16;
-    const int cx = g.a[0] + 2 + 4 * bx + lx, cy = g.a[1] + 2 + 4 * by + ly, cz = g.a[2] + 2 + 4 * bz + lz;
+    const int t = threadIdx.x;
+    if (t >= FC_TX * FC_TY * FC_TZ) return;
+    const int lx = t % FC_TX, ly = (t / FC_TX) % FC_TY, lz = t / (FC_TX * FC_TY);
+    const int cx = g.a[0] + 2 + FC_TX * bx + lx, cy = g.a[1] + 2 + FC_TY * by + ly, cz = g.a[2] + 2 + FC_TZ * bz + lz;
     if (cx > g.a[0] + g.nf[0] / 2 - 2 || cy > g.a[1] + g.nf[1] / 2 - 2 || cz > g.a[2] + g.nf[2] / 2 - 2) return;
     const int ccell = cx + cnxp * (cy + cny * cz);
     if (cmask && is_solid(cmask[ccell])) return;
@@ -294,10 +351,9 @@ __global__ void __launch_bounds__(512) k_interp_fc(const S* ff, long long fqs, i
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const int dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
-        const int l = (2 * lx + dx) + 8 * ((2 * ly + dy) + 8 * (2 * lz + dz));
         float m[9];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) m[q] = fm[l][q];
+        for (int q = 0; q < 9; ++q) m[q] = fm[2 * lz + dz][2 * ly + dy][2 * lx + dx][q];
 #pragma unroll
         for (int q = 0; q < 9; ++q) avg[q] += m[q];
         strain_rate(m, Sc[c]);
