@@ -71,6 +71,8 @@ extern "C" {
 #define LBM_NODE_INLET_WEST 3   /* im 400: West_Extrap (MRTtest/DynInlet3.cu:973-1017) */
 #define LBM_NODE_OUTLET_EAST 4  /* im 300: East_Extrap (MRTtest/DynInlet3.cu:926-971) */
 #define LBM_NODE_LID_WEST_REG 5 /* im 53: DirichletWest_Regularized (CPU_3D/CPU_3D.cpp:974) */
+#define LBM_NODE_SYM_XMIN 6     /* im 26: xsymmetry_bot, x = 0 symmetry plane (MRTtest/DynInlet3.cu:1020-1075) */
+#define LBM_NODE_SYM_XMAX 7     /* im 25: xsymmetry_top, x = nx-1 symmetry plane (MRTtest/DynInlet3.cu:1077-1137) */
 
 /* initial state of lbm_init_equilibrium() */
 #define LBM_INIT_BGK 0 /* explicit BGK feq (CPU_3D/CPU_3D.cpp:1218-1254) */

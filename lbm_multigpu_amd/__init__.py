@@ -21,12 +21,13 @@ MODEL_BGK, MODEL_MRT_CPU3D, MODEL_MRT_LES, MODEL_MRT_DHUMIERES = 0, 1, 2, 3
 BC_PERIODIC, BC_CLAMP, BC_WALL = 0, 1, 2
 STORE_F32, STORE_F16, STORE_BF16, STORE_F64 = 0, 1, 2, 3
 GEOM_NONE, GEOM_CHANNEL, GEOM_CPU3D = 0, 1, 2
-NODE_FLUID, NODE_SOLID, NODE_SOLID_FORCE, NODE_INLET_WEST, NODE_OUTLET_EAST, NODE_LID_WEST_REG = range(6)
+(NODE_FLUID, NODE_SOLID, NODE_SOLID_FORCE, NODE_INLET_WEST, NODE_OUTLET_EAST, NODE_LID_WEST_REG,
+ NODE_SYM_XMIN, NODE_SYM_XMAX) = range(8)
 INIT_BGK, INIT_MRT = 0, 1
 
 # reference `im` code of each node code (MRTtest/DynInlet3.cu:1286-1390, CPU_3D/CPU_3D.cpp:1142)
 NODE_TO_IM = {NODE_FLUID: 0, NODE_SOLID: 1, NODE_SOLID_FORCE: 10, NODE_INLET_WEST: 400,
-              NODE_OUTLET_EAST: 300, NODE_LID_WEST_REG: 53}
+              NODE_OUTLET_EAST: 300, NODE_LID_WEST_REG: 53, NODE_SYM_XMIN: 26, NODE_SYM_XMAX: 25}
 IM_TO_NODE = {v: k for k, v in NODE_TO_IM.items()}
 
 

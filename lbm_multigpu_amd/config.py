@@ -224,7 +224,7 @@ def write_inputs(cfg, path=None):
     o2 = cfg.obst2
     lines = [
         f"Base domain size \t{cfg.nx}x{cfg.ny}x{cfg.nz}",
-        "Base blocksize: \t64x4x1",
+        "Base blocksize: \t256x1x1",
         f"Obst1 location: \t({_g(o1[0])},{_g(o1[1])},{_g(o1[2])})",
         f"Obst1 radius: \t{_g(cfg.obst1_r)}",
         f"Obst2 location: \t({_g(o2[0])},{_g(o2[1])},{_g(o2[2])})",
@@ -244,7 +244,7 @@ def write_inputs(cfg, path=None):
         f"LR domain size \t{cfg.lr_size[0]}x{cfg.lr_size[1]}x{cfg.lr_size[2]}",
         f"LR factor \t{_g(cfg.lr_factor)}",
         f"LR location \t{_g(cfg.lr_loc[0])}x{_g(cfg.lr_loc[1])}x{_g(cfg.lr_loc[2])}",
-        "LR blocksize: \t64x4x1",
+        "LR blocksize: \t256x1x1",
         f"omega in LR \t: {_g(cfg.omega_lr or omega_lr(cfg.omega, cfg.lr_factor))}",
         f"GPUs per node \t: {cfg.gpus}",
         "# lbm-mi355x",

@@ -336,4 +336,47 @@ __device__ __forceinline__ void lid_west_reg(R f[Q], R u_lid)
     }
 }
 
+// x = 0 symmetry plane, xsymmetry_bot (MRTtest/DynInlet3.cu:1020-1075):
+// the populations entering through the plane copy their mirror images, with
+// the reference's own treatment of the y/z edges and corners.  The copies
+// are sequential (later ones read earlier results), as coded.
+template <class R>
+__device__ __forceinline__ void xsym_bot(R f[Q], int y, int z, int ny, int nz)
+{
+    if (y == 0 && z == 0) {
+        f[2] = f[4]; f[13] = f[18]; f[11] = f[18]; f[16] = f[18]; f[6] = f[7]; f[9] = f[14]; f[12] = f[17];
+    } else if (y == 0 && z == nz - 1) {
+        f[4] = f[2]; f[11] = f[13]; f[18] = f[13]; f[16] = f[13]; f[6] = f[7]; f[14] = f[9]; f[17] = f[12];
+    } else if (y == ny - 1 && z == 0) {
+        f[4] = f[2]; f[11] = f[16]; f[18] = f[16]; f[13] = f[16]; f[7] = f[6]; f[9] = f[14]; f[12] = f[17];
+    } else if (y == ny - 1 && z == nz - 1) {
+        f[4] = f[2]; f[16] = f[11]; f[18] = f[11]; f[13] = f[11]; f[7] = f[6]; f[14] = f[9]; f[17] = f[12];
+    } else if (y == 0) {
+        f[2] = f[4]; f[11] = f[13]; f[16] = f[18]; f[8] = f[5];
+    } else if (y == ny - 1) {
+        f[4] = f[2]; f[13] = f[11]; f[18] = f[16]; f[5] = f[8];
+    }
+    f[1] = f[3]; f[5] = f[6]; f[8] = f[7]; f[10] = f[12]; f[15] = f[17];
+}
+
+// x = nx-1 symmetry plane, xsymmetry_top (MRTtest/DynInlet3.cu:1077-1137).
+template <class R>
+__device__ __forceinline__ void xsym_top(R f[Q], int y, int z, int ny, int nz)
+{
+    if (y == 0 && z == 0) {
+        f[2] = f[4]; f[13] = f[18]; f[11] = f[18]; f[16] = f[18]; f[5] = f[8]; f[9] = f[14]; f[10] = f[15];
+    } else if (y == 0 && z == nz - 1) {
+        f[2] = f[4]; f[11] = f[13]; f[18] = f[13]; f[16] = f[13]; f[5] = f[8]; f[14] = f[9]; f[15] = f[10];
+    } else if (y == ny - 1 && z == 0) {
+        f[4] = f[2]; f[18] = f[16]; f[11] = f[16]; f[13] = f[16]; f[8] = f[5]; f[9] = f[14]; f[10] = f[15];
+    } else if (y == ny - 1 && z == nz - 1) {
+        f[4] = f[2]; f[13] = f[11]; f[16] = f[11]; f[18] = f[11]; f[8] = f[5]; f[14] = f[9]; f[15] = f[10];
+    } else if (y == 0) {
+        f[2] = f[4]; f[11] = f[13]; f[16] = f[18]; f[5] = f[8];
+    } else if (y == ny - 1) {
+        f[4] = f[2]; f[13] = f[11]; f[18] = f[16]; f[8] = f[5];
+    }
+    f[3] = f[1]; f[6] = f[5]; f[7] = f[8]; f[12] = f[10]; f[17] = f[15];
+}
+
 } // namespace lbm

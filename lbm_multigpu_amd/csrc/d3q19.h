@@ -73,6 +73,8 @@ constexpr unsigned char NODE_SOLID_FORCE = 2;
 constexpr unsigned char NODE_INLET_WEST = 3;
 constexpr unsigned char NODE_OUTLET_EAST = 4;
 constexpr unsigned char NODE_LID_WEST_REG = 5;
+constexpr unsigned char NODE_SYM_XMIN = 6;
+constexpr unsigned char NODE_SYM_XMAX = 7;
 constexpr unsigned char NODE_NEAR = 0x80;
 
 __host__ __device__ inline bool is_solid(unsigned char code)

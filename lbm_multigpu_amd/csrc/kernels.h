@@ -226,6 +226,10 @@ __device__ __forceinline__ int node_pull(const StepParams& p, int x, int y, int 
         else east_extrap(f, R(1));
     } else if (base == NODE_LID_WEST_REG) {
         lid_west_reg(f, R(p.u_lid));
+    } else if (base == NODE_SYM_XMIN) {
+        xsym_bot(f, y, z, p.ny, p.nz);
+    } else if (base == NODE_SYM_XMAX) {
+        xsym_top(f, y, z, p.ny, p.nz);
     }
     return ACT_COLLIDE;
 }

@@ -412,7 +412,7 @@ extern "C" int lbm_set_geometry(lbm_ctx* ctx, const uint8_t* mask)
     const lbm_config& c = ctx->cfg;
     if (mask) {
         for (size_t i = 0; i < L.cells; ++i)
-            if (mask[i] > LBM_NODE_LID_WEST_REG)
+            if (mask[i] > LBM_NODE_SYM_XMAX)
                 return fail(ctx, LBM_EINVAL, "lbm_set_geometry: unknown node code " + std::to_string(mask[i]));
         std::memcpy(L.host_mask.data(), mask, L.cells);
     } else {
@@ -944,6 +944,35 @@ extern "C" int lbm_get_time(lbm_ctx* ctx, long long* t)
 
 // Tecplot POINT zone as CPU_3D writes output.dat (CPU_3D/CPU_3D.cpp:1590-1614)
 // and WriteResults its first seven columns (MRTtest/DynInlet3.cu:2604-2620).
+// The "Smag" column of WriteResults (MRTtest/DynInlet3.cu:2637-2660): the
+// second-order non-equilibrium moments of a node (m9, m11, m13-15 less their
+// equilibrium parts, with the momentum sums as velocity) turned into the
+// stress components PI and |PI| = sqrt(2 PI:PI).
+static float smag_column(const float f[19])
+{
+    float m[19];
+    for (int k = 0; k < 19; ++k) {
+        float a = 0.f;
+        for (int i = 0; i < 19; ++i) a += (float)mrow(k, i) * f[i];
+        m[k] = a;
+    }
+    const float u = m[3], v = m[5], w = m[7];
+    m[9] -= 2.f * u * u - (v * v + w * w);
+    m[11] -= v * v - w * w;
+    m[13] -= u * v;
+    m[14] -= v * w;
+    m[15] -= u * w;
+    const float p11 = -0.5f * m[9];
+    const float p22 = -(-38.f * m[9] - 3.f * m[11]) / 76.f;
+    const float p33 = -(-38.f * m[9] + 3.f * m[11]) / 76.f;
+    const float p12 = -1.5f * m[13], p23 = -1.5f * m[14], p13 = -1.5f * m[15];
+    return std::sqrt(2.f * (p11 * p11 + p22 * p22 + p33 * p33 + 2.f * p12 * p12 + 2.f * p23 * p23 + 2.f * p13 * p13));
+}
+
+// Tecplot POINT zone per level (WriteResults, MRTtest/DynInlet3.cu:2601-2694,
+// and WriteResultsLR for the fine patch in coarse coordinates).  Without
+// statistics the columns are X,Y,Z,u,v,w,rho; with them the reference's full
+// row: ... velAv[0..2], ufluc, vfluc, wfluc, Smag.
 extern "C" int lbm_write_tecplot(lbm_ctx* ctx, const char* casename)
 {
     if (!ctx || !casename) return fail(ctx, LBM_EINVAL, "lbm_write_tecplot: null argument");
@@ -952,10 +981,22 @@ extern "C" int lbm_write_tecplot(lbm_ctx* ctx, const char* casename)
         std::vector<double> r(L.cells), u(L.cells), v(L.cells), w(L.cells);
         int rc = lbm_get_moments(ctx, level, r.data(), u.data(), v.data(), w.data());
         if (rc) return rc;
+        const bool stats = L.stat != nullptr;
+        std::vector<float> mean, dev, pops;
+        if (stats) {
+            mean.resize(3 * L.cells);
+            dev.resize(3 * L.cells);
+            pops.resize((size_t)Q * L.cells);
+            rc = lbm_stats_get(ctx, level, mean.data(), dev.data());
+            if (!rc) rc = lbm_get_populations(ctx, level, pops.data());
+            if (rc) return rc;
+        }
         std::string name = std::string(casename) + (level == 0 ? ".dat" : "_LR.dat");
         std::ofstream out(name);
         if (!out) return fail(ctx, LBM_EIO, "lbm_write_tecplot: cannot open " + name);
-        out << "VARIABLES = \"X\",\"Y\",\"Z\",\"u\",\"v\",\"w\",\"rho\"\n";
+        out << "VARIABLES = \"X\",\"Y\",\"Z\",\"u\",\"v\",\"w\",\"rho\"";
+        if (stats) out << ",\"velAv[0]\",\"velAv[1]\",\"velAv[2]\",\"ufluc\",\"vfluc\",\"wfluc\",\"Smag\"";
+        out << "\n";
         out << "ZONE F=POINT, I=" << L.nx << ", J=" << L.ny << ", K=" << L.nz << "\n";
         for (int z = 0; z < L.nz; ++z)
             for (int y = 0; y < L.ny; ++y)
@@ -967,7 +1008,15 @@ extern "C" int lbm_write_tecplot(lbm_ctx* ctx, const char* casename)
                         out << ctx->cfg.lr_x0 + 0.5f * x << ", " << ctx->cfg.lr_y0 + 0.5f * y << ", "
                             << ctx->cfg.lr_z0 + 0.5f * z << ", ";
                     }
-                    out << (float)u[c] << "," << (float)v[c] << "," << (float)w[c] << "," << (float)r[c] << "\n";
+                    out << (float)u[c] << "," << (float)v[c] << "," << (float)w[c] << "," << (float)r[c];
+                    if (stats) {
+                        float f[19];
+                        for (int i = 0; i < Q; ++i) f[i] = pops[(size_t)i * L.cells + c];
+                        for (int a = 0; a < 3; ++a) out << "," << mean[a * L.cells + c];
+                        for (int a = 0; a < 3; ++a) out << "," << dev[a * L.cells + c];
+                        out << "," << smag_column(f);
+                    }
+                    out << "\n";
                 }
         if (!out) return fail(ctx, LBM_EIO, "lbm_write_tecplot: write failed for " + name);
     }

@@ -44,7 +44,9 @@ enum {
     ORC_SOLID_FORCE = 2,    /* im 10: bounce-back wall, momentum exchange summed */
     ORC_INLET_WEST = 3,     /* im 400: West_Extrap velocity inlet */
     ORC_OUTLET_EAST = 4,    /* im 300: East_Extrap rho=1 outlet */
-    ORC_LID_WEST_REG = 5    /* CPU_3D im 53: DirichletWest_Regularized */
+    ORC_LID_WEST_REG = 5,   /* CPU_3D im 53: DirichletWest_Regularized */
+    ORC_SYM_XMIN = 6,       /* im 26: xsymmetry_bot */
+    ORC_SYM_XMAX = 7        /* im 25: xsymmetry_top */
 };
 
 typedef struct {
@@ -69,6 +71,8 @@ void orc_moment_matrix(double M[19][19]);
 /* ---- float restatement ---- */
 void orc_moments_f32(const float f[19], float m[19]);
 void orc_collide_f32(const orc_params* p, float f[19]);
+void orc_xsym_bot_f32(float f[19], int y, int z, int ny, int nz);
+void orc_xsym_top_f32(float f[19], int y, int z, int ny, int nz);
 void orc_les_omega_f32(const orc_params* p, const float f[19], float* omega_out, float* q_out);
 void orc_west_extrap_f32(float f[19], float u);
 void orc_east_extrap_f32(float f[19], float rho);
@@ -86,6 +90,8 @@ void orc_stats_update_f32(const orc_params* p, const uint8_t* mask, const float*
 /* ---- double restatement ---- */
 void orc_moments_f64(const double f[19], double m[19]);
 void orc_collide_f64(const orc_params* p, double f[19]);
+void orc_xsym_bot_f64(double f[19], int y, int z, int ny, int nz);
+void orc_xsym_top_f64(double f[19], int y, int z, int ny, int nz);
 void orc_les_omega_f64(const orc_params* p, const double f[19], double* omega_out, double* q_out);
 void orc_west_extrap_f64(double f[19], double u);
 void orc_east_extrap_f64(double f[19], double rho);

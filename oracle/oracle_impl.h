@@ -255,6 +255,45 @@ void FN(orc_lid_west_reg)(R f[19], R u_lid)
     }
 }
 
+/* Symmetry planes x = 0 (xsymmetry_bot, MRTtest/DynInlet3.cu:1020-1075) and
+ * x = nx-1 (xsymmetry_top, :1077-1137): copy statements in the reference's
+ * order, edges and corners of the plane as coded. */
+void FN(orc_xsym_bot)(R f[19], int y, int z, int ny, int nz)
+{
+    if (y == 0 && z == 0) {
+        f[2] = f[4]; f[13] = f[18]; f[11] = f[18]; f[16] = f[18]; f[6] = f[7]; f[9] = f[14]; f[12] = f[17];
+    } else if (y == 0 && z == nz - 1) {
+        f[4] = f[2]; f[11] = f[13]; f[18] = f[13]; f[16] = f[13]; f[6] = f[7]; f[14] = f[9]; f[17] = f[12];
+    } else if (y == ny - 1 && z == 0) {
+        f[4] = f[2]; f[11] = f[16]; f[18] = f[16]; f[13] = f[16]; f[7] = f[6]; f[9] = f[14]; f[12] = f[17];
+    } else if (y == ny - 1 && z == nz - 1) {
+        f[4] = f[2]; f[16] = f[11]; f[18] = f[11]; f[13] = f[11]; f[7] = f[6]; f[14] = f[9]; f[17] = f[12];
+    } else if (y == 0) {
+        f[2] = f[4]; f[11] = f[13]; f[16] = f[18]; f[8] = f[5];
+    } else if (y == ny - 1) {
+        f[4] = f[2]; f[13] = f[11]; f[18] = f[16]; f[5] = f[8];
+    }
+    f[1] = f[3]; f[5] = f[6]; f[8] = f[7]; f[10] = f[12]; f[15] = f[17];
+}
+
+void FN(orc_xsym_top)(R f[19], int y, int z, int ny, int nz)
+{
+    if (y == 0 && z == 0) {
+        f[2] = f[4]; f[13] = f[18]; f[11] = f[18]; f[16] = f[18]; f[5] = f[8]; f[9] = f[14]; f[10] = f[15];
+    } else if (y == 0 && z == nz - 1) {
+        f[2] = f[4]; f[11] = f[13]; f[18] = f[13]; f[16] = f[13]; f[5] = f[8]; f[14] = f[9]; f[15] = f[10];
+    } else if (y == ny - 1 && z == 0) {
+        f[4] = f[2]; f[18] = f[16]; f[11] = f[16]; f[13] = f[16]; f[8] = f[5]; f[9] = f[14]; f[10] = f[15];
+    } else if (y == ny - 1 && z == nz - 1) {
+        f[4] = f[2]; f[13] = f[11]; f[16] = f[11]; f[18] = f[11]; f[8] = f[5]; f[14] = f[9]; f[15] = f[10];
+    } else if (y == 0) {
+        f[2] = f[4]; f[11] = f[13]; f[16] = f[18]; f[5] = f[8];
+    } else if (y == ny - 1) {
+        f[4] = f[2]; f[13] = f[11]; f[18] = f[16]; f[8] = f[5];
+    }
+    f[3] = f[1]; f[6] = f[5]; f[7] = f[8]; f[12] = f[10]; f[17] = f[15];
+}
+
 static inline int FN(is_solid)(uint8_t code)
 {
     return code == ORC_SOLID || code == ORC_SOLID_FORCE;
@@ -341,6 +380,10 @@ void FN(orc_step)(const orc_params* p, const uint8_t* mask, const R* u_in,
                     FN(orc_east_extrap)(f, (R)1);
                 } else if (code == ORC_LID_WEST_REG) {
                     FN(orc_lid_west_reg)(f, (R)p->u_lid);
+                } else if (code == ORC_SYM_XMIN) {
+                    FN(orc_xsym_bot)(f, y, z, ny, nz);
+                } else if (code == ORC_SYM_XMAX) {
+                    FN(orc_xsym_top)(f, y, z, ny, nz);
                 }
                 FN(orc_collide)(p, f);
                 for (int i = 0; i < 19; ++i) fout[i * N + cell] = f[i];

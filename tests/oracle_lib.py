@@ -15,7 +15,7 @@ REF_EXE = os.path.join(ROOT, "oracle", "_ref", "cpu3d_ref")
 
 BGK, MRT_CPU3D, MRT_LES, MRT_DHUM = 0, 1, 2, 3
 PERIODIC, CLAMP, WALL = 0, 1, 2
-FLUID, SOLID, SOLID_FORCE, INLET_WEST, OUTLET_EAST, LID_WEST_REG = 0, 1, 2, 3, 4, 5
+FLUID, SOLID, SOLID_FORCE, INLET_WEST, OUTLET_EAST, LID_WEST_REG, SYM_XMIN, SYM_XMAX = 0, 1, 2, 3, 4, 5, 6, 7
 
 
 class OrcParams(C.Structure):
@@ -47,6 +47,8 @@ def lib():
             getattr(_lib, "orc_west_extrap_" + suf).argtypes = [fp, C.c_float if suf == "f32" else C.c_double]
             getattr(_lib, "orc_east_extrap_" + suf).argtypes = [fp, C.c_float if suf == "f32" else C.c_double]
             getattr(_lib, "orc_lid_west_reg_" + suf).argtypes = [fp, C.c_float if suf == "f32" else C.c_double]
+            getattr(_lib, "orc_xsym_bot_" + suf).argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int]
+            getattr(_lib, "orc_xsym_top_" + suf).argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int]
             sc = C.c_float if suf == "f32" else C.c_double
             getattr(_lib, "orc_feq_bgk_" + suf).argtypes = [sc, sc, sc, sc, fp]
             getattr(_lib, "orc_feq_mrt_" + suf).argtypes = [sc, sc, sc, sc, fp]
@@ -241,3 +243,12 @@ def stats_update(p, f, stat, mode, n, w=1.0, mask=None):
     f = np.ascontiguousarray(f)
     suf = "f32" if f.dtype == np.float32 else "f64"
     getattr(lib(), "orc_stats_update_" + suf)(C.byref(p), _ptr(mask), f, stat.reshape(-1), mode, n, w)
+
+
+def xsym(f, side, y, z, ny, nz):
+    """Symmetry-plane condition on one node's populations (side 'bot': x = 0,
+    'top': x = nx-1)."""
+    f = np.ascontiguousarray(f).copy()
+    suf = "f32" if f.dtype == np.float32 else "f64"
+    getattr(lib(), f"orc_xsym_{side}_" + suf)(f, y, z, ny, nz)
+    return f

@@ -23,7 +23,7 @@ MODEL_TO_ORC = {L.MODEL_BGK: O.BGK, L.MODEL_MRT_CPU3D: O.MRT_CPU3D, L.MODEL_MRT_
                 L.MODEL_MRT_DHUMIERES: O.MRT_DHUM}
 NODE_TO_ORC = {L.NODE_FLUID: O.FLUID, L.NODE_SOLID: O.SOLID, L.NODE_SOLID_FORCE: O.SOLID_FORCE,
                L.NODE_INLET_WEST: O.INLET_WEST, L.NODE_OUTLET_EAST: O.OUTLET_EAST,
-               L.NODE_LID_WEST_REG: O.LID_WEST_REG}
+               L.NODE_LID_WEST_REG: O.LID_WEST_REG, L.NODE_SYM_XMIN: O.SYM_XMIN, L.NODE_SYM_XMAX: O.SYM_XMAX}
 
 
 def say(*a):
@@ -271,3 +271,29 @@ def test_bad_mask_code_rejected():
         assert e.value.code == -22
         with pytest.raises(L.LBMError):
             s.step(1)  # not initialised
+
+
+def test_symmetry_planes_match_oracle():
+    """x = 0 and x = nx-1 symmetry planes (xsymmetry_bot/top, im 26/25) with
+    halfway walls in y and z, so the planes' edges and corners take their
+    special cases: 30 steps of MRT-LES from a random state, GPU vs oracle."""
+    say("symmetry planes")
+    c = L.default_config()
+    c.nx, c.ny, c.nz = 40, 12, 10
+    c.bc[0], c.bc[1], c.bc[2] = L.BC_CLAMP, L.BC_WALL, L.BC_WALL
+    c.halfway = 1
+    c.model, c.omega, c.cs = L.MODEL_MRT_LES, 1.6, 0.01
+    mask = np.zeros((c.nz, c.ny, c.nx), np.uint8)
+    mask[:, :, 0] = L.NODE_SYM_XMIN
+    mask[:, :, -1] = L.NODE_SYM_XMAX
+    rng = np.random.default_rng(11)
+    f = random_state(rng, mask.shape)
+    with L.Solver(c) as s:
+        s.set_geometry(mask)
+        assert np.array_equal(s.get_mask(), mask)
+        s.set_populations(f)
+        s.step(30)
+        g = s.populations()
+    omask = np.vectorize(NODE_TO_ORC.get)(mask).astype(np.uint8)
+    ref, _ = O.run(orc_params(c), f, 30, mask=omask)
+    assert np.abs(g - ref).max() < 1e-5 * np.abs(ref).max()

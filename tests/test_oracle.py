@@ -317,3 +317,25 @@ def test_stats_update_closed_forms():
         for _ in range(2):
             O.stats_update(p, pops(c), st2, 1, float(n), 0.5)
     assert np.array_equal(st2[:3, 0, 0, 0], c)
+
+
+def test_symmetry_plane_is_a_mirror_inside_the_plane():
+    """Away from the plane's edges, xsymmetry_bot/top make the populations
+    entering through the plane the mirror images (c_x -> -c_x) of the ones
+    leaving it, leave every other population alone, and so zero the normal
+    momentum.  (The edge and corner statements of the reference are kept as
+    coded; no reference output pins them.)"""
+    rng = np.random.default_rng(3)
+    C = np.array([(0, 0, 0), (1, 0, 0), (0, 1, 0), (-1, 0, 0), (0, -1, 0), (1, 1, 0), (-1, 1, 0), (-1, -1, 0),
+                  (1, -1, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1), (-1, 0, 1), (0, -1, 1), (0, 0, -1), (1, 0, -1),
+                  (0, 1, -1), (-1, 0, -1), (0, -1, -1)])
+    mirror = [int(np.flatnonzero((C == C[i] * [-1, 1, 1]).all(1))[0]) for i in range(19)]
+    for side, into in (("bot", 1), ("top", -1)):
+        f = rng.uniform(0.01, 0.1, 19)
+        g = O.xsym(f, side, 5, 4, 12, 10)
+        for i in range(19):
+            if C[i][0] == into:
+                assert g[i] == g[mirror[i]] == f[mirror[i]]
+            else:
+                assert g[i] == f[i]
+        assert abs((C[:, 0] * g).sum()) < 1e-15
