@@ -339,23 +339,26 @@ __device__ __forceinline__ void lid_west_reg(R f[Q], R u_lid)
 // x = 0 symmetry plane, xsymmetry_bot (MRTtest/DynInlet3.cu:1020-1075):
 // the populations entering through the plane copy their mirror images, with
 // the reference's own treatment of the y/z edges and corners.  The copies
-// are sequential (later ones read earlier results), as coded.
+// are sequential (later ones read earlier results), as coded.  Each case is
+// applied under its own predicate (one at most holds), so every index stays
+// a compile-time constant and the populations stay in registers.
+#define LBM_SYM_COPY(cond, d, s_) f[d] = (cond) ? f[s_] : f[d]
 template <class R>
 __device__ __forceinline__ void xsym_bot(R f[Q], int y, int z, int ny, int nz)
 {
-    if (y == 0 && z == 0) {
-        f[2] = f[4]; f[13] = f[18]; f[11] = f[18]; f[16] = f[18]; f[6] = f[7]; f[9] = f[14]; f[12] = f[17];
-    } else if (y == 0 && z == nz - 1) {
-        f[4] = f[2]; f[11] = f[13]; f[18] = f[13]; f[16] = f[13]; f[6] = f[7]; f[14] = f[9]; f[17] = f[12];
-    } else if (y == ny - 1 && z == 0) {
-        f[4] = f[2]; f[11] = f[16]; f[18] = f[16]; f[13] = f[16]; f[7] = f[6]; f[9] = f[14]; f[12] = f[17];
-    } else if (y == ny - 1 && z == nz - 1) {
-        f[4] = f[2]; f[16] = f[11]; f[18] = f[11]; f[13] = f[11]; f[7] = f[6]; f[14] = f[9]; f[17] = f[12];
-    } else if (y == 0) {
-        f[2] = f[4]; f[11] = f[13]; f[16] = f[18]; f[8] = f[5];
-    } else if (y == ny - 1) {
-        f[4] = f[2]; f[13] = f[11]; f[18] = f[16]; f[5] = f[8];
-    }
+    const bool y0 = y == 0, y1 = y == ny - 1, z0 = z == 0, z1 = z == nz - 1;
+    const bool a = y0 && z0, b = y0 && z1 && !z0, c = y1 && z0 && !y0, d = y1 && z1 && !y0 && !z0;
+    const bool e = y0 && !a && !b, h = y1 && !y0 && !c && !d;
+    LBM_SYM_COPY(a, 2, 4); LBM_SYM_COPY(a, 13, 18); LBM_SYM_COPY(a, 11, 18); LBM_SYM_COPY(a, 16, 18);
+    LBM_SYM_COPY(a, 6, 7); LBM_SYM_COPY(a, 9, 14); LBM_SYM_COPY(a, 12, 17);
+    LBM_SYM_COPY(b, 4, 2); LBM_SYM_COPY(b, 11, 13); LBM_SYM_COPY(b, 18, 13); LBM_SYM_COPY(b, 16, 13);
+    LBM_SYM_COPY(b, 6, 7); LBM_SYM_COPY(b, 14, 9); LBM_SYM_COPY(b, 17, 12);
+    LBM_SYM_COPY(c, 4, 2); LBM_SYM_COPY(c, 11, 16); LBM_SYM_COPY(c, 18, 16); LBM_SYM_COPY(c, 13, 16);
+    LBM_SYM_COPY(c, 7, 6); LBM_SYM_COPY(c, 9, 14); LBM_SYM_COPY(c, 12, 17);
+    LBM_SYM_COPY(d, 4, 2); LBM_SYM_COPY(d, 16, 11); LBM_SYM_COPY(d, 18, 11); LBM_SYM_COPY(d, 13, 11);
+    LBM_SYM_COPY(d, 7, 6); LBM_SYM_COPY(d, 14, 9); LBM_SYM_COPY(d, 17, 12);
+    LBM_SYM_COPY(e, 2, 4); LBM_SYM_COPY(e, 11, 13); LBM_SYM_COPY(e, 16, 18); LBM_SYM_COPY(e, 8, 5);
+    LBM_SYM_COPY(h, 4, 2); LBM_SYM_COPY(h, 13, 11); LBM_SYM_COPY(h, 18, 16); LBM_SYM_COPY(h, 5, 8);
     f[1] = f[3]; f[5] = f[6]; f[8] = f[7]; f[10] = f[12]; f[15] = f[17];
 }
 
@@ -363,20 +366,21 @@ __device__ __forceinline__ void xsym_bot(R f[Q], int y, int z, int ny, int nz)
 template <class R>
 __device__ __forceinline__ void xsym_top(R f[Q], int y, int z, int ny, int nz)
 {
-    if (y == 0 && z == 0) {
-        f[2] = f[4]; f[13] = f[18]; f[11] = f[18]; f[16] = f[18]; f[5] = f[8]; f[9] = f[14]; f[10] = f[15];
-    } else if (y == 0 && z == nz - 1) {
-        f[2] = f[4]; f[11] = f[13]; f[18] = f[13]; f[16] = f[13]; f[5] = f[8]; f[14] = f[9]; f[15] = f[10];
-    } else if (y == ny - 1 && z == 0) {
-        f[4] = f[2]; f[18] = f[16]; f[11] = f[16]; f[13] = f[16]; f[8] = f[5]; f[9] = f[14]; f[10] = f[15];
-    } else if (y == ny - 1 && z == nz - 1) {
-        f[4] = f[2]; f[13] = f[11]; f[16] = f[11]; f[18] = f[11]; f[8] = f[5]; f[14] = f[9]; f[15] = f[10];
-    } else if (y == 0) {
-        f[2] = f[4]; f[11] = f[13]; f[16] = f[18]; f[5] = f[8];
-    } else if (y == ny - 1) {
-        f[4] = f[2]; f[13] = f[11]; f[18] = f[16]; f[8] = f[5];
-    }
+    const bool y0 = y == 0, y1 = y == ny - 1, z0 = z == 0, z1 = z == nz - 1;
+    const bool a = y0 && z0, b = y0 && z1 && !z0, c = y1 && z0 && !y0, d = y1 && z1 && !y0 && !z0;
+    const bool e = y0 && !a && !b, h = y1 && !y0 && !c && !d;
+    LBM_SYM_COPY(a, 2, 4); LBM_SYM_COPY(a, 13, 18); LBM_SYM_COPY(a, 11, 18); LBM_SYM_COPY(a, 16, 18);
+    LBM_SYM_COPY(a, 5, 8); LBM_SYM_COPY(a, 9, 14); LBM_SYM_COPY(a, 10, 15);
+    LBM_SYM_COPY(b, 2, 4); LBM_SYM_COPY(b, 11, 13); LBM_SYM_COPY(b, 18, 13); LBM_SYM_COPY(b, 16, 13);
+    LBM_SYM_COPY(b, 5, 8); LBM_SYM_COPY(b, 14, 9); LBM_SYM_COPY(b, 15, 10);
+    LBM_SYM_COPY(c, 4, 2); LBM_SYM_COPY(c, 18, 16); LBM_SYM_COPY(c, 11, 16); LBM_SYM_COPY(c, 13, 16);
+    LBM_SYM_COPY(c, 8, 5); LBM_SYM_COPY(c, 9, 14); LBM_SYM_COPY(c, 10, 15);
+    LBM_SYM_COPY(d, 4, 2); LBM_SYM_COPY(d, 13, 11); LBM_SYM_COPY(d, 16, 11); LBM_SYM_COPY(d, 18, 11);
+    LBM_SYM_COPY(d, 8, 5); LBM_SYM_COPY(d, 14, 9); LBM_SYM_COPY(d, 15, 10);
+    LBM_SYM_COPY(e, 2, 4); LBM_SYM_COPY(e, 11, 13); LBM_SYM_COPY(e, 16, 18); LBM_SYM_COPY(e, 5, 8);
+    LBM_SYM_COPY(h, 4, 2); LBM_SYM_COPY(h, 13, 11); LBM_SYM_COPY(h, 18, 16); LBM_SYM_COPY(h, 8, 5);
     f[3] = f[1]; f[6] = f[5]; f[7] = f[8]; f[12] = f[10]; f[17] = f[15];
 }
+#undef LBM_SYM_COPY
 
 } // namespace lbm

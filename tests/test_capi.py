@@ -123,3 +123,40 @@ def test_inputs_parses_reference_separators():
     assert (c.nx, c.ny, c.nz) == (10, 20, 30) and c.umax == 0.1 and c.omega == 1.5
     c = cfgmod.read_inputs("Base domain size \t10x20x30\nUMAX:")
     assert c.nx == 10
+
+
+def _kernel_private_segments():
+    """(kernel name, private segment bytes per lane) of every kernel in the
+    built library's gfx950 code object (offload bundle -> AMDGPU metadata)."""
+    import re
+    import struct
+    import subprocess
+    data = open(L.LIB_PATH, "rb").read()
+    i = data.find(b"__CLANG_OFFLOAD_BUNDLE__")
+    assert i >= 0, "no offload bundle in liblbm.so"
+    n = struct.unpack_from("<Q", data, i + 24)[0]
+    p, code = i + 32, None
+    for _ in range(n):
+        off, size, tl = struct.unpack_from("<QQQ", data, p)
+        triple = data[p + 24:p + 24 + tl].decode()
+        p += 24 + tl
+        if triple.endswith("gfx950"):
+            code = data[i + off:i + off + size]
+    assert code, "no gfx950 code object"
+    import tempfile
+    with tempfile.NamedTemporaryFile(suffix=".co") as fh:
+        fh.write(code)
+        fh.flush()
+        out = subprocess.run(["/opt/rocm/llvm/bin/llvm-readelf", "--notes", fh.name], capture_output=True,
+                             text=True, check=True).stdout
+    names = re.findall(r"\.name:\s+(\S+)", out)
+    priv = [int(v) for v in re.findall(r"\.private_segment_fixed_size:\s+(\d+)", out)]
+    assert len(names) == len(priv) > 50
+    return list(zip(names, priv))
+
+
+def test_kernels_keep_their_state_in_registers():
+    """No kernel of the library spills to scratch (a spilled population array
+    cost the stream-collide a quarter of its bandwidth once)."""
+    spills = [(n, b) for n, b in _kernel_private_segments() if b]
+    assert not spills, spills
