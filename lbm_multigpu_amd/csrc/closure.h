@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float* X, long long n
 // 0..2 u, 3..8 uu uv uw vv vw ww, 9 solid fraction.
 template <class S>
 __global__ void k_box_filter(const S* f, long long qs, int nxp, int nx, int ny, int nz, const unsigned char* mask,
-                             float* out)
+                             float* out, PopShift sh)
 {
     using R = typename Arith<S>::R;
     const int cx = nx / 2, cy = ny / 2, cz = nz / 2;
@@ -233,7 +233,7 @@ __global__ void k_box_filter(const S* f, long long qs, int nxp, int nx, int ny, 
         R rho = 0, u = 0, v = 0, w = 0;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            const R val = Store<S>::to_real(f[i * qs + cell], i);
+            const R val = Store<S>::to_real(f[i * qs + cell], i, sh);
             rho += val;
             if (CX[i]) u += R(CX[i]) * val;
             if (CY[i]) v += R(CY[i]) * val;

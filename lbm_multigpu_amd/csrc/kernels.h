@@ -84,6 +84,7 @@ struct StepParams {
     // per-node collision rate left by the last learned-closure step (padded
     // layout): > 0 replaces the LES rate, 0 means the Smagorinsky rate
     const float* omega_node;
+    PopShift sh; // 16-bit storage offsets
 };
 
 // XCD-aware renumbering: the hardware id hw goes to XCD hw % 8; give XCD j
@@ -169,7 +170,7 @@ __device__ __forceinline__ int node_pull(const StepParams& p, int x, int y, int 
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const int src = row[1 - CY[i]][1 - CZ[i]] + X[1 - CX[i]];
-        f[i] = Store<S>::load(at(fin + i * qs, src), i);
+        f[i] = Store<S>::load(at(fin + i * qs, src), i, p.sh);
     }
 
     if (code & NODE_NEAR) {
@@ -184,7 +185,7 @@ __device__ __forceinline__ int node_pull(const StepParams& p, int x, int y, int 
             if (!out && p.halfway) sc = *at(p.mask, row[1 - CY[i]][1 - CZ[i]] + X[1 - CX[i]]) & 0x7f;
             const bool bounce = out || is_solid(sc);
             if (bounce) {
-                const R v = Store<S>::load(at(fin + OPP[i] * qs, cell), OPP[i]);
+                const R v = Store<S>::load(at(fin + OPP[i] * qs, cell), OPP[i], p.sh);
                 f[i] = v;
                 if (p.fpart && sc == NODE_SOLID_FORCE) {
                     fx += R(2 * CX[OPP[i]]) * v;
@@ -221,7 +222,7 @@ __device__ __forceinline__ int node_pull(const StepParams& p, int x, int y, int 
         const int nb = base == NODE_INLET_WEST ? (x + 1 < p.nx ? cell + 1 : cell)
                                                : (x > 0 ? cell - 1 : cell);
 #pragma unroll
-        for (int i = 0; i < Q; ++i) f[i] = Store<S>::load(at(fin + i * qs, nb), i);
+        for (int i = 0; i < Q; ++i) f[i] = Store<S>::load(at(fin + i * qs, nb), i, p.sh);
         if (base == NODE_INLET_WEST) west_extrap(f, R(p.u_in[z * p.ny + y]));
         else east_extrap(f, R(1));
     } else if (base == NODE_LID_WEST_REG) {
@@ -250,7 +251,7 @@ __device__ __forceinline__ void node_store(const StepParams& p, int cell, const 
 {
     S* __restrict__ fout = static_cast<S*>(p.fout);
 #pragma unroll
-    for (int i = 0; i < Q; ++i) Store<S>::store(at(fout + i * p.qs, cell), f[i], i);
+    for (int i = 0; i < Q; ++i) Store<S>::store(at(fout + i * p.qs, cell), f[i], i, p.sh);
 }
 
 // Running statistics of the post-collision velocity sum c_i f_i at one node:
@@ -367,7 +368,7 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const double* part, lon
 template <class S>
 __global__ void __launch_bounds__(Shape<S>::THREADS) k_mass_momentum(const S* f, const unsigned char* mask,
                                                         long long qs, int nx, int ny, int nxp,
-                                                        int nsegw, int nz, double* part)
+                                                        int nsegw, int nz, double* part, PopShift sh)
 {
     using R = typename Arith<S>::R;
     const int lb = blockIdx.x;
@@ -381,7 +382,7 @@ __global__ void __launch_bounds__(Shape<S>::THREADS) k_mass_momentum(const S* f,
             R r = 0, a = 0, b = 0, c = 0;
 #pragma unroll
             for (int i = 0; i < Q; ++i) {
-                const R v = Store<S>::to_real(f[i * qs + cell], i);
+                const R v = Store<S>::to_real(f[i * qs + cell], i, sh);
                 r += v;
                 if (CX[i]) a += R(CX[i]) * v;
                 if (CY[i]) b += R(CY[i]) * v;
@@ -407,7 +408,7 @@ __global__ void __launch_bounds__(Shape<S>::THREADS) k_mass_momentum(const S* f,
 // Node fields as WriteResults computes them: rho = sum f, (u,v,w) = momentum.
 template <class S>
 __global__ void k_macro(const S* f, long long qs, int nx, int ny, int nz, int nxp, double* rho,
-                        double* u, double* v, double* w)
+                        double* u, double* v, double* w, PopShift sh)
 {
     using R = typename Arith<S>::R;
     const long long n = (long long)nx * ny * nz;
@@ -421,7 +422,7 @@ __global__ void k_macro(const S* f, long long qs, int nx, int ny, int nz, int nx
         R s = 0, a = 0, b = 0, d = 0;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            const R val = Store<S>::to_real(f[i * qs + cell], i);
+            const R val = Store<S>::to_real(f[i * qs + cell], i, sh);
             s += val;
             if (CX[i]) a += R(CX[i]) * val;
             if (CY[i]) b += R(CY[i]) * val;
@@ -433,7 +434,7 @@ __global__ void k_macro(const S* f, long long qs, int nx, int ny, int nz, int nx
 
 // Dense <-> padded population copies with conversion.
 template <class S, class T>
-__global__ void k_pack(const S* f, long long qs, int nx, int ny, int nz, int nxp, T* out)
+__global__ void k_pack(const S* f, long long qs, int nx, int ny, int nz, int nxp, T* out, PopShift sh)
 {
     const long long n = (long long)nx * ny * nz;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n * Q;
@@ -445,12 +446,12 @@ __global__ void k_pack(const S* f, long long qs, int nx, int ny, int nz, int nxp
         const int y = int(r % ny);
         const int z = int(r / ny);
         const long long cell = x + (long long)nxp * (y + (long long)ny * z);
-        out[t] = T(Store<S>::to_real(f[i * qs + cell], i));
+        out[t] = T(Store<S>::to_real(f[i * qs + cell], i, sh));
     }
 }
 
 template <class S, class T>
-__global__ void k_unpack(const T* in, long long qs, int nx, int ny, int nz, int nxp, S* f, S* f2)
+__global__ void k_unpack(const T* in, long long qs, int nx, int ny, int nz, int nxp, S* f, S* f2, PopShift sh)
 {
     using R = typename Arith<S>::R;
     const long long n = (long long)nx * ny * nz;
@@ -463,7 +464,7 @@ __global__ void k_unpack(const T* in, long long qs, int nx, int ny, int nz, int 
         const int y = int(r % ny);
         const int z = int(r / ny);
         const long long cell = x + (long long)nxp * (y + (long long)ny * z);
-        const S v = Store<S>::from_real(R(in[t]), i);
+        const S v = Store<S>::from_real(R(in[t]), i, sh);
         f[i * qs + cell] = v;
         if (f2) f2[i * qs + cell] = v;
     }
@@ -478,7 +479,7 @@ __global__ void k_unpack(const T* in, long long qs, int nx, int ny, int nz, int 
 template <class S>
 __global__ void k_init(S* fa, S* fb, long long qs, int nx, int ny, int nz, int nxp,
                        const unsigned char* mask, const float* rho_in, const float* u_in,
-                       const float* v_in, const float* w_in, float u0, int kind)
+                       const float* v_in, const float* w_in, float u0, int kind, PopShift sh)
 {
     using R = typename Arith<S>::R;
     const long long n = (long long)nx * ny * nz;
@@ -531,7 +532,7 @@ __global__ void k_init(S* fa, S* fb, long long qs, int nx, int ny, int nz, int n
         }
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            const S sv = Store<S>::from_real(feq[i], i);
+            const S sv = Store<S>::from_real(feq[i], i, sh);
             fa[i * qs + cell] = sv;
             fb[i * qs + cell] = sv;
         }

@@ -36,6 +36,7 @@ struct Level {
     unsigned char* mask = nullptr; // device, padded
     std::vector<unsigned char> host_mask; // dense codes
     float* stat = nullptr;         // running statistics, 6 planes of qs
+    PopShift shift{};              // 16-bit storage offsets (equilibrium of the inflow)
     int blocks = 0;
     size_t bytes = 0;
 };
@@ -229,6 +230,17 @@ static int alloc_level(lbm_ctx* ctx, Level& L, int nx, int ny, int nz, int stora
     L.cells = (size_t)nx * ny * nz;
     L.storage = storage;
     L.esize = elem_size(storage);
+    // 16-bit formats store f_i - f_i^eq(1, (UMAX, 0, 0)): in the bulk of an
+    // inflow-driven run the stored deviations are small, so the 11 (FP16) or
+    // 8 (BF16) significant bits go to the flow's variation, not its mean
+    {
+        const float U = ctx->cfg.umax;
+        for (int i = 0; i < Q; ++i) {
+            const float w = float(WNUM[i]) * (1.0f / 36.0f);
+            const float cu = float(CX[i]) * U;
+            L.shift.s[i] = w * (1.f + 3.f * cu + 4.5f * cu * cu - 1.5f * U * U);
+        }
+    }
     const int nw = nwave_of_storage(L.storage);
     L.blocks = (int)(((long long)L.nsegw * ny * nz + nw - 1) / nw);
     const size_t lattice = (size_t)Q * (size_t)L.qs * L.esize;
@@ -480,7 +492,7 @@ static int init_level(lbm_ctx* ctx, int level, int kind, const float* rho, const
     int rc = with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         hipLaunchKernelGGL(k_init<S>, dim3(grid_for(L.cells)), dim3(256), 0, ctx->stream, (S*)L.fa, (S*)L.fb,
-                           L.qs, L.nx, L.ny, L.nz, L.nxp, L.mask, d[0], d[1], d[2], d[3], u0, kind);
+                           L.qs, L.nx, L.ny, L.nz, L.nxp, L.mask, d[0], d[1], d[2], d[3], u0, kind, L.shift);
         return LBM_OK;
     });
     HIP_TRY(ctx, hipGetLastError());
@@ -549,6 +561,7 @@ static StepParams step_params(lbm_ctx* ctx, const Level& L, float omega)
     p.u_lid = c.umax;
     p.s1 = c.s_dhum[1]; p.s2 = c.s_dhum[2]; p.s4 = c.s_dhum[4]; p.s10 = c.s_dhum[10]; p.s16 = c.s_dhum[16];
     p.omega_node = nullptr;
+    p.sh = L.shift;
     p.stat = L.stat;
     p.stat_mode = 0;
     p.stat_n = 0.f;
@@ -701,7 +714,7 @@ static int refine_extract(lbm_ctx* ctx, float* M)
     with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         hipLaunchKernelGGL(k_extract<S>, dim3((unsigned)((NC + 255) / 256)), dim3(256), 0, ctx->stream,
-                           (const S*)L.fa, L.qs, L.nxp, L.ny, ctx->rg, M);
+                           (const S*)L.fa, L.qs, L.nxp, L.ny, ctx->rg, M, L.shift);
         return 0;
     });
     HIP_TRY(ctx, hipGetLastError());
@@ -718,7 +731,7 @@ static int refine_cf(lbm_ctx* ctx, float alpha)
                            ctx->Mnew, alpha, NC, (f32x4v*)ctx->Pcf);
         const long long n = frame_nodes(ctx->rg.nf);
         hipLaunchKernelGGL(k_interp_cf<S>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (S*)F.fa,
-                           F.qs, F.nxp, F.mask, (const f32x4v*)ctx->Pcf, ctx->rg);
+                           F.qs, F.nxp, F.mask, (const f32x4v*)ctx->Pcf, ctx->rg, F.shift);
         return 0;
     });
     HIP_TRY(ctx, hipGetLastError());
@@ -733,7 +746,7 @@ static int refine_fc(lbm_ctx* ctx)
     with_storage(F.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         hipLaunchKernelGGL(k_interp_fc<S>, dim3(nb), dim3(256), 0, ctx->stream, (const S*)F.fa, F.qs, F.nxp,
-                           (S*)L.fa, L.qs, L.nxp, L.ny, L.mask, ctx->rg, ctx->fc_nb[0], ctx->fc_nb[1]);
+                           (S*)L.fa, L.qs, L.nxp, L.ny, L.mask, ctx->rg, ctx->fc_nb[0], ctx->fc_nb[1], F.shift);
         return 0;
     });
     HIP_TRY(ctx, hipGetLastError());
@@ -791,7 +804,7 @@ extern "C" int lbm_get_moments(lbm_ctx* ctx, int level, double* rho, double* u, 
     with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         hipLaunchKernelGGL(k_macro<S>, dim3(grid_for(L.cells)), dim3(256), 0, ctx->stream, (const S*)L.fa, L.qs,
-                           L.nx, L.ny, L.nz, L.nxp, d, d + L.cells, d + 2 * L.cells, d + 3 * L.cells);
+                           L.nx, L.ny, L.nz, L.nxp, d, d + L.cells, d + 2 * L.cells, d + 3 * L.cells, L.shift);
         return 0;
     });
     double* out[4] = {rho, u, v, w};
@@ -818,7 +831,7 @@ static int get_pops(lbm_ctx* ctx, int level, T* f, const char* who)
     with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         hipLaunchKernelGGL((k_pack<S, T>), dim3(grid_for(n)), dim3(256), 0, ctx->stream, (const S*)L.fa, L.qs,
-                           L.nx, L.ny, L.nz, L.nxp, d);
+                           L.nx, L.ny, L.nz, L.nxp, d, L.shift);
         return 0;
     });
     hipError_t e = hipGetLastError();
@@ -847,7 +860,7 @@ static int set_pops(lbm_ctx* ctx, int level, const T* f, const char* who)
         with_storage(L.storage, [&](auto* tag) {
             using S = std::remove_pointer_t<decltype(tag)>;
             hipLaunchKernelGGL((k_unpack<S, T>), dim3(grid_for(n)), dim3(256), 0, ctx->stream, d, L.qs, L.nx,
-                               L.ny, L.nz, L.nxp, (S*)L.fa, (S*)L.fb);
+                               L.ny, L.nz, L.nxp, (S*)L.fa, (S*)L.fb, L.shift);
             return 0;
         });
         e = hipGetLastError();
@@ -909,7 +922,7 @@ extern "C" int lbm_get_mass_momentum(lbm_ctx* ctx, int level, double out[4])
     with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         hipLaunchKernelGGL(k_mass_momentum<S>, dim3(L.blocks), dim3(Shape<S>::THREADS), 0, ctx->stream, (const S*)L.fa,
-                           L.mask, L.qs, L.nx, L.ny, L.nxp, L.nsegw, L.nz, part);
+                           L.mask, L.qs, L.nx, L.ny, L.nxp, L.nsegw, L.nz, part, L.shift);
         return 0;
     });
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, ctx->stream, part, (long long)L.blocks, 4,
@@ -1412,7 +1425,7 @@ extern "C" int lbm_closure_samples(lbm_ctx* ctx, uint64_t seed, float keep, floa
         with_storage(L.storage, [&](auto* tag) {
             using S = std::remove_pointer_t<decltype(tag)>;
             hipLaunchKernelGGL(k_box_filter<S>, dim3((unsigned)((NC + 255) / 256)), dim3(256), 0, ctx->stream,
-                               (const S*)L.fa, L.qs, L.nxp, L.nx, L.ny, L.nz, L.mask, bx);
+                               (const S*)L.fa, L.qs, L.nxp, L.nx, L.ny, L.nz, L.mask, bx, L.shift);
             return 0;
         });
         const uint32_t kp = (uint32_t)std::min(4294967295.0, std::max(0.0, (double)keep * 4294967296.0));

@@ -164,7 +164,8 @@ __device__ __forceinline__ void order2(const float (*Sc)[6], const float (*uc)[3
 
 // Footprint moments of the coarse lattice (one thread per footprint node).
 template <class S>
-__global__ void __launch_bounds__(256) k_extract(const S* fc, long long qs, int nxp, int ny, RefineGeom g, float* M)
+__global__ void __launch_bounds__(256) k_extract(const S* fc, long long qs, int nxp, int ny, RefineGeom g, float* M,
+                                                 PopShift sh)
 {
     const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
     const long long NC = (long long)ncx * ncy * ncz;
@@ -174,7 +175,7 @@ __global__ void __launch_bounds__(256) k_extract(const S* fc, long long qs, int 
     const int cell = (g.a[0] + i) + nxp * ((g.a[1] + j) + ny * (g.a[2] + k));
     float f[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f[q] = float(Store<S>::to_real(fc[q * qs + cell], q));
+    for (int q = 0; q < Q; ++q) f[q] = float(Store<S>::to_real(fc[q * qs + cell], q, sh));
     float mom[9];
     phys_moments(f, mom);
 #pragma unroll
@@ -253,7 +254,7 @@ __global__ void __launch_bounds__(256) k_cf_prep(const float* Mold, const float*
 // correction and the LES-aware rescale.
 template <class S>
 __global__ void __launch_bounds__(256) k_interp_cf(S* ff, long long fqs, int fnxp, const unsigned char* fmask,
-                                                   const f32x4v* P, RefineGeom g)
+                                                   const f32x4v* P, RefineGeom g, PopShift sh)
 {
     int i, j, k;
     if (!frame_node(blockIdx.x * 256LL + threadIdx.x, g, i, j, k)) return;
@@ -300,7 +301,7 @@ __global__ void __launch_bounds__(256) k_interp_cf(S* ff, long long fqs, int fnx
     float f[Q];
     invert_phys(f, mom, SF, g.eq_kind);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) ff[q * fqs + fcell] = Store<S>::from_real(typename Arith<S>::R(f[q]), q);
+    for (int q = 0; q < Q; ++q) ff[q * fqs + fcell] = Store<S>::from_real(typename Arith<S>::R(f[q]), q, sh);
 }
 
 // Fine -> coarse.  One workgroup per tile of 32x2x2 coarse nodes: its 256
@@ -312,7 +313,7 @@ constexpr int FC_TX = 32, FC_TY = 2, FC_TZ = 2;
 template <class S>
 __global__ void __launch_bounds__(256) k_interp_fc(const S* ff, long long fqs, int fnxp, S* fc, long long cqs,
                                                    int cnxp, int cny, const unsigned char* cmask, RefineGeom g,
-                                                   int nbx, int nby)
+                                                   int nbx, int nby, PopShift sh)
 {
     __shared__ float fm[2 * FC_TZ][2 * FC_TY][2 * FC_TX][9]; // fine moments [fz][fy][fx][m]
     const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
@@ -331,7 +332,7 @@ __global__ void __launch_bounds__(256) k_interp_fc(const S* ff, long long fqs, i
             const int fcell = i + fnxp * (j + g.nf[1] * k);
             float f[Q];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) f[q] = float(Store<S>::to_real(ff[q * fqs + fcell], q));
+            for (int q = 0; q < Q; ++q) f[q] = float(Store<S>::to_real(ff[q * fqs + fcell], q, sh));
             phys_moments(f, mom);
         }
 #pragma unroll
@@ -376,7 +377,7 @@ __global__ void __launch_bounds__(256) k_interp_fc(const S* ff, long long fqs, i
     float f[Q];
     invert_phys(f, avg, SF, g.eq_kind);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) fc[q * cqs + ccell] = Store<S>::from_real(typename Arith<S>::R(f[q]), q);
+    for (int q = 0; q < Q; ++q) fc[q * cqs + ccell] = Store<S>::from_real(typename Arith<S>::R(f[q]), q, sh);
 }
 
 } // namespace lbm

@@ -46,23 +46,31 @@ struct BHalf16 { uint16_t bits; };
 template <class S> struct Arith { using R = float; };
 template <> struct Arith<double> { using R = double; };
 
+// Offsets the 16-bit formats subtract before rounding: s_i = f_i^eq(rho = 1,
+// u = (U, 0, 0)), the equilibrium of the inflow (w_i when U = 0, the plain
+// shifted deviation f_i - w_i).  Passed by value in kernel arguments, so a
+// load costs one add with a scalar operand.  Ignored by FP32/FP64 storage.
+struct PopShift {
+    float s[19];
+};
+
 template <class S>
 struct Store;
 
 template <>
 struct Store<float> {
-    __device__ static __forceinline__ float load(const float* p, int) { return ld(p); }
-    __device__ static __forceinline__ void store(float* p, float v, int) { st(p, v); }
-    __host__ __device__ static float to_real(float v, int) { return v; }
-    __host__ __device__ static float from_real(float v, int) { return v; }
+    __device__ static __forceinline__ float load(const float* p, int, const PopShift&) { return ld(p); }
+    __device__ static __forceinline__ void store(float* p, float v, int, const PopShift&) { st(p, v); }
+    __host__ __device__ static float to_real(float v, int, const PopShift&) { return v; }
+    __host__ __device__ static float from_real(float v, int, const PopShift&) { return v; }
 };
 
 template <>
 struct Store<double> {
-    __device__ static __forceinline__ double load(const double* p, int) { return ld(p); }
-    __device__ static __forceinline__ void store(double* p, double v, int) { st(p, v); }
-    __host__ __device__ static double to_real(double v, int) { return v; }
-    __host__ __device__ static double from_real(double v, int) { return v; }
+    __device__ static __forceinline__ double load(const double* p, int, const PopShift&) { return ld(p); }
+    __device__ static __forceinline__ void store(double* p, double v, int, const PopShift&) { st(p, v); }
+    __host__ __device__ static double to_real(double v, int, const PopShift&) { return v; }
+    __host__ __device__ static double from_real(double v, int, const PopShift&) { return v; }
 };
 
 __host__ __device__ inline float half_bits_to_float(uint16_t h)
@@ -99,43 +107,41 @@ __host__ __device__ inline uint16_t float_to_bf16_bits(float v)
 
 template <>
 struct Store<Half16> {
-    __device__ static __forceinline__ float load(const Half16* p, int I)
+    __device__ static __forceinline__ float load(const Half16* p, int I, const PopShift& sh)
     {
-        const uint16_t h = ld(&p->bits);
-        return half_bits_to_float(h) + float(WNUM[I]) * (1.0f / 36.0f);
+        return half_bits_to_float(ld(&p->bits)) + sh.s[I];
     }
-    __device__ static __forceinline__ void store(Half16* p, float v, int I)
+    __device__ static __forceinline__ void store(Half16* p, float v, int I, const PopShift& sh)
     {
-        st(&p->bits, float_to_half_bits(v - float(WNUM[I]) * (1.0f / 36.0f)));
+        st(&p->bits, float_to_half_bits(v - sh.s[I]));
     }
-    __host__ __device__ static float to_real(Half16 v, int i)
+    __host__ __device__ static float to_real(Half16 v, int i, const PopShift& sh)
     {
-        return half_bits_to_float(v.bits) + float(WNUM[i]) * (1.0f / 36.0f);
+        return half_bits_to_float(v.bits) + sh.s[i];
     }
-    __host__ __device__ static Half16 from_real(float v, int i)
+    __host__ __device__ static Half16 from_real(float v, int i, const PopShift& sh)
     {
-        return Half16{float_to_half_bits(v - float(WNUM[i]) * (1.0f / 36.0f))};
+        return Half16{float_to_half_bits(v - sh.s[i])};
     }
 };
 
 template <>
 struct Store<BHalf16> {
-    __device__ static __forceinline__ float load(const BHalf16* p, int I)
+    __device__ static __forceinline__ float load(const BHalf16* p, int I, const PopShift& sh)
     {
-        const uint16_t h = ld(&p->bits);
-        return bf16_bits_to_float(h) + float(WNUM[I]) * (1.0f / 36.0f);
+        return bf16_bits_to_float(ld(&p->bits)) + sh.s[I];
     }
-    __device__ static __forceinline__ void store(BHalf16* p, float v, int I)
+    __device__ static __forceinline__ void store(BHalf16* p, float v, int I, const PopShift& sh)
     {
-        st(&p->bits, float_to_bf16_bits(v - float(WNUM[I]) * (1.0f / 36.0f)));
+        st(&p->bits, float_to_bf16_bits(v - sh.s[I]));
     }
-    __host__ __device__ static float to_real(BHalf16 v, int i)
+    __host__ __device__ static float to_real(BHalf16 v, int i, const PopShift& sh)
     {
-        return bf16_bits_to_float(v.bits) + float(WNUM[i]) * (1.0f / 36.0f);
+        return bf16_bits_to_float(v.bits) + sh.s[i];
     }
-    __host__ __device__ static BHalf16 from_real(float v, int i)
+    __host__ __device__ static BHalf16 from_real(float v, int i, const PopShift& sh)
     {
-        return BHalf16{float_to_bf16_bits(v - float(WNUM[i]) * (1.0f / 36.0f))};
+        return BHalf16{float_to_bf16_bits(v - sh.s[i])};
     }
 };
 
