@@ -52,8 +52,8 @@ extern "C" {
 
 /* population storage in HBM */
 #define LBM_STORE_F32 0  /* reference layout precision */
-#define LBM_STORE_F16 1  /* f_i - w_i in IEEE half, FP32 collide */
-#define LBM_STORE_BF16 2 /* f_i - w_i in bfloat16, FP32 collide */
+#define LBM_STORE_F16 1  /* f_i - f_i^eq(1, (umax,0,0)) in IEEE half, FP32 collide (= f_i - w_i when umax = 0) */
+#define LBM_STORE_BF16 2 /* the same offset in bfloat16, FP32 collide */
 #define LBM_STORE_F64 3  /* FP64 storage and arithmetic (the FP64 yardstick) */
 
 /* geometry generators standing in for ImageFcn() */

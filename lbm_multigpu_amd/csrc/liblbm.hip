@@ -238,7 +238,7 @@ static int alloc_level(lbm_ctx* ctx, Level& L, int nx, int ny, int nz, int stora
         for (int i = 0; i < Q; ++i) {
             const float w = float(WNUM[i]) * (1.0f / 36.0f);
             const float cu = float(CX[i]) * U;
-            L.shift.s[i] = w * (1.f + 3.f * cu + 4.5f * cu * cu - 1.5f * U * U);
+            L.shift.v[PopShift::slot(i)] = w * (1.f + 3.f * cu + 4.5f * cu * cu - 1.5f * U * U);
         }
     }
     const int nw = nwave_of_storage(L.storage);

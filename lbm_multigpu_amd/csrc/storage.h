@@ -50,8 +50,16 @@ template <> struct Arith<double> { using R = double; };
 // u = (U, 0, 0)), the equilibrium of the inflow (w_i when U = 0, the plain
 // shifted deviation f_i - w_i).  Passed by value in kernel arguments, so a
 // load costs one add with a scalar operand.  Ignored by FP32/FP64 storage.
+// Seven distinct values cover the 19 directions (rest; |c| = 1 with c_x =
+// +1, -1, 0; |c| = sqrt2 with c_x = +1, -1, 0), which keeps the scalar
+// registers they occupy low.
 struct PopShift {
-    float s[19];
+    float v[7];
+    __host__ __device__ static constexpr int slot(int i)
+    {
+        return i == 0 ? 0 : (WNUM[i] == 2 ? (CX[i] > 0 ? 1 : (CX[i] < 0 ? 2 : 3)) : (CX[i] > 0 ? 4 : (CX[i] < 0 ? 5 : 6)));
+    }
+    __host__ __device__ float operator[](int i) const { return v[slot(i)]; }
 };
 
 template <class S>
@@ -109,19 +117,19 @@ template <>
 struct Store<Half16> {
     __device__ static __forceinline__ float load(const Half16* p, int I, const PopShift& sh)
     {
-        return half_bits_to_float(ld(&p->bits)) + sh.s[I];
+        return half_bits_to_float(ld(&p->bits)) + sh[I];
     }
     __device__ static __forceinline__ void store(Half16* p, float v, int I, const PopShift& sh)
     {
-        st(&p->bits, float_to_half_bits(v - sh.s[I]));
+        st(&p->bits, float_to_half_bits(v - sh[I]));
     }
     __host__ __device__ static float to_real(Half16 v, int i, const PopShift& sh)
     {
-        return half_bits_to_float(v.bits) + sh.s[i];
+        return half_bits_to_float(v.bits) + sh[i];
     }
     __host__ __device__ static Half16 from_real(float v, int i, const PopShift& sh)
     {
-        return Half16{float_to_half_bits(v - sh.s[i])};
+        return Half16{float_to_half_bits(v - sh[i])};
     }
 };
 
@@ -129,19 +137,19 @@ template <>
 struct Store<BHalf16> {
     __device__ static __forceinline__ float load(const BHalf16* p, int I, const PopShift& sh)
     {
-        return bf16_bits_to_float(ld(&p->bits)) + sh.s[I];
+        return bf16_bits_to_float(ld(&p->bits)) + sh[I];
     }
     __device__ static __forceinline__ void store(BHalf16* p, float v, int I, const PopShift& sh)
     {
-        st(&p->bits, float_to_bf16_bits(v - sh.s[I]));
+        st(&p->bits, float_to_bf16_bits(v - sh[I]));
     }
     __host__ __device__ static float to_real(BHalf16 v, int i, const PopShift& sh)
     {
-        return bf16_bits_to_float(v.bits) + sh.s[i];
+        return bf16_bits_to_float(v.bits) + sh[i];
     }
     __host__ __device__ static BHalf16 from_real(float v, int i, const PopShift& sh)
     {
-        return BHalf16{float_to_bf16_bits(v - sh.s[i])};
+        return BHalf16{float_to_bf16_bits(v - sh[i])};
     }
 };
 
