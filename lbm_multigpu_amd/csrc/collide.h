@@ -123,6 +123,41 @@ __device__ __forceinline__ R les_omega(const Macro<R>& m, R omega0, R cs)
     return R(1) / tau;
 }
 
+// The same Smagorinsky rate from the moments the MRT collide already has:
+// tr P = (m1 + 30 rho)/19, P_xx = (tr P + m9)/3, P_yy - P_zz = m11,
+// P_xy = m13, P_yz = m14, P_xz = m15 (rows of the d'Humieres basis).  FAST
+// (the 16-bit storage kernels, which are bound by the vector ALUs) uses the
+// hardware reciprocal and square root instead of the IEEE sequences.
+template <bool FAST, class R>
+__device__ __forceinline__ R les_omega_from_moments(const R m[Q], R omega0, R cs)
+{
+    const R rho = m[0], jx = m[3], jy = m[5], jz = m[7];
+    const R tr = (m[1] + R(30) * rho) * R(1.0 / 19.0);
+    const R pxx = (tr + m[9]) * R(1.0 / 3.0);
+    const R pyz_sum = tr - pxx;
+    const R pyy = R(0.5) * (pyz_sum + m[11]);
+    const R pzz = R(0.5) * (pyz_sum - m[11]);
+    const R third = rho * R(1.0 / 3.0);
+    const R nxx = pxx - third - jx * jx;
+    const R nyy = pyy - third - jy * jy;
+    const R nzz = pzz - third - jz * jz;
+    const R nxy = m[13] - jx * jy;
+    const R nyz = m[14] - jy * jz;
+    const R nxz = m[15] - jx * jz;
+    const R q2 = nxx * nxx + nyy * nyy + nzz * nzz + R(2) * (nxy * nxy + nyz * nyz + nxz * nxz);
+    if constexpr (FAST && sizeof(R) == 4) {
+        const float q = __builtin_amdgcn_sqrtf(q2);
+        const float tau0 = __builtin_amdgcn_rcpf(omega0);
+        const float tau = 0.5f * tau0 + 0.5f * __builtin_amdgcn_sqrtf(tau0 * tau0 + float(18.0 * 1.4142135623730951) * cs * q);
+        return __builtin_amdgcn_rcpf(tau);
+    } else {
+        const R q = sqrt(q2);
+        const R tau0 = R(1) / omega0;
+        const R tau = R(0.5) * tau0 + R(0.5) * sqrt(tau0 * tau0 + R(18.0 * 1.4142135623730951) * cs * q);
+        return R(1) / tau;
+    }
+}
+
 // AddForce (MRTtest/DynInlet3.cu:229-253): f_i -= 3 w_i c_iy dpdy.
 template <class R>
 __device__ __forceinline__ void add_force(R f[Q], R dpdy)
@@ -174,7 +209,7 @@ __device__ __forceinline__ void collide_bgk(R f[Q], const CollideParams<R>& cp)
 //    whose 3pixx entry is -meq9/6 as coded); s9,11,13,14,15 = omega_LES,
 //    every other moment to equilibrium; then AddForce.
 //  MODEL_MRT_DHUM: as MRT_LES with the d'Humieres rates of :774-783.
-template <int MODEL, class R>
+template <int MODEL, class R, bool FAST = false>
 __device__ __forceinline__ void collide_mrt(R f[Q], const CollideParams<R>& cp)
 {
     R s[NP], d[NP];
@@ -186,10 +221,7 @@ __device__ __forceinline__ void collide_mrt(R f[Q], const CollideParams<R>& cp)
     const R usqr = u * u + v * v + w * w;
 
     R omega = cp.omega;
-    if (MODEL != MODEL_MRT_CPU3D && cp.cs > R(0)) {
-        const Macro<R> mm = macro_from_pairs(f[0], s, d);
-        omega = les_omega(mm, cp.omega, cp.cs);
-    }
+    if (MODEL != MODEL_MRT_CPU3D && cp.cs > R(0)) omega = les_omega_from_moments<FAST>(m, cp.omega, cp.cs);
 
     R meq[Q];
 #pragma unroll
@@ -224,12 +256,13 @@ __device__ __forceinline__ void collide_mrt(R f[Q], const CollideParams<R>& cp)
         rate[16] = cp.s16; rate[17] = cp.s16; rate[18] = cp.s16;
     }
 
-    // scaled non-equilibrium parts dm_k = s_k (m_k - meq_k) / |row_k|^2
+    // scaled non-equilibrium parts dm_k = (s_k / |row_k|^2) (m_k - meq_k);
+    // moments sharing a rate and norm share the product
     R dm[Q];
 #pragma unroll
     for (int k = 0; k < Q; ++k) {
         const bool conserved = (k == 0 || k == 3 || k == 5 || k == 7);
-        dm[k] = conserved ? R(0) : rate[k] * (m[k] - meq[k]) * (R(1) / R(NORM[k]));
+        dm[k] = conserved ? R(0) : (rate[k] * (R(1) / R(NORM[k]))) * (m[k] - meq[k]);
     }
 
     // f_i -= sum_k M[k][i] dm_k, split into even (E) and odd (O) parts per pair
@@ -257,11 +290,11 @@ __device__ __forceinline__ void collide_mrt(R f[Q], const CollideParams<R>& cp)
     if (MODEL != MODEL_MRT_CPU3D && cp.dpdy != R(0)) add_force(f, cp.dpdy);
 }
 
-template <int MODEL, class R>
+template <int MODEL, class R, bool FAST = false>
 __device__ __forceinline__ void collide(R f[Q], const CollideParams<R>& cp)
 {
     if (MODEL == MODEL_BGK) collide_bgk(f, cp);
-    else collide_mrt<MODEL>(f, cp);
+    else collide_mrt<MODEL, R, FAST>(f, cp);
 }
 
 // West_Extrap (MRTtest/DynInlet3.cu:973-1017) applied to the neighbour's

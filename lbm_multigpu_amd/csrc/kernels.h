@@ -36,12 +36,25 @@ enum Bc : int { BC_PERIODIC = 0, BC_CLAMP = 1, BC_WALL = 2 };
 // 1 KB stretch per plane and workgroup); the VALU-bound 16-bit kernels run
 // best with one (measured on the 512x128x128 channel: 35.1 vs 32.2 GLUPS
 // FP32, 46.2 vs 41.7 GLUPS FP16).
+#ifndef LBM_NPL16
+#define LBM_NPL16 1 // nodes per lane of the 16-bit stream-collide (2: 34.1 vs 48.7 GLUPS, 4 waves/SIMD)
+#endif
+#ifndef LBM_MINW16
+#define LBM_MINW16 1 // minimum wavefronts per SIMD for the 16-bit stream-collide registers
+#endif
 template <class S>
 struct Shape {
     static constexpr int NW = sizeof(S) == 2 ? LBM_NWAVE16 : LBM_NWAVE;
     static constexpr int THREADS = 64 * NW;
+    // row segments per wavefront in the stream-collide (a knob for the
+    // 16-bit kernel, which waits on memory 58 % of its cycles; two segments
+    // per lane measured slower: the registers drop it to 4 waves per SIMD)
+    static constexpr int NPL = sizeof(S) == 2 ? LBM_NPL16 : 1;
+    // minimum wavefronts per SIMD the compiler must fit the registers to
+    static constexpr int MIN_WAVES = sizeof(S) == 2 ? LBM_MINW16 : LBM_MIN_BLOCKS;
 };
 __host__ inline int nwave_of_storage(int storage) { return (storage == 1 || storage == 2) ? LBM_NWAVE16 : LBM_NWAVE; }
+__host__ inline int npl_of_storage(int storage) { return (storage == 1 || storage == 2) ? LBM_NPL16 : 1; }
 
 // wavefront index within the workgroup and lane within the wavefront
 __device__ __forceinline__ int wave_id() { return (int)(threadIdx.x >> 6); }
@@ -51,11 +64,11 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 // order (segment fastest, then y, then z) and workgroup lb takes NW
 // consecutive ones, so every workgroup streams one contiguous stretch of
 // each plane whatever nx is.  Returns false past the last row.
-template <int NW>
-__device__ __forceinline__ bool wave_node(int lb, int nsegw, int ny, int nz, int& x, int& y, int& z)
+template <int NW, int NPL = 1>
+__device__ __forceinline__ bool wave_node(int lb, int nsegw, int ny, int nz, int& x, int& y, int& z, int k = 0)
 {
     // wave-uniform: the divisions run on the scalar unit, not per lane
-    const int g = __builtin_amdgcn_readfirstlane(lb * NW + wave_id());
+    const int g = __builtin_amdgcn_readfirstlane((lb * NW + wave_id()) * NPL + k);
     const int row = g / nsegw;
     x = (g - row * nsegw) * 64 + lane_id();
     y = row % ny;
@@ -282,16 +295,11 @@ __device__ __forceinline__ void stats_update(const StepParams& p, int cell, cons
     }
 }
 
-// Fused pull stream + boundary dispatch + collide + bounce-back for one node.
+// Collide (per the node's action) and store one pulled node.
 template <int MODEL, class S>
-__device__ __forceinline__ void node_update(const StepParams& p, int x, int y, int z,
-                                            typename Arith<S>::R& fx, typename Arith<S>::R& fy,
-                                            typename Arith<S>::R& fz)
+__device__ __forceinline__ void node_finish(const StepParams& p, int cell, int act, typename Arith<S>::R f[Q])
 {
     using R = typename Arith<S>::R;
-    const int cell = x + p.nxp * (y + p.ny * z);
-    R f[Q];
-    const int act = node_pull<S>(p, x, y, z, cell, f, fx, fy, fz);
     if (act == ACT_SKIP) return;
     if (act == ACT_COLLIDE) {
         CollideParams<R> cp = collide_params<R>(p);
@@ -302,10 +310,23 @@ __device__ __forceinline__ void node_update(const StepParams& p, int x, int y, i
                 cp.cs = R(0);
             }
         }
-        collide<MODEL>(f, cp);
+        collide<MODEL, R, sizeof(S) == 2>(f, cp);
         if (p.stat_mode) stats_update(p, cell, f);
     }
     node_store<S>(p, cell, f);
+}
+
+// Fused pull stream + boundary dispatch + collide + bounce-back for one node.
+template <int MODEL, class S>
+__device__ __forceinline__ void node_update(const StepParams& p, int x, int y, int z,
+                                            typename Arith<S>::R& fx, typename Arith<S>::R& fy,
+                                            typename Arith<S>::R& fz)
+{
+    using R = typename Arith<S>::R;
+    const int cell = x + p.nxp * (y + p.ny * z);
+    R f[Q];
+    const int act = node_pull<S>(p, x, y, z, cell, f, fx, fy, fz);
+    node_finish<MODEL, S>(p, cell, act, f);
 }
 
 // Momentum-exchange partial of one workgroup: wavefront sums, then the
@@ -332,15 +353,35 @@ __device__ __forceinline__ void block_force_partial(const StepParams& p, int lb,
 }
 
 template <int MODEL, class S>
-__global__ void __launch_bounds__(Shape<S>::THREADS, LBM_MIN_BLOCKS) k_stream_collide(StepParams p)
+__global__ void __launch_bounds__(Shape<S>::THREADS, Shape<S>::MIN_WAVES) k_stream_collide(StepParams p)
 {
     using R = typename Arith<S>::R;
-    constexpr int NW = Shape<S>::NW;
+    constexpr int NW = Shape<S>::NW, NPL = Shape<S>::NPL;
     const int lb = logical_block();
-    int x, y, z;
-    const bool row_ok = wave_node<NW>(lb, p.nsegw, p.ny, p.nz, x, y, z);
     R fx = 0, fy = 0, fz = 0;
-    if (row_ok && x < p.nx) node_update<MODEL, S>(p, x, y, z, fx, fy, fz);
+    if constexpr (NPL == 1) {
+        int x, y, z;
+        const bool row_ok = wave_node<NW>(lb, p.nsegw, p.ny, p.nz, x, y, z);
+        if (row_ok && x < p.nx) node_update<MODEL, S>(p, x, y, z, fx, fy, fz);
+    } else {
+        // pull every segment's node first, so all their loads are in flight
+        // together, then collide and store them one by one
+        R f[NPL][Q];
+        int act[NPL], cell[NPL];
+#pragma unroll
+        for (int k = 0; k < NPL; ++k) {
+            int x, y, z;
+            const bool row_ok = wave_node<NW, NPL>(lb, p.nsegw, p.ny, p.nz, x, y, z, k);
+            act[k] = ACT_SKIP;
+            cell[k] = 0;
+            if (row_ok && x < p.nx) {
+                cell[k] = x + p.nxp * (y + p.ny * z);
+                act[k] = node_pull<S>(p, x, y, z, cell[k], f[k], fx, fy, fz);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NPL; ++k) node_finish<MODEL, S>(p, cell[k], act[k], f[k]);
+    }
     if (p.fpart) block_force_partial<NW>(p, lb, fx, fy, fz);
 }
 

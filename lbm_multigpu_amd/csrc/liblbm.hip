@@ -37,7 +37,8 @@ struct Level {
     std::vector<unsigned char> host_mask; // dense codes
     float* stat = nullptr;         // running statistics, 6 planes of qs
     PopShift shift{};              // 16-bit storage offsets (equilibrium of the inflow)
-    int blocks = 0;
+    int blocks = 0;    // stream-collide workgroups (Shape::NPL row segments per wavefront)
+    int blocks1 = 0;   // workgroups of the one-node-per-lane kernels (closure, mass sums)
     size_t bytes = 0;
 };
 
@@ -241,8 +242,10 @@ static int alloc_level(lbm_ctx* ctx, Level& L, int nx, int ny, int nz, int stora
             L.shift.v[PopShift::slot(i)] = w * (1.f + 3.f * cu + 4.5f * cu * cu - 1.5f * U * U);
         }
     }
-    const int nw = nwave_of_storage(L.storage);
-    L.blocks = (int)(((long long)L.nsegw * ny * nz + nw - 1) / nw);
+    const int nw = nwave_of_storage(L.storage), npl = npl_of_storage(L.storage);
+    const long long segs = (long long)L.nsegw * ny * nz;
+    L.blocks = (int)((segs + (long long)nw * npl - 1) / ((long long)nw * npl));
+    L.blocks1 = (int)((segs + nw - 1) / nw);
     const size_t lattice = (size_t)Q * (size_t)L.qs * L.esize;
     HIP_TRY(ctx, hipMalloc(&L.fa, lattice));
     HIP_TRY(ctx, hipMalloc(&L.fb, lattice));
@@ -304,8 +307,8 @@ extern "C" int lbm_create(const lbm_config* cfg, lbm_ctx** out)
     std::vector<float> prof((size_t)cfg->ny * cfg->nz);
     inlet_profile(cfg->ny, cfg->nz, cfg->umax, cfg->inlet_amp, cfg->seed, prof.data());
     if (hipMalloc(&ctx->u_in, prof.size() * sizeof(float)) != hipSuccess ||
-        hipMalloc(&ctx->fpart, (size_t)L.blocks * 3 * sizeof(double)) != hipSuccess ||
-        hipMalloc(&ctx->mpart, (size_t)L.blocks * 4 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ctx->fpart, (size_t)L.blocks1 * 3 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ctx->mpart, (size_t)L.blocks1 * 4 * sizeof(double)) != hipSuccess ||
         hipMalloc(&ctx->red, 8 * sizeof(double)) != hipSuccess)
         return bail(fail(nullptr, LBM_ENOMEM, "lbm_create: device allocation failed"));
     if (cfg->force_steps > 0) {
@@ -612,14 +615,17 @@ static int coarse_step(lbm_ctx* ctx)
     StepParams p = step_params(ctx, L, c.omega);
     const bool rec = c.force_steps > 0 && ctx->t >= c.startf && (ctx->t - c.startf) < c.force_steps;
     if (rec) p.fpart = ctx->fpart;
+    int nparts = L.blocks; // force partials the launch writes
     with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         if (!ctx->mlp) {
             launch_sc<S>(c.model, p, L.blocks, ctx->stream);
         } else if (ctx->closure_every <= 1) {
-            launch_closure<S>(ctx, p, L.blocks, nullptr);
+            launch_closure<S>(ctx, p, L.blocks1, nullptr);
+            nparts = L.blocks1;
         } else if ((ctx->t - ctx->closure_t0) % ctx->closure_every == 0) {
-            launch_closure<S>(ctx, p, L.blocks, ctx->omega_field); // refresh: network + rates for the next steps
+            launch_closure<S>(ctx, p, L.blocks1, ctx->omega_field); // refresh: network + rates for the next steps
+            nparts = L.blocks1;
         } else {
             StepParams q = p;
             q.omega_node = ctx->omega_field;
@@ -630,7 +636,7 @@ static int coarse_step(lbm_ctx* ctx)
     if (rec) {
         const long long slot = ctx->t - c.startf;
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, ctx->stream, ctx->fpart,
-                           (long long)L.blocks, 3, ctx->fhist + slot * 3);
+                           (long long)nparts, 3, ctx->fhist + slot * 3);
         ctx->force_recorded = (int)std::max<long long>(ctx->force_recorded, slot + 1);
     }
     std::swap(L.fa, L.fb);
@@ -916,16 +922,16 @@ extern "C" int lbm_get_mass_momentum(lbm_ctx* ctx, int level, double out[4])
     double* part = ctx->mpart;
     double* tmp = nullptr;
     if (level > 0) {
-        HIP_TRY(ctx, hipMalloc(&tmp, (size_t)L.blocks * 4 * sizeof(double)));
+        HIP_TRY(ctx, hipMalloc(&tmp, (size_t)L.blocks1 * 4 * sizeof(double)));
         part = tmp;
     }
     with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        hipLaunchKernelGGL(k_mass_momentum<S>, dim3(L.blocks), dim3(Shape<S>::THREADS), 0, ctx->stream, (const S*)L.fa,
+        hipLaunchKernelGGL(k_mass_momentum<S>, dim3(L.blocks1), dim3(Shape<S>::THREADS), 0, ctx->stream, (const S*)L.fa,
                            L.mask, L.qs, L.nx, L.ny, L.nxp, L.nsegw, L.nz, part, L.shift);
         return 0;
     });
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, ctx->stream, part, (long long)L.blocks, 4,
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, ctx->stream, part, (long long)L.blocks1, 4,
                        ctx->red);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(out, ctx->red, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);

@@ -113,15 +113,25 @@ __host__ __device__ inline uint16_t float_to_bf16_bits(float v)
     return (uint16_t)(u >> 16);
 }
 
+// The FP16 load and store each fold the conversion and the offset into one
+// mixed-precision FMA (v_fma_mix_f32: f16 source times 1.0 plus the offset;
+// v_fma_mixlo_f16: the f32 value minus the offset, rounded once to f16),
+// instead of a conversion plus an add.  The offsets are kernel arguments, so
+// they sit in scalar registers.
 template <>
 struct Store<Half16> {
     __device__ static __forceinline__ float load(const Half16* p, int I, const PopShift& sh)
     {
-        return half_bits_to_float(ld(&p->bits)) + sh[I];
+        const unsigned h = ld(&p->bits);
+        float v;
+        asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(v) : "v"(h), "s"(sh[I]));
+        return v;
     }
     __device__ static __forceinline__ void store(Half16* p, float v, int I, const PopShift& sh)
     {
-        st(&p->bits, float_to_half_bits(v - sh[I]));
+        unsigned o;
+        asm("v_fma_mixlo_f16 %0, %1, 1.0, -%2" : "=v"(o) : "v"(v), "s"(sh[I]));
+        st(&p->bits, (uint16_t)o);
     }
     __host__ __device__ static float to_real(Half16 v, int i, const PopShift& sh)
     {
@@ -141,7 +151,8 @@ struct Store<BHalf16> {
     }
     __device__ static __forceinline__ void store(BHalf16* p, float v, int I, const PopShift& sh)
     {
-        st(&p->bits, float_to_bf16_bits(v - sh[I]));
+        // v_cvt_pk_bf16_f32: round to nearest even, as float_to_bf16_bits
+        st(&p->bits, __builtin_bit_cast(uint16_t, (__bf16)(v - sh[I])));
     }
     __host__ __device__ static float to_real(BHalf16 v, int i, const PopShift& sh)
     {
