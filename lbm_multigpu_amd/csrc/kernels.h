@@ -27,17 +27,28 @@ enum Bc : int { BC_PERIODIC = 0, BC_CLAMP = 1, BC_WALL = 2 };
 #ifndef LBM_XCD_REMAP
 #define LBM_XCD_REMAP 0
 #endif
+#ifndef LBM_XCD_CHUNK
+#define LBM_XCD_CHUNK 4 // consecutive stream-collide workgroups per XCD, FP32/FP64
+#endif
+#ifndef LBM_XCD_CHUNK16
+#define LBM_XCD_CHUNK16 7 // the same for the 16-bit formats
+#endif
 #ifndef LBM_MIN_BLOCKS
 #define LBM_MIN_BLOCKS 1 // __launch_bounds__ minimum workgroups per CU
 #endif
 
 // Workgroup shape per population type: NW wavefronts of 64 lanes on NW
-// consecutive row segments.  FP32 streams best with four (one contiguous
-// 1 KB stretch per plane and workgroup); the VALU-bound 16-bit kernels run
-// best with one (measured on the 512x128x128 channel: 35.1 vs 32.2 GLUPS
-// FP32, 46.2 vs 41.7 GLUPS FP16).
-#ifndef LBM_NPL16
-#define LBM_NPL16 1 // nodes per lane of the 16-bit stream-collide (2: 34.1 vs 48.7 GLUPS, 4 waves/SIMD)
+// consecutive row segments, one node per lane.  FP32 streams best with four
+// (one contiguous 1 KB stretch per plane and workgroup); the VALU-bound
+// 16-bit kernels run best with one (measured on the 512x128x128 channel:
+// 35.1 vs 32.2 GLUPS FP32, 46.2 vs 41.7 GLUPS FP16).  Each wavefront takes
+// SEGS row segments in turn (a loop, no software pipelining), so a
+// workgroup streams SEGS x NW consecutive segments.
+#ifndef LBM_SEGS
+#define LBM_SEGS 1 // row segments per wavefront of the FP32/FP64 stream-collide
+#endif
+#ifndef LBM_SEGS16
+#define LBM_SEGS16 1 // row segments per wavefront of the 16-bit stream-collide
 #endif
 #ifndef LBM_MINW16
 #define LBM_MINW16 1 // minimum wavefronts per SIMD for the 16-bit stream-collide registers
@@ -46,15 +57,20 @@ template <class S>
 struct Shape {
     static constexpr int NW = sizeof(S) == 2 ? LBM_NWAVE16 : LBM_NWAVE;
     static constexpr int THREADS = 64 * NW;
-    // row segments per wavefront in the stream-collide (a knob for the
-    // 16-bit kernel, which waits on memory 58 % of its cycles; two segments
-    // per lane measured slower: the registers drop it to 4 waves per SIMD)
-    static constexpr int NPL = sizeof(S) == 2 ? LBM_NPL16 : 1;
+    static constexpr int SEGS = sizeof(S) == 2 ? LBM_SEGS16 : LBM_SEGS;
     // minimum wavefronts per SIMD the compiler must fit the registers to
     static constexpr int MIN_WAVES = sizeof(S) == 2 ? LBM_MINW16 : LBM_MIN_BLOCKS;
 };
-__host__ inline int nwave_of_storage(int storage) { return (storage == 1 || storage == 2) ? LBM_NWAVE16 : LBM_NWAVE; }
-__host__ inline int npl_of_storage(int storage) { return (storage == 1 || storage == 2) ? LBM_NPL16 : 1; }
+__host__ inline bool is16_storage(int storage) { return storage == 1 || storage == 2; }
+__host__ inline int nwave_of_storage(int storage) { return is16_storage(storage) ? LBM_NWAVE16 : LBM_NWAVE; }
+__host__ inline int segs_of_storage(int storage) { return is16_storage(storage) ? LBM_SEGS16 : LBM_SEGS; }
+// consecutive stream-collide workgroups per XCD (logical_block) on a level
+// with nsegw 64-node segments per row: only rows longer than a workgroup's
+// stretch have workgroup boundaries inside them
+__host__ inline int xcd_chunk_of(int storage, int nsegw)
+{
+    return nsegw > nwave_of_storage(storage) ? (is16_storage(storage) ? LBM_XCD_CHUNK16 : LBM_XCD_CHUNK) : 1;
+}
 
 // wavefront index within the workgroup and lane within the wavefront
 __device__ __forceinline__ int wave_id() { return (int)(threadIdx.x >> 6); }
@@ -64,11 +80,11 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 // order (segment fastest, then y, then z) and workgroup lb takes NW
 // consecutive ones, so every workgroup streams one contiguous stretch of
 // each plane whatever nx is.  Returns false past the last row.
-template <int NW, int NPL = 1>
-__device__ __forceinline__ bool wave_node(int lb, int nsegw, int ny, int nz, int& x, int& y, int& z, int k = 0)
+template <int NW>
+__device__ __forceinline__ bool wave_node(int lb, int nsegw, int ny, int nz, int& x, int& y, int& z)
 {
     // wave-uniform: the divisions run on the scalar unit, not per lane
-    const int g = __builtin_amdgcn_readfirstlane((lb * NW + wave_id()) * NPL + k);
+    const int g = __builtin_amdgcn_readfirstlane(lb * NW + wave_id());
     const int row = g / nsegw;
     x = (g - row * nsegw) * 64 + lane_id();
     y = row % ny;
@@ -97,17 +113,33 @@ struct StepParams {
     // per-node collision rate left by the last learned-closure step (padded
     // layout): > 0 replaces the LES rate, 0 means the Smagorinsky rate
     const float* omega_node;
-    PopShift sh; // 16-bit storage offsets
+    PopShift sh;   // 16-bit storage offsets
+    int xcd_chunk; // consecutive workgroups per XCD (logical_block)
 };
 
-// XCD-aware renumbering: the hardware id hw goes to XCD hw % 8; give XCD j
-// the logical block range [j*G/8, (j+1)*G/8) (bijective for any G).
-__device__ __forceinline__ int logical_block()
+// XCD-aware renumbering.  The hardware id hw goes to XCD hw % 8.
+//  - LBM_XCD_REMAP: XCD j takes the logical block range [j*G/8, (j+1)*G/8)
+//    (bijective for any G);
+//  - chunk = C > 1: within each window of 8C hardware ids, XCD j
+//    takes the C consecutive logical blocks from j*C, so the 8 XCDs still
+//    sweep the lattice together but each streams C workgroups' stretches
+//    back to back (the x-shifted loads' straddling lines then stay in one
+//    XCD's L2).  Ids past the last whole window map to themselves.
+__device__ __forceinline__ int logical_block(int chunk = 1)
 {
-    if (!LBM_XCD_REMAP) return blockIdx.x;
     const int G = gridDim.x, hw = blockIdx.x;
-    const int per = G >> 3, rem = G & 7, xcd = hw & 7;
-    return xcd * per + (xcd < rem ? xcd : rem) + (hw >> 3);
+    if (LBM_XCD_REMAP) {
+        const int per = G >> 3, rem = G & 7, xcd = hw & 7;
+        return xcd * per + (xcd < rem ? xcd : rem) + (hw >> 3);
+    }
+    if (chunk > 1) { // wave-uniform: scalar arithmetic
+        const int W = 8 * chunk;
+        if (hw < (G / W) * W) {
+            const int r = hw % W;
+            return hw - r + (r & 7) * chunk + (r >> 3);
+        }
+    }
+    return hw;
 }
 
 __device__ __forceinline__ void axis_nb(int c, int n, int bc, int& m, int& p, bool& om, bool& op)
@@ -149,23 +181,12 @@ __device__ __forceinline__ R wave_sum(R v)
 
 enum NodeAction : int { ACT_SKIP = 0, ACT_STORE = 1, ACT_COLLIDE = 2 };
 
-// Pull stream + bounce-back + boundary conditions for one node (the work of
-// update_inn before mrt_collide, MRTtest/DynInlet3.cu:1612-1771, and of the
-// CPU sweep CPU_3D/CPU_3D.cpp:1320-1420).  Leaves in f the populations to
-// collide (ACT_COLLIDE), or the reversed ones of a node bounce-back solid
-// (ACT_STORE), or nothing (ACT_SKIP: a solid under link bounce-back).
+// The 19 pulled populations of node (x, y, z): neighbour rows per axis
+// (periodic, clamped or walled), then one load per direction.
 template <class S>
-__device__ __forceinline__ int node_pull(const StepParams& p, int x, int y, int z, int cell,
-                                         typename Arith<S>::R f[Q], typename Arith<S>::R& fx,
-                                         typename Arith<S>::R& fy, typename Arith<S>::R& fz)
+__device__ __forceinline__ void node_load(const StepParams& p, int x, int y, int z, typename Arith<S>::R f[Q])
 {
-    using R = typename Arith<S>::R;
     const S* __restrict__ fin = static_cast<const S*>(p.fin);
-    const long long qs = p.qs;
-    const unsigned char code = p.mask ? *at(p.mask, cell) : (unsigned char)0;
-    const unsigned char base = code & 0x7f;
-    if (p.halfway && is_solid(base)) return ACT_SKIP; // inert under link bounce-back
-
     int xm, xp, ym, yp, zm, zp;
     bool oxm, oxp, oym, oyp, ozm, ozp;
     axis_nb(x, p.nx, p.bcx, xm, xp, oxm, oxp);
@@ -174,28 +195,49 @@ __device__ __forceinline__ int node_pull(const StepParams& p, int x, int y, int 
     const int Y[3] = {ym, y, yp};
     const int Z[3] = {zm, z, zp};
     const int X[3] = {xm, x, xp};
-    int row[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) row[a][b] = p.nxp * (Y[a] + p.ny * Z[b]);
-
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        const int src = row[1 - CY[i]][1 - CZ[i]] + X[1 - CX[i]];
-        f[i] = Store<S>::load(at(fin + i * qs, src), i, p.sh);
+        const int src = p.nxp * (Y[1 - CY[i]] + p.ny * Z[1 - CZ[i]]) + X[1 - CX[i]];
+        f[i] = Store<S>::load(at(fin + i * p.qs, src), i, p.sh);
     }
+}
+
+// Boundary handling of a pulled node (the work of update_inn before
+// mrt_collide, MRTtest/DynInlet3.cu:1612-1771, and of the CPU sweep
+// CPU_3D/CPU_3D.cpp:1320-1420), given its node code.  Leaves in f the
+// populations to collide (ACT_COLLIDE), or the reversed ones of a node
+// bounce-back solid (ACT_STORE), or nothing (ACT_SKIP: a solid under link
+// bounce-back).
+template <class S>
+__device__ __forceinline__ int node_fix(const StepParams& p, int x, int y, int z, int cell, unsigned char code,
+                                        typename Arith<S>::R f[Q], typename Arith<S>::R& fx,
+                                        typename Arith<S>::R& fy, typename Arith<S>::R& fz)
+{
+    using R = typename Arith<S>::R;
+    const S* __restrict__ fin = static_cast<const S*>(p.fin);
+    const long long qs = p.qs;
+    const unsigned char base = code & 0x7f;
+    if (p.halfway && is_solid(base)) return ACT_SKIP; // inert under link bounce-back
 
     if (code & NODE_NEAR) {
         // links that bounce: out of the domain through a wall face, or (link
         // mode) from a solid node; the population comes back reversed from
         // this node's own previous post-collision value.
+        int xm, xp, ym, yp, zm, zp;
+        bool oxm, oxp, oym, oyp, ozm, ozp;
+        axis_nb(x, p.nx, p.bcx, xm, xp, oxm, oxp);
+        axis_nb(y, p.ny, p.bcy, ym, yp, oym, oyp);
+        axis_nb(z, p.nz, p.bcz, zm, zp, ozm, ozp);
+        const int Y[3] = {ym, y, yp};
+        const int Z[3] = {zm, z, zp};
+        const int X[3] = {xm, x, xp};
 #pragma unroll
         for (int i = 1; i < Q; ++i) {
             const bool out = (CX[i] > 0 && oxm) || (CX[i] < 0 && oxp) || (CY[i] > 0 && oym) ||
                              (CY[i] < 0 && oyp) || (CZ[i] > 0 && ozm) || (CZ[i] < 0 && ozp);
             unsigned char sc = 0;
-            if (!out && p.halfway) sc = *at(p.mask, row[1 - CY[i]][1 - CZ[i]] + X[1 - CX[i]]) & 0x7f;
+            if (!out && p.halfway)
+                sc = *at(p.mask, p.nxp * (Y[1 - CY[i]] + p.ny * Z[1 - CZ[i]]) + X[1 - CX[i]]) & 0x7f;
             const bool bounce = out || is_solid(sc);
             if (bounce) {
                 const R v = Store<S>::load(at(fin + OPP[i] * qs, cell), OPP[i], p.sh);
@@ -248,6 +290,19 @@ __device__ __forceinline__ int node_pull(const StepParams& p, int x, int y, int 
     return ACT_COLLIDE;
 }
 
+// Pull stream + bounce-back + boundary conditions for one node.  A solid
+// under link bounce-back loads nothing.
+template <class S>
+__device__ __forceinline__ int node_pull(const StepParams& p, int x, int y, int z, int cell,
+                                         typename Arith<S>::R f[Q], typename Arith<S>::R& fx,
+                                         typename Arith<S>::R& fy, typename Arith<S>::R& fz)
+{
+    const unsigned char code = p.mask ? *at(p.mask, cell) : (unsigned char)0;
+    if (p.halfway && is_solid(code & 0x7f)) return ACT_SKIP;
+    node_load<S>(p, x, y, z, f);
+    return node_fix<S>(p, x, y, z, cell, code, f, fx, fy, fz);
+}
+
 template <class R>
 __device__ __forceinline__ CollideParams<R> collide_params(const StepParams& p)
 {
@@ -295,24 +350,30 @@ __device__ __forceinline__ void stats_update(const StepParams& p, int cell, cons
     }
 }
 
+// Collide a pulled node per its action (plus its running statistics).
+template <int MODEL, class S>
+__device__ __forceinline__ void node_collide(const StepParams& p, int cell, int act, typename Arith<S>::R f[Q])
+{
+    using R = typename Arith<S>::R;
+    if (act != ACT_COLLIDE) return;
+    CollideParams<R> cp = collide_params<R>(p);
+    if (p.omega_node) {
+        const float w = *at(p.omega_node, cell);
+        if (w > 0.f) {
+            cp.omega = R(w);
+            cp.cs = R(0);
+        }
+    }
+    collide<MODEL, R, sizeof(S) == 2>(f, cp);
+    if (p.stat_mode) stats_update(p, cell, f);
+}
+
 // Collide (per the node's action) and store one pulled node.
 template <int MODEL, class S>
 __device__ __forceinline__ void node_finish(const StepParams& p, int cell, int act, typename Arith<S>::R f[Q])
 {
-    using R = typename Arith<S>::R;
     if (act == ACT_SKIP) return;
-    if (act == ACT_COLLIDE) {
-        CollideParams<R> cp = collide_params<R>(p);
-        if (p.omega_node) {
-            const float w = *at(p.omega_node, cell);
-            if (w > 0.f) {
-                cp.omega = R(w);
-                cp.cs = R(0);
-            }
-        }
-        collide<MODEL, R, sizeof(S) == 2>(f, cp);
-        if (p.stat_mode) stats_update(p, cell, f);
-    }
+    node_collide<MODEL, S>(p, cell, act, f);
     node_store<S>(p, cell, f);
 }
 
@@ -356,31 +417,20 @@ template <int MODEL, class S>
 __global__ void __launch_bounds__(Shape<S>::THREADS, Shape<S>::MIN_WAVES) k_stream_collide(StepParams p)
 {
     using R = typename Arith<S>::R;
-    constexpr int NW = Shape<S>::NW, NPL = Shape<S>::NPL;
-    const int lb = logical_block();
+    constexpr int NW = Shape<S>::NW, SEGS = Shape<S>::SEGS;
+    const int lb = logical_block(p.xcd_chunk);
     R fx = 0, fy = 0, fz = 0;
-    if constexpr (NPL == 1) {
+    if constexpr (SEGS == 1) {
         int x, y, z;
         const bool row_ok = wave_node<NW>(lb, p.nsegw, p.ny, p.nz, x, y, z);
         if (row_ok && x < p.nx) node_update<MODEL, S>(p, x, y, z, fx, fy, fz);
     } else {
-        // pull every segment's node first, so all their loads are in flight
-        // together, then collide and store them one by one
-        R f[NPL][Q];
-        int act[NPL], cell[NPL];
-#pragma unroll
-        for (int k = 0; k < NPL; ++k) {
+#pragma unroll 1
+        for (int k = 0; k < SEGS; ++k) {
             int x, y, z;
-            const bool row_ok = wave_node<NW, NPL>(lb, p.nsegw, p.ny, p.nz, x, y, z, k);
-            act[k] = ACT_SKIP;
-            cell[k] = 0;
-            if (row_ok && x < p.nx) {
-                cell[k] = x + p.nxp * (y + p.ny * z);
-                act[k] = node_pull<S>(p, x, y, z, cell[k], f[k], fx, fy, fz);
-            }
+            const bool row_ok = wave_node<NW>(lb * SEGS + k, p.nsegw, p.ny, p.nz, x, y, z);
+            if (row_ok && x < p.nx) node_update<MODEL, S>(p, x, y, z, fx, fy, fz);
         }
-#pragma unroll
-        for (int k = 0; k < NPL; ++k) node_finish<MODEL, S>(p, cell[k], act[k], f[k]);
     }
     if (p.fpart) block_force_partial<NW>(p, lb, fx, fy, fz);
 }

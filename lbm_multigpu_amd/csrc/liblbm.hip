@@ -37,7 +37,7 @@ struct Level {
     std::vector<unsigned char> host_mask; // dense codes
     float* stat = nullptr;         // running statistics, 6 planes of qs
     PopShift shift{};              // 16-bit storage offsets (equilibrium of the inflow)
-    int blocks = 0;    // stream-collide workgroups (Shape::NPL row segments per wavefront)
+    int blocks = 0;    // stream-collide workgroups (Shape::SEGS row segments per wavefront)
     int blocks1 = 0;   // workgroups of the one-node-per-lane kernels (closure, mass sums)
     size_t bytes = 0;
 };
@@ -191,6 +191,15 @@ extern "C" void lbm_inlet_profile(int ny, int nz, float u0, float amp, uint64_t 
     inlet_profile(ny, nz, u0, amp, seed, out);
 }
 
+// Padded row length: whole 64-node wavefront segments.
+static int padded_nx(int nx, int) { return ((nx + 63) / 64) * 64; }
+
+// Kernels address a direction plane with 32-bit byte offsets.
+static bool plane_fits(int nx, int ny, int nz, int storage)
+{
+    return (long long)padded_nx(nx, storage) * ny * nz * elem_size(storage) <= (1LL << 32) - 64;
+}
+
 static int validate(const lbm_config* c, std::string& why)
 {
     if (c->nx < 2 || c->ny < 1 || c->nz < 1) { why = "domain must be at least 2x1x1"; return 0; }
@@ -199,6 +208,7 @@ static int validate(const lbm_config* c, std::string& why)
         if (c->bc[a] < 0 || c->bc[a] > 2) { why = "bc must be LBM_BC_PERIODIC/CLAMP/WALL"; return 0; }
     if (c->model < 0 || c->model > 3) { why = "unknown model"; return 0; }
     if (c->storage < 0 || c->storage > 3) { why = "unknown storage"; return 0; }
+    if (!plane_fits(c->nx, c->ny, c->nz, c->storage)) { why = "a padded direction plane exceeds 4 GiB"; return 0; }
     if (!(c->omega > 0.0f && c->omega < 2.0f)) { why = "omega must lie in (0, 2) (tau > 0.5)"; return 0; }
     if (c->cs < 0.0f) { why = "CS must be >= 0"; return 0; }
     if (c->cs > 0.0f && c->model == LBM_MODEL_MRT_CPU3D) { why = "MRT_CPU3D has no LES"; return 0; }
@@ -218,6 +228,7 @@ static int validate(const lbm_config* c, std::string& why)
             if (ai < 1.0f || ai + nf[a] / 2 > n[a] - 2) { why = "the refinement patch must lie inside the domain"; return 0; }
         }
         if (c->model == LBM_MODEL_MRT_CPU3D) { why = "refinement needs an LES-family or BGK model"; return 0; }
+        if (!plane_fits(nf[0], nf[1], nf[2], c->storage)) { why = "a padded fine-level plane exceeds 4 GiB"; return 0; }
     }
     return 1;
 }
@@ -225,8 +236,8 @@ static int validate(const lbm_config* c, std::string& why)
 static int alloc_level(lbm_ctx* ctx, Level& L, int nx, int ny, int nz, int storage)
 {
     L.nx = nx; L.ny = ny; L.nz = nz;
-    L.nxp = ((nx + 63) / 64) * 64;     // rows padded to whole wavefronts
-    L.nsegw = L.nxp / 64;              // wavefront segments per row
+    L.nxp = padded_nx(nx, storage);    // rows padded to whole wavefront segments
+    L.nsegw = L.nxp / 64;              // 64-node wavefront segments per row
     L.qs = (long long)L.nxp * ny * nz;
     L.cells = (size_t)nx * ny * nz;
     L.storage = storage;
@@ -242,9 +253,9 @@ static int alloc_level(lbm_ctx* ctx, Level& L, int nx, int ny, int nz, int stora
             L.shift.v[PopShift::slot(i)] = w * (1.f + 3.f * cu + 4.5f * cu * cu - 1.5f * U * U);
         }
     }
-    const int nw = nwave_of_storage(L.storage), npl = npl_of_storage(L.storage);
+    const int nw = nwave_of_storage(L.storage), ns = segs_of_storage(L.storage);
     const long long segs = (long long)L.nsegw * ny * nz;
-    L.blocks = (int)((segs + (long long)nw * npl - 1) / ((long long)nw * npl));
+    L.blocks = (int)((segs + (long long)nw * ns - 1) / ((long long)nw * ns));
     L.blocks1 = (int)((segs + nw - 1) / nw);
     const size_t lattice = (size_t)Q * (size_t)L.qs * L.esize;
     HIP_TRY(ctx, hipMalloc(&L.fa, lattice));
@@ -565,6 +576,7 @@ static StepParams step_params(lbm_ctx* ctx, const Level& L, float omega)
     p.s1 = c.s_dhum[1]; p.s2 = c.s_dhum[2]; p.s4 = c.s_dhum[4]; p.s10 = c.s_dhum[10]; p.s16 = c.s_dhum[16];
     p.omega_node = nullptr;
     p.sh = L.shift;
+    p.xcd_chunk = xcd_chunk_of(L.storage, L.nsegw);
     p.stat = L.stat;
     p.stat_mode = 0;
     p.stat_n = 0.f;

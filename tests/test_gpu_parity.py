@@ -261,6 +261,117 @@ def test_16bit_storage_error_vs_fp64(storage, tol):
     assert err < tol
 
 
+W19 = np.array([12] + [2] * 4 + [1] * 4 + [2] + [1] * 4 + [2] + [1] * 4, np.float32) / np.float32(36)
+CX19 = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1, 0, 1, 0, -1, 0, 0, 1, 0, -1, 0], np.float32)
+
+
+def storage_offsets(umax):
+    """Offsets the 16-bit formats subtract before rounding: the equilibrium
+    populations at rho = 1, u = (umax, 0, 0) (float32 arithmetic)."""
+    U = np.float32(umax)
+    cu = CX19 * U
+    return W19 * (np.float32(1) + np.float32(3) * cu + np.float32(4.5) * cu * cu - np.float32(1.5) * U * U)
+
+
+def on_16bit_grid(f, storage, sh):
+    """Round f - offset to FP16 / BF16 (nearest even) and add the offset
+    back; also the spacing of that grid at each value."""
+    d = (f - sh[:, None, None, None]).astype(np.float32)
+    if storage == "f16":
+        h = d.astype(np.float16)
+        q = h.astype(np.float32)
+        ulp = np.spacing(np.abs(h)).astype(np.float32)
+    else:
+        b = d.view(np.uint32).astype(np.uint64)
+        b = ((b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+        q = b.view(np.float32)
+        _, e = np.frexp(np.abs(q))
+        ulp = np.ldexp(np.float32(1), e - 8).astype(np.float32)
+    return q + sh[:, None, None, None], ulp
+
+
+def _bc_case(case):
+    """(config, mask) of a 16-bit parity case: odd and even widths, one and
+    several 128-node segments, every x boundary condition and node kind."""
+    c = L.default_config()
+    c.umax = 0.05
+    c.omega, c.cs, c.dpdy = 1.6, 0.01, 1e-5
+    c.model = L.MODEL_MRT_LES
+    if case == "periodic_odd":
+        c.nx, c.ny, c.nz = 37, 6, 5
+        mask = np.zeros((c.nz, c.ny, c.nx), np.uint8)
+    elif case == "walls_halfway_solid":
+        c.nx, c.ny, c.nz = 130, 7, 6
+        c.bc[0], c.bc[1], c.bc[2] = L.BC_WALL, L.BC_WALL, L.BC_WALL
+        c.halfway, c.force_steps = 1, 8
+        mask = np.zeros((c.nz, c.ny, c.nx), np.uint8)
+        mask[1:4, 2:5, 63:67] = L.NODE_SOLID_FORCE
+        mask[1:3, 1:3, 0:2] = L.NODE_SOLID
+    elif case == "inlet_outlet_node_bb":
+        c.nx, c.ny, c.nz = 69, 8, 6
+        c.bc[0], c.bc[1], c.bc[2] = L.BC_CLAMP, L.BC_PERIODIC, L.BC_CLAMP
+        c.model, c.force_steps = L.MODEL_BGK, 8
+        mask = np.zeros((c.nz, c.ny, c.nx), np.uint8)
+        mask[:, :, 0] = L.NODE_INLET_WEST
+        mask[:, :, -1] = L.NODE_OUTLET_EAST
+        mask[2:4, 3:6, 30:33] = L.NODE_SOLID_FORCE
+        mask[0, :, 1:68] = L.NODE_SOLID
+    else:  # symmetry planes
+        c.nx, c.ny, c.nz = 256, 6, 5
+        c.bc[0], c.bc[1], c.bc[2] = L.BC_CLAMP, L.BC_WALL, L.BC_WALL
+        c.halfway = 1
+        c.model = L.MODEL_MRT_DHUMIERES
+        mask = np.zeros((c.nz, c.ny, c.nx), np.uint8)
+        mask[:, :, 0] = L.NODE_SYM_XMIN
+        mask[:, :, -1] = L.NODE_SYM_XMAX
+    return c, mask
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("case", ["periodic_odd", "walls_halfway_solid", "inlet_outlet_node_bb", "symmetry"])
+def test_16bit_steps_match_oracle_on_the_16bit_grid(storage, case):
+    """Three single steps of 16-bit storage, each from the GPU's own previous
+    state (as stored: 16-bit value plus offset), against the FP32 oracle step
+    of that state rounded onto the same 16-bit grid.  The GPU's collide
+    differs from the oracle's by FP32 rounding only (fast reciprocal and
+    square root in the 16-bit kernels; the FP32 single-step bar is 2e-6 of
+    max|f|), so a value may round to a neighbouring grid point: allowed by
+    one grid spacing plus that FP32 bar (which matters where the deviation is
+    tiny and BF16's grid is fine), on under 2 % of the values.  A value
+    pulled from a wrong neighbour or boundary misses by ~1e-3."""
+    say("16-bit steps", storage, case)
+    c, mask = _bc_case(case)
+    c.storage = {"f16": 1, "bf16": 2}[storage]  # LBM_STORE_F16 / LBM_STORE_BF16
+    sh = storage_offsets(c.umax)
+    rng = np.random.default_rng(5)
+    f = random_state(rng, mask.shape, amp=1e-2)
+    omask = np.vectorize(NODE_TO_ORC.get)(mask).astype(np.uint8)
+    u_in = O.inlet_profile(c.ny, c.nz, c.umax, c.inlet_amp, c.seed)
+    solid = np.isin(mask, (L.NODE_SOLID, L.NODE_SOLID_FORCE))
+    keep = ~solid if c.halfway else np.ones_like(solid)
+    keep = np.broadcast_to(keep, (19,) + mask.shape)
+    with L.Solver(c) as s:
+        s.set_geometry(mask)
+        s.set_populations(f)
+        for t in range(3):
+            g0 = s.populations()
+            s.step(1)
+            g = s.populations()
+            fo, Fo = O.run(orc_params(c), g0, 1, mask=omask, u_in=u_in, forces=True)
+            q, ulp = on_16bit_grid(fo, storage, sh)
+            err = np.abs(g - q)[keep]
+            off = err > 0
+            tol32 = 2e-6 * np.abs(fo).max()
+            say(f"  step {t}: {off.mean():.2e} of values off the oracle's grid point, max {err.max():.2e} "
+                f"(bar: one spacing + {tol32:.1e})")
+            assert (err <= ulp[keep] * 1.0001 + tol32).all()
+            assert off.mean() < 0.02
+        if c.force_steps:
+            F = s.forces()
+            assert np.abs(Fo).max() > 0
+            assert np.abs(F[2] - Fo[0]).max() < 1e-5 * np.abs(Fo[0]).max()
+
+
 def test_bad_mask_code_rejected():
     c = L.default_config()
     c.nx, c.ny, c.nz = 8, 4, 4
