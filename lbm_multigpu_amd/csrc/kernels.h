@@ -95,7 +95,7 @@ __device__ __forceinline__ bool wave_node(int lb, int nsegw, int ny, int nz, int
 struct StepParams {
     const void* fin;
     void* fout;
-    const unsigned char* mask; // node codes, padded layout; nullptr = all fluid
+    const unsigned char* mask; // node codes, padded layout (every level has one; 0 = fluid)
     const float* u_in;         // west inlet profile [z*ny + y]
     double* fpart;             // momentum-exchange partials [blocks*3], or nullptr
     long long qs;              // elements per direction plane
@@ -290,15 +290,17 @@ __device__ __forceinline__ int node_fix(const StepParams& p, int x, int y, int z
     return ACT_COLLIDE;
 }
 
-// Pull stream + bounce-back + boundary conditions for one node.  A solid
-// under link bounce-back loads nothing.
+// Pull stream + bounce-back + boundary conditions for one node.  The 19
+// loads are issued together with the node code's, not after it: waiting for
+// the code first would put a second memory round trip on every wavefront's
+// path.  A solid under link bounce-back then loads values it never uses (in
+// bounds; solids are a small share of the nodes).
 template <class S>
 __device__ __forceinline__ int node_pull(const StepParams& p, int x, int y, int z, int cell,
                                          typename Arith<S>::R f[Q], typename Arith<S>::R& fx,
                                          typename Arith<S>::R& fy, typename Arith<S>::R& fz)
 {
-    const unsigned char code = p.mask ? *at(p.mask, cell) : (unsigned char)0;
-    if (p.halfway && is_solid(code & 0x7f)) return ACT_SKIP;
+    const unsigned char code = *at(p.mask, cell);
     node_load<S>(p, x, y, z, f);
     return node_fix<S>(p, x, y, z, cell, code, f, fx, fy, fz);
 }
