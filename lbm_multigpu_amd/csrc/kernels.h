@@ -321,7 +321,10 @@ __device__ __forceinline__ void node_store(const StepParams& p, int cell, const 
 {
     S* __restrict__ fout = static_cast<S*>(p.fout);
 #pragma unroll
-    for (int i = 0; i < Q; ++i) Store<S>::store(at(fout + i * p.qs, cell), f[i], i, p.sh);
+    for (int i = 0; i < Q; ++i) {
+        const __amdgpu_buffer_rsrc_t r = plane_rsrc(fout + i * p.qs, (uint32_t)(p.qs * (long long)sizeof(S)));
+        buffer_put(Store<S>::raw(f[i], i, p.sh), r, (uint32_t)cell * (uint32_t)sizeof(S));
+    }
 }
 
 // Running statistics of the post-collision velocity sum c_i f_i at one node:

@@ -40,6 +40,8 @@ __device__ __forceinline__ void st(T* p, T v)
     else *p = v;
 }
 
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
 struct Half16 { uint16_t bits; };
 struct BHalf16 { uint16_t bits; };
 
@@ -65,10 +67,34 @@ struct PopShift {
 template <class S>
 struct Store;
 
+// Buffer store of one stored word at byte offset off of a direction plane:
+// the plane's resource descriptor sits in scalar registers and every lane
+// supplies a 32-bit offset, so the 19 stores of a node share one offset
+// register (global stores there took a 64-bit per-lane address add each).
+// Descriptor word 3 is the raw-buffer format of gfx9-family targets.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(void* plane, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(plane, 0, (int)bytes, 0x00020000);
+}
+constexpr int kStoreAux = LBM_NT_STORE ? 2 : 0; // 2: nontemporal
+__device__ __forceinline__ void buffer_put(uint16_t w, __amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    __builtin_amdgcn_raw_buffer_store_b16(w, r, (int)off, 0, kStoreAux);
+}
+__device__ __forceinline__ void buffer_put(uint32_t w, __amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(w, r, (int)off, 0, kStoreAux);
+}
+__device__ __forceinline__ void buffer_put(u32x2 w, __amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(w, r, (int)off, 0, kStoreAux);
+}
+
 template <>
 struct Store<float> {
     __device__ static __forceinline__ float load(const float* p, int, const PopShift&) { return ld(p); }
     __device__ static __forceinline__ void store(float* p, float v, int, const PopShift&) { st(p, v); }
+    __device__ static __forceinline__ uint32_t raw(float v, int, const PopShift&) { return __builtin_bit_cast(uint32_t, v); }
     __host__ __device__ static float to_real(float v, int, const PopShift&) { return v; }
     __host__ __device__ static float from_real(float v, int, const PopShift&) { return v; }
 };
@@ -77,6 +103,7 @@ template <>
 struct Store<double> {
     __device__ static __forceinline__ double load(const double* p, int, const PopShift&) { return ld(p); }
     __device__ static __forceinline__ void store(double* p, double v, int, const PopShift&) { st(p, v); }
+    __device__ static __forceinline__ u32x2 raw(double v, int, const PopShift&) { return __builtin_bit_cast(u32x2, v); }
     __host__ __device__ static double to_real(double v, int, const PopShift&) { return v; }
     __host__ __device__ static double from_real(double v, int, const PopShift&) { return v; }
 };
@@ -127,11 +154,15 @@ struct Store<Half16> {
         asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(v) : "v"(h), "s"(sh[I]));
         return v;
     }
-    __device__ static __forceinline__ void store(Half16* p, float v, int I, const PopShift& sh)
+    __device__ static __forceinline__ uint16_t raw(float v, int I, const PopShift& sh)
     {
         unsigned o;
         asm("v_fma_mixlo_f16 %0, %1, 1.0, -%2" : "=v"(o) : "v"(v), "s"(sh[I]));
-        st(&p->bits, (uint16_t)o);
+        return (uint16_t)o;
+    }
+    __device__ static __forceinline__ void store(Half16* p, float v, int I, const PopShift& sh)
+    {
+        st(&p->bits, raw(v, I, sh));
     }
     __host__ __device__ static float to_real(Half16 v, int i, const PopShift& sh)
     {
@@ -149,10 +180,14 @@ struct Store<BHalf16> {
     {
         return bf16_bits_to_float(ld(&p->bits)) + sh[I];
     }
-    __device__ static __forceinline__ void store(BHalf16* p, float v, int I, const PopShift& sh)
+    __device__ static __forceinline__ uint16_t raw(float v, int I, const PopShift& sh)
     {
         // v_cvt_pk_bf16_f32: round to nearest even, as float_to_bf16_bits
-        st(&p->bits, __builtin_bit_cast(uint16_t, (__bf16)(v - sh[I])));
+        return __builtin_bit_cast(uint16_t, (__bf16)(v - sh[I]));
+    }
+    __device__ static __forceinline__ void store(BHalf16* p, float v, int I, const PopShift& sh)
+    {
+        st(&p->bits, raw(v, I, sh));
     }
     __host__ __device__ static float to_real(BHalf16 v, int i, const PopShift& sh)
     {
