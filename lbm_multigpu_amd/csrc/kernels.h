@@ -41,15 +41,7 @@ enum Bc : int { BC_PERIODIC = 0, BC_CLAMP = 1, BC_WALL = 2 };
 // consecutive row segments, one node per lane.  FP32 streams best with four
 // (one contiguous 1 KB stretch per plane and workgroup); the VALU-bound
 // 16-bit kernels run best with one (measured on the 512x128x128 channel:
-// 35.1 vs 32.2 GLUPS FP32, 46.2 vs 41.7 GLUPS FP16).  Each wavefront takes
-// SEGS row segments in turn (a loop, no software pipelining), so a
-// workgroup streams SEGS x NW consecutive segments.
-#ifndef LBM_SEGS
-#define LBM_SEGS 1 // row segments per wavefront of the FP32/FP64 stream-collide
-#endif
-#ifndef LBM_SEGS16
-#define LBM_SEGS16 1 // row segments per wavefront of the 16-bit stream-collide
-#endif
+// 35.1 vs 32.2 GLUPS FP32, 46.2 vs 41.7 GLUPS FP16).
 #ifndef LBM_MINW16
 #define LBM_MINW16 1 // minimum wavefronts per SIMD for the 16-bit stream-collide registers
 #endif
@@ -57,13 +49,11 @@ template <class S>
 struct Shape {
     static constexpr int NW = sizeof(S) == 2 ? LBM_NWAVE16 : LBM_NWAVE;
     static constexpr int THREADS = 64 * NW;
-    static constexpr int SEGS = sizeof(S) == 2 ? LBM_SEGS16 : LBM_SEGS;
     // minimum wavefronts per SIMD the compiler must fit the registers to
     static constexpr int MIN_WAVES = sizeof(S) == 2 ? LBM_MINW16 : LBM_MIN_BLOCKS;
 };
 __host__ inline bool is16_storage(int storage) { return storage == 1 || storage == 2; }
 __host__ inline int nwave_of_storage(int storage) { return is16_storage(storage) ? LBM_NWAVE16 : LBM_NWAVE; }
-__host__ inline int segs_of_storage(int storage) { return is16_storage(storage) ? LBM_SEGS16 : LBM_SEGS; }
 // consecutive stream-collide workgroups per XCD (logical_block) on a level
 // with nsegw 64-node segments per row: only rows longer than a workgroup's
 // stretch have workgroup boundaries inside them
@@ -422,21 +412,12 @@ template <int MODEL, class S>
 __global__ void __launch_bounds__(Shape<S>::THREADS, Shape<S>::MIN_WAVES) k_stream_collide(StepParams p)
 {
     using R = typename Arith<S>::R;
-    constexpr int NW = Shape<S>::NW, SEGS = Shape<S>::SEGS;
+    constexpr int NW = Shape<S>::NW;
     const int lb = logical_block(p.xcd_chunk);
     R fx = 0, fy = 0, fz = 0;
-    if constexpr (SEGS == 1) {
-        int x, y, z;
-        const bool row_ok = wave_node<NW>(lb, p.nsegw, p.ny, p.nz, x, y, z);
-        if (row_ok && x < p.nx) node_update<MODEL, S>(p, x, y, z, fx, fy, fz);
-    } else {
-#pragma unroll 1
-        for (int k = 0; k < SEGS; ++k) {
-            int x, y, z;
-            const bool row_ok = wave_node<NW>(lb * SEGS + k, p.nsegw, p.ny, p.nz, x, y, z);
-            if (row_ok && x < p.nx) node_update<MODEL, S>(p, x, y, z, fx, fy, fz);
-        }
-    }
+    int x, y, z;
+    const bool row_ok = wave_node<NW>(lb, p.nsegw, p.ny, p.nz, x, y, z);
+    if (row_ok && x < p.nx) node_update<MODEL, S>(p, x, y, z, fx, fy, fz);
     if (p.fpart) block_force_partial<NW>(p, lb, fx, fy, fz);
 }
 

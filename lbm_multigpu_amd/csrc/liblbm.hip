@@ -37,8 +37,7 @@ struct Level {
     std::vector<unsigned char> host_mask; // dense codes
     float* stat = nullptr;         // running statistics, 6 planes of qs
     PopShift shift{};              // 16-bit storage offsets (equilibrium of the inflow)
-    int blocks = 0;    // stream-collide workgroups (Shape::SEGS row segments per wavefront)
-    int blocks1 = 0;   // workgroups of the one-node-per-lane kernels (closure, mass sums)
+    int blocks = 0;    // workgroups of the one-node-per-lane kernels (Shape::NW row segments each)
     size_t bytes = 0;
 };
 
@@ -66,7 +65,9 @@ struct lbm_ctx {
     float* Mold = nullptr;     // footprint moments at the step's start
     float* Mnew = nullptr;     // and at its end
     int fc_nb[3] = {0, 0, 0};  // FC workgroups per axis (FC_TX x FC_TY x FC_TZ coarse nodes each)
-    float* Pcf = nullptr;      // footprint moments + strain rates of one CF pass (16 floats per node)
+    float* Pold = nullptr;     // CF data (moments + strain rates, 16 floats per node) at the step's start
+    float* Pnew = nullptr;     // at its end (the next step's Pold)
+    float* Phalf = nullptr;    // at its midpoint (temporal interpolation)
     // learned closure
     lbm_mlp* mlp = nullptr;    // not owned
     float* dwall = nullptr;    // wall distance per node (padded layout)
@@ -99,8 +100,8 @@ struct lbm_mlp {
 
 static int refine_setup(lbm_ctx* ctx);
 static void classify_fine(lbm_ctx* ctx, bool user_mask);
-static int refine_extract(lbm_ctx* ctx, float* M);
-static int refine_cf(lbm_ctx* ctx, float alpha);
+static int refine_extract(lbm_ctx* ctx, const float* Mold, float* M, float* Pnew, float* Phalf);
+static int refine_cf(lbm_ctx* ctx, const float* P);
 static int refine_fc(lbm_ctx* ctx);
 static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f);
 
@@ -253,10 +254,9 @@ static int alloc_level(lbm_ctx* ctx, Level& L, int nx, int ny, int nz, int stora
             L.shift.v[PopShift::slot(i)] = w * (1.f + 3.f * cu + 4.5f * cu * cu - 1.5f * U * U);
         }
     }
-    const int nw = nwave_of_storage(L.storage), ns = segs_of_storage(L.storage);
+    const int nw = nwave_of_storage(L.storage);
     const long long segs = (long long)L.nsegw * ny * nz;
-    L.blocks = (int)((segs + (long long)nw * ns - 1) / ((long long)nw * ns));
-    L.blocks1 = (int)((segs + nw - 1) / nw);
+    L.blocks = (int)((segs + nw - 1) / nw);
     const size_t lattice = (size_t)Q * (size_t)L.qs * L.esize;
     HIP_TRY(ctx, hipMalloc(&L.fa, lattice));
     HIP_TRY(ctx, hipMalloc(&L.fb, lattice));
@@ -318,8 +318,8 @@ extern "C" int lbm_create(const lbm_config* cfg, lbm_ctx** out)
     std::vector<float> prof((size_t)cfg->ny * cfg->nz);
     inlet_profile(cfg->ny, cfg->nz, cfg->umax, cfg->inlet_amp, cfg->seed, prof.data());
     if (hipMalloc(&ctx->u_in, prof.size() * sizeof(float)) != hipSuccess ||
-        hipMalloc(&ctx->fpart, (size_t)L.blocks1 * 3 * sizeof(double)) != hipSuccess ||
-        hipMalloc(&ctx->mpart, (size_t)L.blocks1 * 4 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ctx->fpart, (size_t)L.blocks * 3 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ctx->mpart, (size_t)L.blocks * 4 * sizeof(double)) != hipSuccess ||
         hipMalloc(&ctx->red, 8 * sizeof(double)) != hipSuccess)
         return bail(fail(nullptr, LBM_ENOMEM, "lbm_create: device allocation failed"));
     if (cfg->force_steps > 0) {
@@ -352,7 +352,8 @@ extern "C" void lbm_destroy(lbm_ctx* ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (int l = 0; l < 2; ++l) free_level(ctx->lv[l]);
     for (void* p : {(void*)ctx->u_in, (void*)ctx->fpart, (void*)ctx->fhist, (void*)ctx->red, (void*)ctx->mpart,
-                    (void*)ctx->Mold, (void*)ctx->Mnew, (void*)ctx->Pcf, (void*)ctx->dwall,
+                    (void*)ctx->Mold, (void*)ctx->Mnew, (void*)ctx->Pold, (void*)ctx->Pnew, (void*)ctx->Phalf,
+                    (void*)ctx->dwall,
                     (void*)ctx->omega_field})
         if (p) (void)hipFree(p);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -531,7 +532,7 @@ extern "C" int lbm_init_equilibrium(lbm_ctx* ctx, int kind, const float* rho, co
     if (ctx->nlevels > 1) {
         rc = init_level(ctx, 1, kind, nullptr, nullptr, nullptr, nullptr, ctx->cfg.umax);
         if (rc) return rc;
-        rc = refine_extract(ctx, ctx->Mold);
+        rc = refine_extract(ctx, nullptr, ctx->Mold, ctx->Pold, nullptr);
         if (rc) return rc;
     }
     ctx->t = 0;
@@ -627,17 +628,14 @@ static int coarse_step(lbm_ctx* ctx)
     StepParams p = step_params(ctx, L, c.omega);
     const bool rec = c.force_steps > 0 && ctx->t >= c.startf && (ctx->t - c.startf) < c.force_steps;
     if (rec) p.fpart = ctx->fpart;
-    int nparts = L.blocks; // force partials the launch writes
     with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         if (!ctx->mlp) {
             launch_sc<S>(c.model, p, L.blocks, ctx->stream);
         } else if (ctx->closure_every <= 1) {
-            launch_closure<S>(ctx, p, L.blocks1, nullptr);
-            nparts = L.blocks1;
+            launch_closure<S>(ctx, p, L.blocks, nullptr);
         } else if ((ctx->t - ctx->closure_t0) % ctx->closure_every == 0) {
-            launch_closure<S>(ctx, p, L.blocks1, ctx->omega_field); // refresh: network + rates for the next steps
-            nparts = L.blocks1;
+            launch_closure<S>(ctx, p, L.blocks, ctx->omega_field); // refresh: network + rates for the next steps
         } else {
             StepParams q = p;
             q.omega_node = ctx->omega_field;
@@ -648,7 +646,7 @@ static int coarse_step(lbm_ctx* ctx)
     if (rec) {
         const long long slot = ctx->t - c.startf;
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, ctx->stream, ctx->fpart,
-                           (long long)nparts, 3, ctx->fhist + slot * 3);
+                           (long long)L.blocks, 3, ctx->fhist + slot * 3); // one partial per workgroup
         ctx->force_recorded = (int)std::max<long long>(ctx->force_recorded, slot + 1);
     }
     std::swap(L.fa, L.fb);
@@ -695,7 +693,9 @@ static int refine_setup(lbm_ctx* ctx)
     const size_t NC = (size_t)g.nc[0] * g.nc[1] * g.nc[2];
     HIP_TRY(ctx, hipMalloc(&ctx->Mold, 9 * NC * sizeof(float)));
     HIP_TRY(ctx, hipMalloc(&ctx->Mnew, 9 * NC * sizeof(float)));
-    HIP_TRY(ctx, hipMalloc(&ctx->Pcf, 16 * NC * sizeof(float)));
+    HIP_TRY(ctx, hipMalloc(&ctx->Pold, 16 * NC * sizeof(float)));
+    HIP_TRY(ctx, hipMalloc(&ctx->Pnew, 16 * NC * sizeof(float)));
+    HIP_TRY(ctx, hipMalloc(&ctx->Phalf, 16 * NC * sizeof(float)));
     return LBM_OK;
 }
 
@@ -725,31 +725,29 @@ static void classify_fine(lbm_ctx* ctx, bool user_mask)
             }
 }
 
-static int refine_extract(lbm_ctx* ctx, float* M)
+static int refine_extract(lbm_ctx* ctx, const float* Mold, float* M, float* Pnew, float* Phalf)
 {
     Level& L = ctx->lv[0];
     const size_t NC = (size_t)ctx->rg.nc[0] * ctx->rg.nc[1] * ctx->rg.nc[2];
     with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         hipLaunchKernelGGL(k_extract<S>, dim3((unsigned)((NC + 255) / 256)), dim3(256), 0, ctx->stream,
-                           (const S*)L.fa, L.qs, L.nxp, L.ny, ctx->rg, M, L.shift);
+                           (const S*)L.fa, L.qs, L.nxp, L.ny, ctx->rg, Mold, M, (f32x4v*)Pnew, (f32x4v*)Phalf,
+                           L.shift);
         return 0;
     });
     HIP_TRY(ctx, hipGetLastError());
     return LBM_OK;
 }
 
-static int refine_cf(lbm_ctx* ctx, float alpha)
+static int refine_cf(lbm_ctx* ctx, const float* P)
 {
     Level& F = ctx->lv[1];
     with_storage(F.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        const long long NC = (long long)ctx->rg.nc[0] * ctx->rg.nc[1] * ctx->rg.nc[2];
-        hipLaunchKernelGGL(k_cf_prep, dim3((unsigned)((NC + 255) / 256)), dim3(256), 0, ctx->stream, ctx->Mold,
-                           ctx->Mnew, alpha, NC, (f32x4v*)ctx->Pcf);
         const long long n = frame_nodes(ctx->rg.nf);
         hipLaunchKernelGGL(k_interp_cf<S>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (S*)F.fa,
-                           F.qs, F.nxp, F.mask, (const f32x4v*)ctx->Pcf, ctx->rg, F.shift);
+                           F.qs, F.nxp, F.mask, (const f32x4v*)P, ctx->rg, F.shift);
         return 0;
     });
     HIP_TRY(ctx, hipGetLastError());
@@ -781,14 +779,16 @@ extern "C" int lbm_step(lbm_ctx* ctx, int nsteps)
         if (ctx->nlevels > 1) {
             // coarse t -> t+1 done; fine sub-steps t -> t+1/2 -> t+1 (main(),
             // MRTtest/DynInlet3.cu:3180-3245), then the fine -> coarse restriction
-            int rc = refine_extract(ctx, ctx->Mnew);
-            if (!rc) rc = refine_cf(ctx, 0.0f);
+            const bool temporal = ctx->cfg.temporal_interp;
+            int rc = refine_extract(ctx, ctx->Mold, ctx->Mnew, ctx->Pnew, temporal ? ctx->Phalf : nullptr);
+            if (!rc) rc = refine_cf(ctx, ctx->Pold);
             if (!rc) rc = lbm_internal_fine_step(ctx, ctx->rg.omega_f);
-            if (!rc && ctx->cfg.temporal_interp) rc = refine_cf(ctx, 0.5f);
+            if (!rc && temporal) rc = refine_cf(ctx, ctx->Phalf);
             if (!rc) rc = lbm_internal_fine_step(ctx, ctx->rg.omega_f);
             if (!rc) rc = refine_fc(ctx);
             if (rc) return rc;
             std::swap(ctx->Mold, ctx->Mnew);
+            std::swap(ctx->Pold, ctx->Pnew);
         }
         ctx->t += 1;
     }
@@ -889,7 +889,7 @@ static int set_pops(lbm_ctx* ctx, int level, const T* f, const char* who)
     if (level == 0) {
         ctx->initialised = true;
         if (ctx->nlevels > 1) {
-            int rc = refine_extract(ctx, ctx->Mold);
+            int rc = refine_extract(ctx, nullptr, ctx->Mold, ctx->Pold, nullptr);
             if (rc) return rc;
         }
     }
@@ -934,16 +934,16 @@ extern "C" int lbm_get_mass_momentum(lbm_ctx* ctx, int level, double out[4])
     double* part = ctx->mpart;
     double* tmp = nullptr;
     if (level > 0) {
-        HIP_TRY(ctx, hipMalloc(&tmp, (size_t)L.blocks1 * 4 * sizeof(double)));
+        HIP_TRY(ctx, hipMalloc(&tmp, (size_t)L.blocks * 4 * sizeof(double)));
         part = tmp;
     }
     with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        hipLaunchKernelGGL(k_mass_momentum<S>, dim3(L.blocks1), dim3(Shape<S>::THREADS), 0, ctx->stream, (const S*)L.fa,
+        hipLaunchKernelGGL(k_mass_momentum<S>, dim3(L.blocks), dim3(Shape<S>::THREADS), 0, ctx->stream, (const S*)L.fa,
                            L.mask, L.qs, L.nx, L.ny, L.nxp, L.nsegw, L.nz, part, L.shift);
         return 0;
     });
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, ctx->stream, part, (long long)L.blocks1, 4,
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, ctx->stream, part, (long long)L.blocks, 4,
                        ctx->red);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(out, ctx->red, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);

@@ -162,9 +162,29 @@ __device__ __forceinline__ void order2(const float (*Sc)[6], const float (*uc)[3
     mom[3] += ex * wx + ey * wy + ez * wz;
 }
 
-// Footprint moments of the coarse lattice (one thread per footprint node).
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+
+// Footprint node data for one CF pass: 9 moments and their 6 strain rates,
+// 16 floats per node (AoS, so a CF thread fetches a corner with four 16-byte
+// loads).
+__device__ __forceinline__ void cf_pack(const float m[9], f32x4v* P)
+{
+    float Sv[6];
+    strain_rate(m, Sv);
+    P[0] = f32x4v{m[0], m[1], m[2], m[3]};
+    P[1] = f32x4v{m[4], m[5], m[6], m[7]};
+    P[2] = f32x4v{m[8], Sv[0], Sv[1], Sv[2]};
+    P[3] = f32x4v{Sv[3], Sv[4], Sv[5], 0.f};
+}
+
+// Footprint moments of the coarse lattice at the step's end (one thread per
+// footprint node), and the CF data of the two fine sub-steps that follow:
+// Pnew packs these moments (the first CF pass of the next coarse step, whose
+// start they are), Phalf the midpoint (Mold + Mnew)/2 (the second CF pass of
+// this step, temporal interpolation; skipped when Phalf is null).
 template <class S>
-__global__ void __launch_bounds__(256) k_extract(const S* fc, long long qs, int nxp, int ny, RefineGeom g, float* M,
+__global__ void __launch_bounds__(256) k_extract(const S* fc, long long qs, int nxp, int ny, RefineGeom g,
+                                                 const float* Mold, float* M, f32x4v* Pnew, f32x4v* Phalf,
                                                  PopShift sh)
 {
     const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
@@ -180,6 +200,13 @@ __global__ void __launch_bounds__(256) k_extract(const S* fc, long long qs, int 
     phys_moments(f, mom);
 #pragma unroll
     for (int m = 0; m < 9; ++m) M[m * NC + o] = mom[m];
+    cf_pack(mom, Pnew + 4 * o);
+    if (Phalf) {
+        float mid[9];
+#pragma unroll
+        for (int m = 0; m < 9; ++m) mid[m] = (1.f - 0.5f) * Mold[m * NC + o] + 0.5f * mom[m];
+        cf_pack(mid, Phalf + 4 * o);
+    }
 }
 
 // Frame node number o of the CF frame (two fine nodes thick on all six faces)
@@ -224,32 +251,8 @@ __host__ __device__ inline long long frame_nodes(const int nf[3])
     return 4LL * nf[0] * nf[1] + 4LL * nf[0] * (nf[2] - 4) + 4LL * (nf[1] - 4) * (nf[2] - 4);
 }
 
-// Footprint node data for one CF pass: the 9 moments blended in time by
-// alpha and their 6 strain rates, 16 floats per node (AoS, so a CF thread
-// fetches a corner with four 16-byte loads).
-typedef float f32x4v __attribute__((ext_vector_type(4)));
-
-__global__ void __launch_bounds__(256) k_cf_prep(const float* Mold, const float* Mnew, float alpha, long long NC,
-                                                 f32x4v* P)
-{
-    const long long o = blockIdx.x * 256LL + threadIdx.x;
-    if (o >= NC) return;
-    float m[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) {
-        const float a0 = Mold[q * NC + o];
-        m[q] = alpha == 0.f ? a0 : (1.f - alpha) * a0 + alpha * Mnew[q * NC + o];
-    }
-    float Sv[6];
-    strain_rate(m, Sv);
-    P[4 * o + 0] = f32x4v{m[0], m[1], m[2], m[3]};
-    P[4 * o + 1] = f32x4v{m[4], m[5], m[6], m[7]};
-    P[4 * o + 2] = f32x4v{m[8], Sv[0], Sv[1], Sv[2]};
-    P[4 * o + 3] = f32x4v{Sv[3], Sv[4], Sv[5], 0.f};
-}
-
 // Coarse -> fine frame, one thread per frame node: the 8 surrounding
-// footprint nodes' blended moments and strain rates (k_cf_prep, L2-resident
+// footprint nodes' blended moments and strain rates (k_extract, L2-resident
 // and shared by neighbouring threads), then trilinear interpolation, ORDER-2
 // correction and the LES-aware rescale.
 template <class S>

@@ -71,6 +71,41 @@ def test_refined_channel_matches_oracle(temporal, interface_eq):
         assert rel_vel([g[sel] for g in got[1:]], [x[sel] for x in ref[1:]]) < 1e-5
 
 
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-5), ("f16", 2e-3), ("bf16", 2e-2)])
+def test_refined_channel_other_storage_formats(storage, tol):
+    """Both levels, the interface kernels and the 16-bit offsets with FP64,
+    FP16 and BF16 populations: 10 coarse steps against the FP32 oracle from
+    the same (stored) initial state.  FP64 differs from it by FP32 rounding
+    only; the 16-bit bars are those of the uniform-grid 16-bit tests."""
+    say("refine storage", storage)
+    cc = small_case()
+    cc.storage = storage
+    c = cc.to_c()
+    T = 10
+    with L.Solver(c) as s:
+        s.set_geometry(None)
+        cm, fm = to_orc(s.get_mask(0)), to_orc(s.get_mask(1))
+        s.init_equilibrium(L.INIT_MRT)
+        c0, f0 = s.populations(0), s.populations(1)
+        s.step(T)
+        cg, fg = s.moments(0), s.moments(1)
+    pc = orc_params(c)
+    r = O.refine([int(round(x - 0.25)) for x in cc.lr_loc], cc.lr_size, float(np.float32(c.omega)),
+                 cs=float(np.float32(c.cs)), temporal=1, eq_kind=O.EQ_BGK_MEQ)
+    pf = O.params(*cc.lr_size, bc=(O.CLAMP,) * 3, collision=pc.collision, halfway=1, omega=r.omega_f,
+                  cs=pc.cs, dpdy=pc.dpdy)
+    u_in = O.inlet_profile(c.ny, c.nz, c.umax, c.inlet_amp, c.seed)
+    ca, fa = O.refine_run(pc, pf, r, c0, f0, T, cmask=cm, fmask=fm, u_in=u_in)
+    co = O.macro(pc, ca)
+    fo = O.macro(O.params(*cc.lr_size), fa)
+    inner = np.zeros_like(fm, dtype=bool)
+    inner[2:-2, 2:-2, 2:-2] = True
+    for name, got, ref, sel in (("coarse", cg, co, cm == O.FLUID), ("fine", fg, fo, (fm == O.FLUID) & inner)):
+        err = rel_vel([g[sel] for g in got[1:]], [x[sel] for x in ref[1:]])
+        say(f"  {name}: velocity error vs the FP32 oracle {err:.2e} (bar {tol})")
+        assert err < tol
+
+
 def test_fine_mask_is_geometry_at_fine_nodes():
     cc = small_case()
     c = cc.to_c()
