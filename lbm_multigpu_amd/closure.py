@@ -151,12 +151,19 @@ def l2_errors(u, u_ref, region, u0):
             "wake_l2": float(np.linalg.norm(d) / np.linalg.norm((u_ref - u0)[region]))}
 
 
-def run(scale=1.0, spinup_fine=16000, snapshots=200, every=100, keep=0.02, train_steps=3000,
-        resolutions=(0.625, 0.75), seed=0, refresh=(1, 10), log=print):
-    """The full experiment at `scale` times the 512x128x128 fine grid."""
+def run(scale=1.0, spinup_fine=16000, snapshots=400, every=100, keep=0.02, train_steps=3000,
+        resolutions=(0.5625, 0.625, 0.6875, 0.75), seed=0, refresh=(1, 10), tau_fine=0.52, log=print):
+    """The full experiment at `scale` times the 512x128x128 fine grid.
+
+    Coarse runs (half the fine resolution): the learned closure at each
+    network interval in `refresh`, Smagorinsky Cs = 0.1 (the solver's
+    constant), Smagorinsky with the a-priori best-fit constant, and no
+    sub-grid model; then Smagorinsky Cs = 0.1 at each of `resolutions`
+    (fractions of the fine grid), the curve against which the learned run's
+    error is priced in GPU time."""
     t0 = time.time()
     base = tuple(int(round(b * scale)) for b in (512, 128, 128))
-    fine = channel_at(1.0, base, seed=seed)
+    fine = channel_at(1.0, base, tau_fine=tau_fine, seed=seed)
     u0 = fine.umax
     fine_avg = snapshots * every
     log(f"fine run {base} tau {1 / fine.omega:.4f}: spin-up {spinup_fine}, {snapshots} snapshots every {every}")
@@ -179,20 +186,23 @@ def run(scale=1.0, spinup_fine=16000, snapshots=200, every=100, keep=0.02, train
                "corr_smag": float(np.corrcoef(smagorinsky_nu(X[te], 1.0), y[te])[0, 1])}
     log(f"  a-priori test MSE: mlp {apriori['mse_mlp']:.3e}, best-fit Smagorinsky {apriori['mse_smag_bestfit']:.3e}")
     u_ref = at_coarse_centres(fout["mean"][0], 1.0, 0.5)
-    coarse = channel_at(0.5, base, seed=seed)
+    coarse = channel_at(0.5, base, tau_fine=tau_fine, seed=seed)
     # the same physical time as the fine run: half the steps
     sp_c, av_c = spinup_fine // 2, fine_avg // 2
     runs = {}
-    cases = [("coarse_learned" if k == 1 else f"coarse_learned_refresh{k}", 0.5, k) for k in refresh]
-    cases += [("coarse_smagorinsky", 0.5, 0)] + [(f"smagorinsky_r{r}", r, 0) for r in resolutions]
-    for name, r, k in cases:
-        cfg = channel_at(r, base, seed=seed)
+    cs_fit = math.sqrt(max(c_fit, 0.0))
+    cases = [("coarse_learned" if k == 1 else f"coarse_learned_refresh{k}", 0.5, k, 0.1) for k in refresh]
+    cases += [("coarse_smagorinsky", 0.5, 0, 0.1), ("coarse_smagorinsky_bestfit", 0.5, 0, cs_fit),
+              ("coarse_no_model", 0.5, 0, 0.0)]
+    cases += [(f"smagorinsky_r{r}", r, 0, 0.1) for r in resolutions]
+    for name, r, k, cs in cases:
+        cfg = channel_at(r, base, tau_fine=tau_fine, cs_smag=cs, seed=seed)
         sp, av = int(round(spinup_fine * r)), int(round(fine_avg * r))
         mean, mask, ms, ms_step = mean_velocity(cfg, sp, av, mlp if k else None, max(k, 1))
         u_c = at_coarse_centres(mean[0], r, 0.5) if r != 0.5 else mean[0].astype(np.float64)
         if r == 0.5:
             region = eval_region(mask)
-        runs[name] = {"grid": [cfg.nx, cfg.ny, cfg.nz], "tau": 1 / cfg.omega, "steps": sp + av,
+        runs[name] = {"grid": [cfg.nx, cfg.ny, cfg.nz], "tau": 1 / cfg.omega, "steps": sp + av, "cs": cs,
                       "gpu_ms": ms, "ms_per_step": ms_step, "_u": u_c}
         log(f"  {name}: grid {cfg.nx}x{cfg.ny}x{cfg.nz}, {sp + av} steps, {ms / 1e3:.2f} s GPU")
     assert sp_c + av_c == runs["coarse_smagorinsky"]["steps"]
@@ -206,15 +216,17 @@ def run(scale=1.0, spinup_fine=16000, snapshots=200, every=100, keep=0.02, train
            "norm": norm, "runs": runs, "wall_s": time.time() - t0, "eval_cells": int(region.sum())}
     # GPU time Smagorinsky needs for the learned run's error (log-log
     # interpolation of its error against its GPU time)
-    pts = sorted((v["gpu_ms"], v["wake_l2"]) for k, v in runs.items() if k.startswith("smag") or k == "coarse_smagorinsky")
+    pts = sorted((v["gpu_ms"], v["wake_l2"]) for k, v in runs.items()
+                 if k.startswith("smagorinsky_r") or k == "coarse_smagorinsky")
     target = runs[cases[0][0]]["wake_l2"]
     tms = [p[0] for p in pts]
     errs = [p[1] for p in pts]
     res["smagorinsky_ms_at_learned_error"] = None
-    for (t1, e1), (t2, e2) in zip(pts, pts[1:]):
+    for (t1, e1), (t2, e2) in zip(pts, pts[1:]):  # the cheapest crossing
         if (e1 - target) * (e2 - target) <= 0 and e1 != e2:
             a = (math.log(target) - math.log(e1)) / (math.log(e2) - math.log(e1))
             res["smagorinsky_ms_at_learned_error"] = math.exp(math.log(t1) + a * (math.log(t2) - math.log(t1)))
+            break
     res["smagorinsky_curve"] = {"gpu_ms": tms, "wake_l2": errs}
     return res
 
@@ -223,16 +235,17 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--scale", type=float, default=1.0, help="grid scale (1: the 512x128x128 fine run)")
     ap.add_argument("--spinup", type=int, default=16000, help="fine-run steps before sampling")
-    ap.add_argument("--snapshots", type=int, default=200)
+    ap.add_argument("--snapshots", type=int, default=400)
     ap.add_argument("--every", type=int, default=100)
     ap.add_argument("--keep", type=float, default=0.02)
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--refresh", type=int, nargs="+", default=[1, 10],
                     help="network evaluation intervals of the learned coarse runs")
+    ap.add_argument("--tau", type=float, default=0.52, help="relaxation time of the fine run")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     res = run(a.scale, a.spinup, a.snapshots, a.every, a.keep, a.train_steps, refresh=tuple(a.refresh),
-              log=lambda *m: print(*m, flush=True))
+              tau_fine=a.tau, log=lambda *m: print(*m, flush=True))
     txt = json.dumps(res, indent=1)
     if a.out:
         with open(a.out, "w") as fh:
