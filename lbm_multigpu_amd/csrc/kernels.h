@@ -105,6 +105,7 @@ struct StepParams {
     const float* omega_node;
     PopShift sh;   // 16-bit storage offsets
     int xcd_chunk; // consecutive workgroups per XCD (logical_block)
+    int nt_store;  // nontemporal population stores (lattices larger than the MALL)
 };
 
 // XCD-aware renumbering.  The hardware id hw goes to XCD hw % 8.
@@ -306,15 +307,26 @@ __device__ __forceinline__ CollideParams<R> collide_params(const StepParams& p)
     return cp;
 }
 
+// Store one node's 19 populations: buffer stores from per-plane resource
+// descriptors, so the 19 stores share one 32-bit offset register (global
+// stores took a 64-bit per-lane address add each); out-of-range offsets are
+// dropped by the hardware.  AUX is the cache policy (kStoreNT or 0).
+template <class S, int AUX>
+__device__ __forceinline__ void node_store_policy(const StepParams& p, int cell, const typename Arith<S>::R f[Q])
+{
+    S* __restrict__ fout = static_cast<S*>(p.fout);
+    const uint32_t bytes = (uint32_t)(p.qs * (long long)sizeof(S)), off = (uint32_t)cell * (uint32_t)sizeof(S);
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+        buffer_put<AUX>(Store<S>::raw(f[i], i, p.sh), plane_rsrc(fout + i * p.qs, bytes), off);
+}
+
+// The same with the level's policy chosen at run time (p.nt_store).
 template <class S>
 __device__ __forceinline__ void node_store(const StepParams& p, int cell, const typename Arith<S>::R f[Q])
 {
-    S* __restrict__ fout = static_cast<S*>(p.fout);
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-        const __amdgpu_buffer_rsrc_t r = plane_rsrc(fout + i * p.qs, (uint32_t)(p.qs * (long long)sizeof(S)));
-        buffer_put(Store<S>::raw(f[i], i, p.sh), r, (uint32_t)cell * (uint32_t)sizeof(S));
-    }
+    if (p.nt_store) node_store_policy<S, kStoreNT>(p, cell, f);
+    else node_store_policy<S, 0>(p, cell, f);
 }
 
 // Running statistics of the post-collision velocity sum c_i f_i at one node:
@@ -364,16 +376,16 @@ __device__ __forceinline__ void node_collide(const StepParams& p, int cell, int 
 }
 
 // Collide (per the node's action) and store one pulled node.
-template <int MODEL, class S>
+template <int MODEL, class S, int AUX>
 __device__ __forceinline__ void node_finish(const StepParams& p, int cell, int act, typename Arith<S>::R f[Q])
 {
     if (act == ACT_SKIP) return;
     node_collide<MODEL, S>(p, cell, act, f);
-    node_store<S>(p, cell, f);
+    node_store_policy<S, AUX>(p, cell, f);
 }
 
 // Fused pull stream + boundary dispatch + collide + bounce-back for one node.
-template <int MODEL, class S>
+template <int MODEL, class S, int AUX>
 __device__ __forceinline__ void node_update(const StepParams& p, int x, int y, int z,
                                             typename Arith<S>::R& fx, typename Arith<S>::R& fy,
                                             typename Arith<S>::R& fz)
@@ -382,7 +394,7 @@ __device__ __forceinline__ void node_update(const StepParams& p, int x, int y, i
     const int cell = x + p.nxp * (y + p.ny * z);
     R f[Q];
     const int act = node_pull<S>(p, x, y, z, cell, f, fx, fy, fz);
-    node_finish<MODEL, S>(p, cell, act, f);
+    node_finish<MODEL, S, AUX>(p, cell, act, f);
 }
 
 // Momentum-exchange partial of one workgroup: wavefront sums, then the
@@ -408,7 +420,9 @@ __device__ __forceinline__ void block_force_partial(const StepParams& p, int lb,
     }
 }
 
-template <int MODEL, class S>
+// The stream-collide step; NT selects nontemporal population stores (the
+// level's p.nt_store, as a template argument so each variant is straight-line).
+template <int MODEL, class S, bool NT>
 __global__ void __launch_bounds__(Shape<S>::THREADS, Shape<S>::MIN_WAVES) k_stream_collide(StepParams p)
 {
     using R = typename Arith<S>::R;
@@ -417,7 +431,7 @@ __global__ void __launch_bounds__(Shape<S>::THREADS, Shape<S>::MIN_WAVES) k_stre
     R fx = 0, fy = 0, fz = 0;
     int x, y, z;
     const bool row_ok = wave_node<NW>(lb, p.nsegw, p.ny, p.nz, x, y, z);
-    if (row_ok && x < p.nx) node_update<MODEL, S>(p, x, y, z, fx, fy, fz);
+    if (row_ok && x < p.nx) node_update<MODEL, S, NT ? kStoreNT : 0>(p, x, y, z, fx, fy, fz);
     if (p.fpart) block_force_partial<NW>(p, lb, fx, fy, fz);
 }
 

@@ -76,18 +76,24 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(void* plane, uint32
 {
     return __builtin_amdgcn_make_buffer_rsrc(plane, 0, (int)bytes, 0x00020000);
 }
-constexpr int kStoreAux = LBM_NT_STORE ? 2 : 0; // 2: nontemporal
+// Cache policy of the stream-collide's stores: nontemporal (aux 2) when the
+// two lattices exceed the 256 MB last-level cache (MALL), normal (aux 0)
+// when they fit and the next step's loads can hit it.
+constexpr int kStoreNT = 2;
+template <int AUX>
 __device__ __forceinline__ void buffer_put(uint16_t w, __amdgpu_buffer_rsrc_t r, uint32_t off)
 {
-    __builtin_amdgcn_raw_buffer_store_b16(w, r, (int)off, 0, kStoreAux);
+    __builtin_amdgcn_raw_buffer_store_b16(w, r, (int)off, 0, AUX);
 }
+template <int AUX>
 __device__ __forceinline__ void buffer_put(uint32_t w, __amdgpu_buffer_rsrc_t r, uint32_t off)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(w, r, (int)off, 0, kStoreAux);
+    __builtin_amdgcn_raw_buffer_store_b32(w, r, (int)off, 0, AUX);
 }
+template <int AUX>
 __device__ __forceinline__ void buffer_put(u32x2 w, __amdgpu_buffer_rsrc_t r, uint32_t off)
 {
-    __builtin_amdgcn_raw_buffer_store_b64(w, r, (int)off, 0, kStoreAux);
+    __builtin_amdgcn_raw_buffer_store_b64(w, r, (int)off, 0, AUX);
 }
 
 template <>
