@@ -41,9 +41,11 @@ enum Bc : int { BC_PERIODIC = 0, BC_CLAMP = 1, BC_WALL = 2 };
 // consecutive row segments, one node per lane.  FP32 streams best with four
 // (one contiguous 1 KB stretch per plane and workgroup); the VALU-bound
 // 16-bit kernels run best with one (measured on the 512x128x128 channel:
-// 35.1 vs 32.2 GLUPS FP32, 46.2 vs 41.7 GLUPS FP16).
+// 35.1 vs 32.2 GLUPS FP32, 46.2 vs 41.7 GLUPS FP16).  The 16-bit kernels
+// fit 64 VGPRs without spilling, so they are held to 8 wavefronts per SIMD
+// (56.3 vs 54.4 GLUPS at 7); the FP32 kernel at 8 would spill.
 #ifndef LBM_MINW16
-#define LBM_MINW16 1 // minimum wavefronts per SIMD for the 16-bit stream-collide registers
+#define LBM_MINW16 8 // minimum wavefronts per SIMD for the 16-bit stream-collide registers
 #endif
 template <class S>
 struct Shape {
