@@ -33,8 +33,8 @@ enum Bc : int { BC_PERIODIC = 0, BC_CLAMP = 1, BC_WALL = 2 };
 #ifndef LBM_XCD_CHUNK16
 #define LBM_XCD_CHUNK16 7 // the same for the 16-bit formats
 #endif
-#ifndef LBM_MIN_BLOCKS
-#define LBM_MIN_BLOCKS 1 // __launch_bounds__ minimum workgroups per CU
+#ifndef LBM_MINW32
+#define LBM_MINW32 1 // minimum wavefronts per SIMD for the FP32/FP64 stream-collide registers
 #endif
 
 // Workgroup shape per population type: NW wavefronts of 64 lanes on NW
@@ -52,7 +52,7 @@ struct Shape {
     static constexpr int NW = sizeof(S) == 2 ? LBM_NWAVE16 : LBM_NWAVE;
     static constexpr int THREADS = 64 * NW;
     // minimum wavefronts per SIMD the compiler must fit the registers to
-    static constexpr int MIN_WAVES = sizeof(S) == 2 ? LBM_MINW16 : LBM_MIN_BLOCKS;
+    static constexpr int MIN_WAVES = sizeof(S) == 2 ? LBM_MINW16 : (sizeof(S) == 4 ? LBM_MINW32 : 1);
 };
 __host__ inline bool is16_storage(int storage) { return storage == 1 || storage == 2; }
 __host__ inline int nwave_of_storage(int storage) { return is16_storage(storage) ? LBM_NWAVE16 : LBM_NWAVE; }
