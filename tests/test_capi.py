@@ -49,6 +49,20 @@ def test_create_rejects_bad_config(field, value, msg):
     assert e.value.code == -22 and msg in str(e.value)
 
 
+def test_create_rejects_planes_beyond_32bit_offsets():
+    """Kernels address a direction plane with 32-bit byte offsets, so the
+    limit follows the padded plane: 2 x 2^14 x 2^14 is 2^29 nodes (within the
+    node limit) but 2^34 padded ones (x padded from 2 to 64), over 4 GiB per
+    plane in FP64 and in FP16 alike."""
+    c = L.default_config()
+    c.nx, c.ny, c.nz = 2, 1 << 14, 1 << 14
+    for st in (L.STORE_F64, L.STORE_F16):
+        c.storage = st
+        with pytest.raises(L.LBMError) as e:
+            L.Solver(c)
+        assert e.value.code == -22 and "4 GiB" in str(e.value)
+
+
 def test_cpu3d_model_rejects_les():
     c = L.default_config()
     c.model = L.MODEL_MRT_CPU3D
