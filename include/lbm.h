@@ -113,6 +113,17 @@ typedef struct lbm_config {
                               InterpFC rebuild: 0 the collide's own (default);
                               1 mrt_meq as InvertPhysicalMoments codes it
                               (MRTtest/DynInlet3.cu:437-481) */
+    /* dynamic-inlet precursor (MRTtest/DynInlet3.cu:1744, :2561-2600,
+       :3147-3170): the body force acts only on nodes with y <= dyn_y1
+       ("if(y>DYNY1) dpdy = 0.f"); after every coarse step t with
+       t % ctrl_every == 0 and t > ctrl_start, the mean v over the plane
+       y = dyn_y1 (AverageV: sum over all x and z, divided by (nx-2) nz)
+       drives the P-controller dpdy += (Av_V - UMAX)/UMAX * kp * |DPDY|. */
+    int dyn_y1;            /* DYNY1; -1: the force acts everywhere and the
+                              plane is y = ny - 1 */
+    float kp;              /* "KP" (0: the plane mean is recorded, dpdy held) */
+    int ctrl_every;        /* 100 as the reference; 0 switches the control off */
+    int ctrl_start;        /* 10000 as the reference */
 } lbm_config;
 
 typedef struct lbm_sizes {
@@ -182,11 +193,30 @@ int lbm_get_forces(lbm_ctx* ctx, double* fxyz, int max_steps, int* nsteps);
  * fixed order (deterministic).  No reference counterpart. */
 int lbm_get_mass_momentum(lbm_ctx* ctx, int level, double out[4]);
 
-/* Tecplot ASCII output as WriteResults (MRTtest/DynInlet3.cu:2601-2694):
- * writes <casename>.dat (POINT zone, X,Y,Z,u,v,w,rho). */
+/* Tecplot ASCII output as WriteResults / WriteResultsLR
+ * (MRTtest/DynInlet3.cu:2601-2787): <casename>.dat (coarse level, POINT zone,
+ * X,Y,Z,u,v,w,rho, plus velAv[0..2], ufluc, vfluc, wfluc, Smag with running
+ * statistics), <casename>_LR.dat (fine level in coarse units; with
+ * statistics velAv[0..1], ufluc, vfluc, Smag as WriteResultsLR) and
+ * <casename>_slice.dat (coarse plane z = 1, fine plane k = 3). */
 int lbm_write_tecplot(lbm_ctx* ctx, const char* casename);
 
 int lbm_query_sizes(lbm_ctx* ctx, int level, lbm_sizes* out);
+
+/* History of the dynamic-inlet control: one row (t, Av_V, dpdy after the
+ * update) per control event, oldest first; the reference writes the rows
+ * with t % 1000 == 0 to CASENAME.vel (MRTtest/DynInlet3.cu:3170).  With
+ * rows == NULL only *nrows is set. */
+int lbm_get_inlet_control(lbm_ctx* ctx, double* rows, int max_rows, int* nrows);
+
+/* AverageV (MRTtest/DynInlet3.cu:2561-2600) on the current coarse lattice:
+ * the sum of v = sum_i c_iy f_i over the plane y (all x, all z) in FP64, and
+ * the reference's mean, that sum / ((nx - 2) nz).  No reference
+ * counterpart as an entry point. */
+int lbm_plane_mean_v(lbm_ctx* ctx, int y, double* sum, double* mean);
+
+/* Body force currently applied (DPDY as the P-controller left it). */
+int lbm_get_dpdy(lbm_ctx* ctx, float* dpdy);
 
 /* Steps taken so far (the reference's t). */
 int lbm_get_time(lbm_ctx* ctx, long long* t);

@@ -48,7 +48,8 @@ class lbm_config(C.Structure):
                 ("obst_z", C.c_float), ("obst_r", C.c_float), ("startf", C.c_int),
                 ("force_steps", C.c_int), ("refinement", C.c_int), ("lr_nx", C.c_int),
                 ("lr_ny", C.c_int), ("lr_nz", C.c_int), ("lr_x0", C.c_float), ("lr_y0", C.c_float),
-                ("lr_z0", C.c_float), ("temporal_interp", C.c_int), ("interface_eq", C.c_int)]
+                ("lr_z0", C.c_float), ("temporal_interp", C.c_int), ("interface_eq", C.c_int),
+                ("dyn_y1", C.c_int), ("kp", C.c_float), ("ctrl_every", C.c_int), ("ctrl_start", C.c_int)]
 
 
 class lbm_sizes(C.Structure):
@@ -67,6 +68,7 @@ EXPORTS = [
     "lbm_mlp_set_params", "lbm_mlp_get_params", "lbm_mlp_set_norm", "lbm_mlp_set_data",
     "lbm_mlp_gradient", "lbm_mlp_train", "lbm_mlp_forward", "lbm_set_closure",
     "lbm_closure_samples", "lbm_stats_enable", "lbm_stats_get", "lbm_set_closure_interval",
+    "lbm_get_inlet_control", "lbm_plane_mean_v", "lbm_get_dpdy",
 ]
 
 # closure network: 7 -> 64 -> 64 -> 64 -> 1, flat FP32 parameter vector
@@ -116,6 +118,9 @@ def load_library(path=LIB_PATH):
     lib.lbm_inlet_profile.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint64, f32]
     lib.lbm_inlet_profile.restype = None
     lib.lbm_stats_enable.argtypes = [P, C.c_longlong, C.c_longlong]
+    lib.lbm_get_inlet_control.argtypes = [P, P, C.c_int, C.POINTER(C.c_int)]
+    lib.lbm_plane_mean_v.argtypes = [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.lbm_get_dpdy.argtypes = [P, C.POINTER(C.c_float)]
     lib.lbm_stats_get.argtypes = [P, C.c_int, P, P]
     lib.lbm_mlp_create.argtypes = [C.c_uint64, C.POINTER(P)]
     lib.lbm_mlp_destroy.argtypes = [P]
@@ -254,6 +259,27 @@ class Solver:
         if n.value:
             self._check(self.lib.lbm_get_forces(self.h, out.ctypes.data, n.value, C.byref(n)))
         return out
+
+    def inlet_control(self):
+        """Dynamic-inlet control history: rows (t, Av_V, dpdy after the update)."""
+        n = C.c_int()
+        self._check(self.lib.lbm_get_inlet_control(self.h, None, 0, C.byref(n)))
+        out = np.zeros((max(n.value, 0), 3), np.float64)
+        if n.value:
+            self._check(self.lib.lbm_get_inlet_control(self.h, out.ctypes.data, n.value, C.byref(n)))
+        return out
+
+    def plane_mean_v(self, y):
+        """AverageV on the coarse lattice: (FP64 sum of v over plane y, the
+        reference's mean sum / ((nx - 2) nz))."""
+        sm, mean = C.c_double(), C.c_double()
+        self._check(self.lib.lbm_plane_mean_v(self.h, int(y), C.byref(sm), C.byref(mean)))
+        return sm.value, mean.value
+
+    def dpdy(self):
+        v = C.c_float()
+        self._check(self.lib.lbm_get_dpdy(self.h, C.byref(v)))
+        return v.value
 
     def mass_momentum(self, level=0):
         out = np.zeros(4, np.float64)

@@ -24,7 +24,7 @@ REFERENCE_KEYS = [
 EXTENSION_KEYS = [
     "MRT rates:", "Boundary x:", "Boundary y:", "Boundary z:", "Halfway bounce-back:", "Storage:",
     "Geometry:", "Inlet perturbation:", "Seed:", "Temporal interpolation:", "Force steps:",
-    "Statistics:",
+    "Statistics:", "DYNY1:", "Inlet control:",
 ]
 
 _BC = {"periodic": 0, "clamp": 1, "wall": 2}
@@ -70,6 +70,11 @@ class Config:
     temporal_interp: bool = True
     interface_eq: int = 0       # 0: the collide's own equilibrium; 1: mrt_meq as coded
     gpus: int = 1
+    # dynamic-inlet precursor (MRTtest/DynInlet3.cu:1744, :2561-2600, :3147-3170)
+    dyn_y1: int = -1            # DYNY1: rows past it get no body force; -1: none cut
+    kp: float = 0.0             # "KP": P-control gain on dpdy
+    ctrl_every: int = 100       # control interval (steps); 0: off
+    ctrl_start: int = 10000     # control from steps t > ctrl_start
 
     def model_code(self):
         if self.model.upper() == "BGK":
@@ -103,6 +108,8 @@ class Config:
         c.lr_x0, c.lr_y0, c.lr_z0 = self.lr_loc
         c.temporal_interp = int(self.temporal_interp)
         c.interface_eq = int(self.interface_eq)
+        c.dyn_y1, c.kp = int(self.dyn_y1), float(self.kp)
+        c.ctrl_every, c.ctrl_start = int(self.ctrl_every), int(self.ctrl_start)
         return c
 
 
@@ -176,7 +183,8 @@ def read_inputs(path_or_text):
                            ("TMAX:", "tmax", int), ("STARTF:", "startf", int),
                            ("START_VELAV:", "start_velav", int), ("START_VELFLUC:", "start_velfluc", int),
                            ("CS:", "cs", float), ("omega in LR", "omega_lr", float),
-                           ("LR factor", "lr_factor", float), ("GPUs per node", "gpus", int)):
+                           ("LR factor", "lr_factor", float), ("GPUs per node", "gpus", int),
+                           ("KP:", "kp", float), ("DYNY1:", "dyn_y1", int)):
         if key in seen:
             setattr(cfg, attr, typ(float(seen[key])) if typ is int else typ(seen[key]))
     if "REFINEMENT:" in seen:
@@ -208,6 +216,9 @@ def read_inputs(path_or_text):
         if k in seen:
             bc[a] = seen[k].strip()
     cfg.bc = tuple(bc)
+    if "Inlet control:" in seen:  # "<every>x<start>"
+        every, start = (int(float(a)) for a in seen["Inlet control:"].split("x"))
+        cfg.ctrl_every, cfg.ctrl_start = every, start
     return cfg
 
 
@@ -260,6 +271,9 @@ def write_inputs(cfg, path=None):
         f"Temporal interpolation: \t{int(cfg.temporal_interp)}",
         f"Force steps: \t{cfg.force_steps}",
         f"Statistics: \t{int(cfg.velav)}",
+        f"DYNY1: \t{cfg.dyn_y1}",
+        f"KP: \t{_g(cfg.kp)}",
+        f"Inlet control: \t{cfg.ctrl_every}x{cfg.ctrl_start}",
     ]
     text = "\n".join(lines) + "\n"
     if path:

@@ -360,7 +360,7 @@ __global__ void __launch_bounds__(Shape<S>::THREADS) k_stream_collide_closure(St
     const float zo = mlp_chain_infer(s, xb[w]);
     if (act != ACT_SKIP) {
         if (act == ACT_COLLIDE) {
-            CollideParams<R> cp = collide_params<R>(p);
+            CollideParams<R> cp = collide_params<R>(p, y);
             if (in_range) {
                 const float nut = fminf(ca.nrm.nu_scale * softplus(zo), ca.nu_cap);
                 const float wr = 1.f / (1.f / p.omega + 3.f * nut);

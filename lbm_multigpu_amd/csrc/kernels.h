@@ -106,6 +106,7 @@ struct StepParams {
     // layout): > 0 replaces the LES rate, 0 means the Smagorinsky rate
     const float* omega_node;
     PopShift sh;   // 16-bit storage offsets
+    int dyn_y1;    // rows y > dyn_y1 get no body force (INT_MAX: all rows get it)
     int xcd_chunk; // consecutive workgroups per XCD (logical_block)
     int nt_store;  // nontemporal population stores (lattices larger than the MALL)
 };
@@ -299,12 +300,12 @@ __device__ __forceinline__ int node_pull(const StepParams& p, int x, int y, int 
 }
 
 template <class R>
-__device__ __forceinline__ CollideParams<R> collide_params(const StepParams& p)
+__device__ __forceinline__ CollideParams<R> collide_params(const StepParams& p, int y)
 {
     CollideParams<R> cp;
     cp.omega = R(p.omega);
     cp.cs = R(p.cs);
-    cp.dpdy = R(p.dpdy);
+    cp.dpdy = y > p.dyn_y1 ? R(0) : R(p.dpdy);
     cp.s1 = R(p.s1); cp.s2 = R(p.s2); cp.s4 = R(p.s4); cp.s10 = R(p.s10); cp.s16 = R(p.s16);
     return cp;
 }
@@ -359,13 +360,14 @@ __device__ __forceinline__ void stats_update(const StepParams& p, int cell, cons
     }
 }
 
-// Collide a pulled node per its action (plus its running statistics).
+// Collide a pulled node per its action (plus its running statistics); the
+// body force acts only up to row y = dyn_y1 (MRTtest/DynInlet3.cu:1744).
 template <int MODEL, class S>
-__device__ __forceinline__ void node_collide(const StepParams& p, int cell, int act, typename Arith<S>::R f[Q])
+__device__ __forceinline__ void node_collide(const StepParams& p, int y, int cell, int act, typename Arith<S>::R f[Q])
 {
     using R = typename Arith<S>::R;
     if (act != ACT_COLLIDE) return;
-    CollideParams<R> cp = collide_params<R>(p);
+    CollideParams<R> cp = collide_params<R>(p, y);
     if (p.omega_node) {
         const float w = *at(p.omega_node, cell);
         if (w > 0.f) {
@@ -379,10 +381,10 @@ __device__ __forceinline__ void node_collide(const StepParams& p, int cell, int 
 
 // Collide (per the node's action) and store one pulled node.
 template <int MODEL, class S, int AUX>
-__device__ __forceinline__ void node_finish(const StepParams& p, int cell, int act, typename Arith<S>::R f[Q])
+__device__ __forceinline__ void node_finish(const StepParams& p, int y, int cell, int act, typename Arith<S>::R f[Q])
 {
     if (act == ACT_SKIP) return;
-    node_collide<MODEL, S>(p, cell, act, f);
+    node_collide<MODEL, S>(p, y, cell, act, f);
     node_store_policy<S, AUX>(p, cell, f);
 }
 
@@ -396,7 +398,7 @@ __device__ __forceinline__ void node_update(const StepParams& p, int x, int y, i
     const int cell = x + p.nxp * (y + p.ny * z);
     R f[Q];
     const int act = node_pull<S>(p, x, y, z, cell, f, fx, fy, fz);
-    node_finish<MODEL, S, AUX>(p, cell, act, f);
+    node_finish<MODEL, S, AUX>(p, y, cell, act, f);
 }
 
 // Momentum-exchange partial of one workgroup: wavefront sums, then the
@@ -455,6 +457,28 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const double* part, lon
         if (threadIdx.x == 0) out[c] = sh[0];
         __syncthreads();
     }
+}
+
+// AverageV (MRTtest/DynInlet3.cu:2561-2600): v = sum_i c_iy f_i at each node
+// of the plane y, summed in the reference's order, then over the 64-node
+// row segment of the wavefront in FP64; one partial per (segment, z).
+template <class S>
+__global__ void __launch_bounds__(64) k_plane_v(const S* f, long long qs, int nx, int nxp, int ny, int nsegw, int y,
+                                                double* part, PopShift sh)
+{
+    using R = typename Arith<S>::R;
+    const int seg = blockIdx.x % nsegw, z = blockIdx.x / nsegw;
+    const int x = seg * 64 + lane_id();
+    double v = 0.0;
+    if (x < nx) {
+        const long long cell = x + (long long)nxp * (y + (long long)ny * z);
+        R g[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) g[i] = R(Store<S>::to_real(f[i * qs + cell], i, sh));
+        v = double(g[2] - g[4] + g[5] + g[6] - g[7] - g[8] + g[11] - g[13] + g[16] - g[18]);
+    }
+    v = wave_sum(v);
+    if (lane_id() == 0) part[blockIdx.x] = v;
 }
 
 // Per-block FP64 sums of (rho, jx, jy, jz) over non-solid nodes.

@@ -38,7 +38,8 @@ const char* const kRefKeys[] = {
     "LR factor", "LR location", "LR blocksize:", "omega in LR", "GPUs per node", "KP:"};
 const char* const kExtKeys[] = {
     "MRT rates:", "Boundary x:", "Boundary y:", "Boundary z:", "Halfway bounce-back:", "Storage:",
-    "Geometry:", "Inlet perturbation:", "Seed:", "Temporal interpolation:", "Force steps:", "Statistics:"};
+    "Geometry:", "Inlet perturbation:", "Seed:", "Temporal interpolation:", "Force steps:", "Statistics:",
+    "DYNY1:", "Inlet control:"};
 
 struct Run {
     int nx = 64, ny = 64, nz = 64;
@@ -51,6 +52,9 @@ struct Run {
     double obst1[3] = {0, 0, 0}, obst1_r = 0, obst2[3] = {0, 0, 0}, obst2_r = 0;
     int lr[3] = {0, 0, 0};
     double lr_loc[3] = {0, 0, 0};
+    // dynamic-inlet precursor (MRTtest/DynInlet3.cu:1744, :2561-2600, :3147-3170)
+    int dyn_y1 = -1, ctrl_every = 100, ctrl_start = 10000;
+    double kp = 0;
 };
 
 std::string trim(const std::string& s)
@@ -159,6 +163,12 @@ bool parse(const std::string& path, Run& r)
     const char* axis[3] = {"Boundary x:", "Boundary y:", "Boundary z:"};
     for (int a = 0; a < 3; ++a)
         if (has(axis[a])) r.bc[a] = trim(seen[axis[a]]);
+    if (has("KP:")) r.kp = num("KP:");
+    if (has("DYNY1:")) r.dyn_y1 = (int)num("DYNY1:");
+    if (has("Inlet control:")) { // "<every>x<start>"
+        auto v = numbers(seen["Inlet control:"], "x");
+        if (v.size() == 2) { r.ctrl_every = (int)v[0]; r.ctrl_start = (int)v[1]; }
+    }
     return true;
 }
 
@@ -219,6 +229,9 @@ std::string inputs_record(const Run& r)
     o << "Temporal interpolation: \t" << (int)r.temporal << "\n";
     o << "Force steps: \t" << (r.force_steps < 0 ? 0 : r.force_steps) << "\n";
     o << "Statistics: \t" << (int)r.velav << "\n";
+    o << "DYNY1: \t" << r.dyn_y1 << "\n";
+    o << "KP: \t" << g(r.kp) << "\n";
+    o << "Inlet control: \t" << r.ctrl_every << "x" << r.ctrl_start << "\n";
     return o.str();
 }
 
@@ -271,6 +284,10 @@ bool to_config(const Run& r, lbm_config& c, std::string& why)
     c.lr_nx = r.lr[0]; c.lr_ny = r.lr[1]; c.lr_nz = r.lr[2];
     c.lr_x0 = (float)r.lr_loc[0]; c.lr_y0 = (float)r.lr_loc[1]; c.lr_z0 = (float)r.lr_loc[2];
     c.temporal_interp = r.temporal;
+    c.dyn_y1 = r.dyn_y1;
+    c.kp = (float)r.kp;
+    c.ctrl_every = r.ctrl_every;
+    c.ctrl_start = r.ctrl_start;
     return true;
 }
 
@@ -361,6 +378,18 @@ int main(int argc, char** argv)
         for (int i = 0; i < nf; ++i)
             o << i + r.startf << ", " << (float)F[3 * i] / ref << ", " << (float)F[3 * i + 1] / ref << ", "
               << (float)F[3 * i + 2] / ref << "\n";
+    }
+    // the dynamic-inlet record: "t, Av_V, dpdy" at control events with
+    // t % 1000 == 0 (main(), MRTtest/DynInlet3.cu:3170), in CASENAME.vel
+    {
+        int nc = 0;
+        if (lbm_get_inlet_control(ctx, nullptr, 0, &nc)) return die("lbm_get_inlet_control", ctx);
+        std::vector<double> rows(3 * (size_t)std::max(nc, 1));
+        if (nc > 0 && lbm_get_inlet_control(ctx, rows.data(), nc, &nc)) return die("lbm_get_inlet_control", ctx);
+        std::ofstream o(casename + ".vel");
+        for (int i = 0; i < nc; ++i)
+            if ((long long)rows[3 * i] % 1000 == 0)
+                o << (long long)rows[3 * i] << ", " << (float)rows[3 * i + 1] << ", " << (float)rows[3 * i + 2] << "\n";
     }
     std::printf("lbm_run: %lld steps in %.3f s GPU (%.3f s wall), %.2f GLUPS\n", r.tmax, gpu_ms * 1e-3, wall,
                 r.tmax > 0 ? updates * r.tmax / (gpu_ms * 1e-3) / 1e9 : 0.0);

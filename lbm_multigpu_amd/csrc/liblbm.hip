@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -73,6 +74,8 @@ struct lbm_ctx {
     float* dwall = nullptr;    // wall distance per node (padded layout)
     float nu_cap = 0.05f;
     int closure_every = 1;     // network refresh interval (steps)
+    float dpdy = 0.f;          // body force as the inlet control leaves it
+    std::vector<double> ctrl;  // control history, rows (t, Av_V, dpdy)
     long long closure_t0 = 0;  // step of the first refresh
     float* omega_field = nullptr; // per-node rate between refreshes (padded layout)
     std::string err;
@@ -153,6 +156,11 @@ extern "C" void lbm_config_default(lbm_config* c)
     c->force_steps = 0;
     c->refinement = 0;
     c->temporal_interp = 1;
+    // dynamic-inlet control as main() runs it (MRTtest/DynInlet3.cu:3147)
+    c->dyn_y1 = -1;
+    c->kp = 0.0f;
+    c->ctrl_every = 100;
+    c->ctrl_start = 10000;
 }
 
 // ---------------------------------------------------------------- helpers
@@ -215,6 +223,9 @@ static int validate(const lbm_config* c, std::string& why)
     if (c->cs > 0.0f && c->model == LBM_MODEL_MRT_CPU3D) { why = "MRT_CPU3D has no LES"; return 0; }
     if (c->force_steps < 0 || c->startf < 0) { why = "force window must be >= 0"; return 0; }
     if (c->geometry < 0 || c->geometry > 2) { why = "unknown geometry"; return 0; }
+    if (c->dyn_y1 < -1 || c->dyn_y1 >= c->ny) { why = "DYNY1 must be -1 or a row of the domain"; return 0; }
+    if (c->ctrl_every < 0 || c->ctrl_start < 0) { why = "the inlet control interval and start must be >= 0"; return 0; }
+    if (c->kp != 0.0f && !(c->umax > 0.0f)) { why = "the inlet P-control (KP) needs UMAX > 0"; return 0; }
     if (c->model == LBM_MODEL_MRT_DHUMIERES)
         for (int k : {1, 2, 4, 6, 8, 10, 12, 16, 17, 18})
             if (!(c->s_dhum[k] > 0.0f && c->s_dhum[k] < 2.0f)) { why = "d'Humieres rates must lie in (0,2)"; return 0; }
@@ -301,6 +312,7 @@ extern "C" int lbm_create(const lbm_config* cfg, lbm_ctx** out)
     lbm_ctx* ctx = new lbm_ctx();
     ctx->cfg = *cfg;
     ctx->device = dev;
+    ctx->dpdy = cfg->dpdy;
     int rc;
     auto bail = [&](int code) {
         lbm_destroy(ctx);
@@ -537,6 +549,8 @@ extern "C" int lbm_init_equilibrium(lbm_ctx* ctx, int kind, const float* rho, co
     }
     ctx->t = 0;
     ctx->force_recorded = 0;
+    ctx->dpdy = ctx->cfg.dpdy;
+    ctx->ctrl.clear();
     ctx->initialised = true;
     return LBM_OK;
 }
@@ -552,6 +566,8 @@ extern "C" int lbm_init_taylor_green(lbm_ctx* ctx, float u0)
     if (rc) return rc;
     ctx->t = 0;
     ctx->force_recorded = 0;
+    ctx->dpdy = ctx->cfg.dpdy;
+    ctx->ctrl.clear();
     ctx->initialised = true;
     return LBM_OK;
 }
@@ -572,7 +588,18 @@ static StepParams step_params(lbm_ctx* ctx, const Level& L, float omega)
     p.halfway = c.halfway;
     p.omega = omega;
     p.cs = c.cs;
-    p.dpdy = c.dpdy;
+    // body force: the control's current DPDY; on the fine level scaled by
+    // LRFACTOR = 1/2 (a force density in lattice units halves under the
+    // 2:1 acoustic scaling).  The reference's fine-level kernels pass the
+    // constant LRFACTOR itself as mrt_collide's dpdy argument
+    // (MRTtest/DynInlet3.cu:1918, :2024, :2151), which AddForce would apply
+    // as a body force of 0.5 per fine step; that is not reproduced.
+    const bool fine = &L != &ctx->lv[0];
+    p.dpdy = fine ? 0.5f * ctx->dpdy : ctx->dpdy;
+    // rows past DYNY1 get no force (:1744); fine row j sits at coarse y = lr_y0 + j/2
+    p.dyn_y1 = INT_MAX;
+    if (c.dyn_y1 >= 0)
+        p.dyn_y1 = fine ? (int)std::floor(2.0f * ((float)c.dyn_y1 - c.lr_y0)) : c.dyn_y1;
     p.u_lid = c.umax;
     p.s1 = c.s_dhum[1]; p.s2 = c.s_dhum[2]; p.s4 = c.s_dhum[4]; p.s10 = c.s_dhum[10]; p.s16 = c.s_dhum[16];
     p.omega_node = nullptr;
@@ -780,6 +807,80 @@ static int refine_fc(lbm_ctx* ctx)
     return LBM_OK;
 }
 
+// AverageV on the current coarse lattice: FP64 sum of v over the plane y.
+static int plane_v(lbm_ctx* ctx, int y, double* sum)
+{
+    Level& L = ctx->lv[0];
+    const int nb = L.nsegw * L.nz; // <= 4 * L.blocks partials fit in mpart
+    with_storage(L.storage, [&](auto* tag) {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL(k_plane_v<S>, dim3(nb), dim3(64), 0, ctx->stream, (const S*)L.fa, L.qs, L.nx, L.nxp,
+                           L.ny, L.nsegw, y, ctx->mpart, L.shift);
+        return 0;
+    });
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, ctx->stream, ctx->mpart, (long long)nb, 1,
+                       ctx->red);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipMemcpyAsync(sum, ctx->red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return LBM_OK;
+}
+
+// The reference's mean of that sum, in FP32 as main() forms it (:3160).
+static float plane_mean(const Level& L, double sum)
+{
+    return (float)sum / (float)((L.nx - 2) * L.nz);
+}
+
+// Dynamic-inlet P-control after coarse step t (main(), MRTtest/DynInlet3.cu:
+// 3147-3170): AverageV on the plane y = DYNY1, then dpdy += diff*KP*|DPDY|
+// with diff = (Av_V - UMAX)/UMAX, in FP32.  DPDY is the configured initial
+// force.  The new dpdy acts from the next step on.
+static int inlet_control(lbm_ctx* ctx, long long t)
+{
+    const lbm_config& c = ctx->cfg;
+    if (c.ctrl_every <= 0 || t % c.ctrl_every != 0 || t <= c.ctrl_start) return LBM_OK;
+    const Level& L = ctx->lv[0];
+    double sum = 0;
+    int rc = plane_v(ctx, c.dyn_y1 >= 0 ? c.dyn_y1 : L.ny - 1, &sum);
+    if (rc) return rc;
+    const float av = plane_mean(L, sum);
+    if (c.kp != 0.0f) {
+        const float diff = (av - c.umax) / c.umax;
+        ctx->dpdy += diff * c.kp * std::fabs(c.dpdy);
+    }
+    ctx->ctrl.insert(ctx->ctrl.end(), {(double)t, (double)av, (double)ctx->dpdy});
+    return LBM_OK;
+}
+
+extern "C" int lbm_get_inlet_control(lbm_ctx* ctx, double* rows, int max_rows, int* nrows)
+{
+    if (!ctx || !nrows) return fail(ctx, LBM_EINVAL, "lbm_get_inlet_control: null argument");
+    const int n = (int)(ctx->ctrl.size() / 3);
+    *nrows = n;
+    if (rows) std::copy(ctx->ctrl.begin(), ctx->ctrl.begin() + 3 * (size_t)std::min(n, std::max(max_rows, 0)), rows);
+    return LBM_OK;
+}
+
+extern "C" int lbm_plane_mean_v(lbm_ctx* ctx, int y, double* sum, double* mean)
+{
+    if (!ctx || !sum || !mean) return fail(ctx, LBM_EINVAL, "lbm_plane_mean_v: null argument");
+    if (!ctx->initialised) return fail(ctx, LBM_ESTATE, "lbm_plane_mean_v: call lbm_init_equilibrium first");
+    const Level& L = ctx->lv[0];
+    if (y < 0 || y >= L.ny) return fail(ctx, LBM_EINVAL, "lbm_plane_mean_v: y outside the domain");
+    int rc = plane_v(ctx, y, sum);
+    if (rc) return rc;
+    *mean = plane_mean(L, *sum);
+    return LBM_OK;
+}
+
+extern "C" int lbm_get_dpdy(lbm_ctx* ctx, float* dpdy)
+{
+    if (!ctx || !dpdy) return fail(ctx, LBM_EINVAL, "lbm_get_dpdy: null argument");
+    *dpdy = ctx->dpdy;
+    return LBM_OK;
+}
+
 extern "C" int lbm_step(lbm_ctx* ctx, int nsteps)
 {
     if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_step: null context");
@@ -787,6 +888,7 @@ extern "C" int lbm_step(lbm_ctx* ctx, int nsteps)
     if (nsteps < 0) return fail(ctx, LBM_EINVAL, "lbm_step: nsteps < 0");
     for (int s = 0; s < nsteps; ++s) {
         coarse_step(ctx);
+        if (int rc = inlet_control(ctx, ctx->t)) return rc;
         if (ctx->nlevels > 1) {
             // coarse t -> t+1 done; fine sub-steps t -> t+1/2 -> t+1 (main(),
             // MRTtest/DynInlet3.cu:3180-3245), then the fine -> coarse restriction
@@ -1018,6 +1120,11 @@ static float smag_column(const float f[19])
 extern "C" int lbm_write_tecplot(lbm_ctx* ctx, const char* casename)
 {
     if (!ctx || !casename) return fail(ctx, LBM_EINVAL, "lbm_write_tecplot: null argument");
+    // the slice file (WriteResults/WriteResultsLR's outputslice): the coarse
+    // plane z = 1 (:2672) and the fine plane k = 3 (:2762), one zone each
+    const std::string slice_name = std::string(casename) + "_slice.dat";
+    std::ofstream slice(slice_name);
+    if (!slice) return fail(ctx, LBM_EIO, "lbm_write_tecplot: cannot open " + slice_name);
     for (int level = 0; level < ctx->nlevels; ++level) {
         const Level& L = ctx->lv[level];
         std::vector<double> r(L.cells), u(L.cells), v(L.cells), w(L.cells);
@@ -1033,35 +1140,51 @@ extern "C" int lbm_write_tecplot(lbm_ctx* ctx, const char* casename)
             if (!rc) rc = lbm_get_populations(ctx, level, pops.data());
             if (rc) return rc;
         }
+        // columns: X,Y,Z,u,v,w,rho, then with statistics the coarse level's
+        // velAv[0..2], ufluc, vfluc, wfluc, Smag (WriteResults, :2605) or the
+        // fine level's velAv[0..1], ufluc, vfluc, Smag (WriteResultsLR, :2699)
+        const int nstat = level == 0 ? 3 : 2;
+        auto header = [&](std::ostream& out, int K) {
+            out << "VARIABLES = \"X\",\"Y\",\"Z\",\"u\",\"v\",\"w\",\"rho\"";
+            if (stats) {
+                out << (level == 0 ? ",\"velAv[0]\",\"velAv[1]\",\"velAv[2]\",\"ufluc\",\"vfluc\",\"wfluc\",\"Smag\""
+                                   : ",\"velAv[0]\",\"velAv[1]\",\"ufluc\",\"vfluc\",\"Smag\"");
+            }
+            out << "\n";
+            out << "ZONE F=POINT, I=" << L.nx << ", J=" << L.ny << ", K=" << K << "\n";
+        };
+        auto row = [&](std::ostream& out, int x, int y, int z) {
+            const size_t c = (size_t)x + (size_t)L.nx * (y + (size_t)L.ny * z);
+            if (level == 0) out << x << ", " << y << ", " << z << ", ";
+            else {
+                // fine node position in coarse units (WriteResultsLR)
+                out << ctx->cfg.lr_x0 + 0.5f * x << ", " << ctx->cfg.lr_y0 + 0.5f * y << ", "
+                    << ctx->cfg.lr_z0 + 0.5f * z << ", ";
+            }
+            out << (float)u[c] << "," << (float)v[c] << "," << (float)w[c] << "," << (float)r[c];
+            if (stats) {
+                float f[19];
+                for (int i = 0; i < Q; ++i) f[i] = pops[(size_t)i * L.cells + c];
+                for (int a = 0; a < nstat; ++a) out << "," << mean[a * L.cells + c];
+                for (int a = 0; a < nstat; ++a) out << "," << dev[a * L.cells + c];
+                out << "," << smag_column(f);
+            }
+            out << "\n";
+        };
         std::string name = std::string(casename) + (level == 0 ? ".dat" : "_LR.dat");
         std::ofstream out(name);
         if (!out) return fail(ctx, LBM_EIO, "lbm_write_tecplot: cannot open " + name);
-        out << "VARIABLES = \"X\",\"Y\",\"Z\",\"u\",\"v\",\"w\",\"rho\"";
-        if (stats) out << ",\"velAv[0]\",\"velAv[1]\",\"velAv[2]\",\"ufluc\",\"vfluc\",\"wfluc\",\"Smag\"";
-        out << "\n";
-        out << "ZONE F=POINT, I=" << L.nx << ", J=" << L.ny << ", K=" << L.nz << "\n";
+        header(out, L.nz);
         for (int z = 0; z < L.nz; ++z)
             for (int y = 0; y < L.ny; ++y)
-                for (int x = 0; x < L.nx; ++x) {
-                    const size_t c = (size_t)x + (size_t)L.nx * (y + (size_t)L.ny * z);
-                    if (level == 0) out << x << ", " << y << ", " << z << ", ";
-                    else {
-                        // fine node position in coarse units (WriteResultsLR)
-                        out << ctx->cfg.lr_x0 + 0.5f * x << ", " << ctx->cfg.lr_y0 + 0.5f * y << ", "
-                            << ctx->cfg.lr_z0 + 0.5f * z << ", ";
-                    }
-                    out << (float)u[c] << "," << (float)v[c] << "," << (float)w[c] << "," << (float)r[c];
-                    if (stats) {
-                        float f[19];
-                        for (int i = 0; i < Q; ++i) f[i] = pops[(size_t)i * L.cells + c];
-                        for (int a = 0; a < 3; ++a) out << "," << mean[a * L.cells + c];
-                        for (int a = 0; a < 3; ++a) out << "," << dev[a * L.cells + c];
-                        out << "," << smag_column(f);
-                    }
-                    out << "\n";
-                }
+                for (int x = 0; x < L.nx; ++x) row(out, x, y, z);
         if (!out) return fail(ctx, LBM_EIO, "lbm_write_tecplot: write failed for " + name);
+        const int zs = std::min(level == 0 ? 1 : 3, L.nz - 1);
+        header(slice, 1);
+        for (int y = 0; y < L.ny; ++y)
+            for (int x = 0; x < L.nx; ++x) row(slice, x, y, zs);
     }
+    if (!slice) return fail(ctx, LBM_EIO, "lbm_write_tecplot: write failed for " + slice_name);
     return LBM_OK;
 }
 

@@ -118,6 +118,7 @@ void orc_default_params(orc_params* p)
     p->s_dhum[4] = p->s_dhum[6] = p->s_dhum[8] = 1.2;
     p->s_dhum[10] = p->s_dhum[12] = 1.4;
     p->s_dhum[16] = p->s_dhum[17] = p->s_dhum[18] = 1.98;
+    p->dyn_y1 = -1;
 }
 
 #define FN(name) name##_f32

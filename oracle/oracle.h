@@ -60,6 +60,8 @@ typedef struct {
     double dpdy;        /* AddForce pressure gradient (LES-family operators only) */
     double u_lid;       /* UMAX used by DirichletWest_Regularized */
     double s_dhum[19];  /* rates for ORC_MRT_DHUM (entries 9,11,13,14,15 ignored: omega) */
+    int dyn_y1;         /* rows y > dyn_y1 get no body force, "if(y>DYNY1) dpdy = 0.f"
+                           (MRTtest/DynInlet3.cu:1744); -1: every row gets it */
 } orc_params;
 
 void orc_default_params(orc_params* p);

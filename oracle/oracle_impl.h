@@ -310,6 +310,8 @@ void FN(orc_step)(const orc_params* p, const uint8_t* mask, const R* u_in,
     const size_t N = (size_t)nx * ny * nz;
     const int n[3] = {nx, ny, nz};
     double F0 = 0, F1 = 0, F2 = 0;
+    orc_params p_noforce = *p; /* rows past DYNY1 */
+    p_noforce.dpdy = 0.0;
 #pragma omp parallel for collapse(2) schedule(static) reduction(+ : F0, F1, F2)
     for (int z = 0; z < nz; ++z)
         for (int y = 0; y < ny; ++y)
@@ -385,7 +387,7 @@ void FN(orc_step)(const orc_params* p, const uint8_t* mask, const R* u_in,
                 } else if (code == ORC_SYM_XMAX) {
                     FN(orc_xsym_top)(f, y, z, ny, nz);
                 }
-                FN(orc_collide)(p, f);
+                FN(orc_collide)((p->dyn_y1 >= 0 && y > p->dyn_y1) ? &p_noforce : p, f);
                 for (int i = 0; i < 19; ++i) fout[i * N + cell] = f[i];
                 F0 += F[0]; F1 += F[1]; F2 += F[2];
             }

@@ -22,7 +22,7 @@ class OrcParams(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
                 ("bc", C.c_int * 3), ("collision", C.c_int), ("halfway", C.c_int),
                 ("omega", C.c_double), ("cs", C.c_double), ("dpdy", C.c_double),
-                ("u_lid", C.c_double), ("s_dhum", C.c_double * 19)]
+                ("u_lid", C.c_double), ("s_dhum", C.c_double * 19), ("dyn_y1", C.c_int)]
 
 
 _lib = None
@@ -79,7 +79,7 @@ def ref():
 
 
 def params(nx=1, ny=1, nz=1, bc=(PERIODIC,) * 3, collision=BGK, halfway=0, omega=1.0,
-           cs=0.0, dpdy=0.0, u_lid=0.0):
+           cs=0.0, dpdy=0.0, u_lid=0.0, dyn_y1=-1):
     p = OrcParams()
     lib().orc_default_params(C.byref(p))
     p.nx, p.ny, p.nz = nx, ny, nz
@@ -87,7 +87,30 @@ def params(nx=1, ny=1, nz=1, bc=(PERIODIC,) * 3, collision=BGK, halfway=0, omega
         p.bc[a] = bc[a]
     p.collision, p.halfway = collision, halfway
     p.omega, p.cs, p.dpdy, p.u_lid = omega, cs, dpdy, u_lid
+    p.dyn_y1 = dyn_y1
     return p
+
+
+def plane_v_sum(f, y):
+    """AverageV (MRTtest/DynInlet3.cu:2561-2600) restated: v = sum_i c_iy f_i
+    at each node of the plane y in the reference's FP32 order
+    (f2-f4+f5+f6-f7-f8+f11-f13+f16-f18), summed over all x and z in FP64.
+    f: populations [19][nz][ny][nx]."""
+    g = np.asarray(f, np.float32)[:, :, y, :]
+    v = g[2] - g[4] + g[5] + g[6] - g[7] - g[8] + g[11] - g[13] + g[16] - g[18]
+    return float(v.astype(np.float64).sum())
+
+
+def inlet_control_update(dpdy, sum_v, nx, nz, umax, kp, dpdy0):
+    """The host P-controller of main() (MRTtest/DynInlet3.cu:3159-3165), in
+    FP32: Av_V = sum / ((nx - 2) nz), dpdy += (Av_V - UMAX)/UMAX * KP * |DPDY|.
+    Returns (Av_V, new dpdy)."""
+    f = np.float32
+    av = f(sum_v) / f((nx - 2) * nz)
+    if kp != 0:
+        diff = (av - f(umax)) / f(umax)
+        dpdy = f(f(dpdy) + diff * f(kp) * abs(f(dpdy0)))
+    return float(av), float(dpdy)
 
 
 def _ptr(a):

@@ -85,6 +85,11 @@ def test_driver_run_matches_solver(tmp_path):
     assert (tmp_path / "drv.inputs").read_text() == C.write_inputs(cfg)
     dat = np.loadtxt(tmp_path / "drv.dat", delimiter=",", skiprows=2)
     assert dat.shape == (96 * 32 * 24, 14)
+    # the slice file is the z = 1 plane of the same columns (WriteResults' outputslice)
+    sl = np.loadtxt(tmp_path / "drv_slice.dat", delimiter=",", skiprows=2)
+    assert np.array_equal(sl, dat[96 * 32:2 * 96 * 32])
+    # the control window (t > 10000) is not reached: an empty CASENAME.vel
+    assert (tmp_path / "drv.vel").read_text() == ""
     force = np.loadtxt(tmp_path / "drv.force", delimiter=",")
     with L.Solver(C.read_inputs(str(path))) as s:
         s.set_geometry()

@@ -28,8 +28,9 @@ def to_orc(m):
     return lut[m]
 
 
-def small_case(temporal=True, interface_eq=0, rates="lrmgpu"):
+def small_case(temporal=True, interface_eq=0, rates="lrmgpu", dpdy=0.0):
     cc = cfgmod.channel(nx=64, ny=32, nz=32, side=6, x_center=22, tau=0.55, rates=rates)
+    cc.dpdy = dpdy
     cc.refinement = True
     cc.lr_size = (32, 24, 24)
     cc.lr_loc = (12.25, 4.25, 4.25)
@@ -38,10 +39,13 @@ def small_case(temporal=True, interface_eq=0, rates="lrmgpu"):
     return cc
 
 
-@pytest.mark.parametrize("temporal,interface_eq", [(True, 0), (False, 0), (True, 1)])
-def test_refined_channel_matches_oracle(temporal, interface_eq):
-    say("refine parity temporal", temporal, "eq", interface_eq)
-    cc = small_case(temporal, interface_eq)
+@pytest.mark.parametrize("temporal,interface_eq,dpdy", [(True, 0, 0.0), (False, 0, 0.0), (True, 1, 0.0),
+                                                         (True, 0, 2e-5)])
+def test_refined_channel_matches_oracle(temporal, interface_eq, dpdy):
+    """With a body force the fine level takes LRFACTOR x DPDY (the force
+    density of the 2:1 acoustic scaling)."""
+    say("refine parity temporal", temporal, "eq", interface_eq, "dpdy", dpdy)
+    cc = small_case(temporal, interface_eq, dpdy=dpdy)
     c = cc.to_c()
     T = 10
     with L.Solver(c) as s:
@@ -56,7 +60,7 @@ def test_refined_channel_matches_oracle(temporal, interface_eq):
                  cs=float(np.float32(c.cs)), temporal=int(temporal),
                  eq_kind=(O.EQ_MRT_MEQ if interface_eq else O.EQ_BGK_MEQ))
     pf = O.params(*cc.lr_size, bc=(O.CLAMP,) * 3, collision=pc.collision, halfway=1, omega=r.omega_f,
-                  cs=pc.cs, dpdy=pc.dpdy)
+                  cs=pc.cs, dpdy=0.5 * pc.dpdy)
     u_in = O.inlet_profile(c.ny, c.nz, c.umax, c.inlet_amp, c.seed)
     ca, fa = O.refine_run(pc, pf, r, c0, f0, T, cmask=cm, fmask=fm, u_in=u_in)
     co = O.macro(pc, ca)
@@ -93,7 +97,7 @@ def test_refined_channel_other_storage_formats(storage, tol):
     r = O.refine([int(round(x - 0.25)) for x in cc.lr_loc], cc.lr_size, float(np.float32(c.omega)),
                  cs=float(np.float32(c.cs)), temporal=1, eq_kind=O.EQ_BGK_MEQ)
     pf = O.params(*cc.lr_size, bc=(O.CLAMP,) * 3, collision=pc.collision, halfway=1, omega=r.omega_f,
-                  cs=pc.cs, dpdy=pc.dpdy)
+                  cs=pc.cs, dpdy=0.5 * pc.dpdy)
     u_in = O.inlet_profile(c.ny, c.nz, c.umax, c.inlet_amp, c.seed)
     ca, fa = O.refine_run(pc, pf, r, c0, f0, T, cmask=cm, fmask=fm, u_in=u_in)
     co = O.macro(pc, ca)

@@ -4,6 +4,7 @@ stream-collide, against the oracle's restatement applied after each oracle
 step on the same inputs.  Tolerance: the populations of the two FP32 paths
 agree to ~1e-7 per step, so the means agree within 1e-5 of their scale and
 the deviations within 1e-3 of theirs."""
+import os
 import sys
 
 import numpy as np
@@ -77,6 +78,20 @@ def test_statistics_phases_and_fine_level():
             assert np.abs(mean[0] - 0.04).max() < 3e-5
             assert np.abs(mean[1:]).max() < 3e-5
             assert np.abs(dev).max() < 1e-9
+        # Tecplot: the fine level in WriteResultsLR's 12 columns (X,Y,Z in
+        # coarse units, u,v,w,rho, velAv[0..1], ufluc, vfluc, Smag)
+        import tempfile
+        with tempfile.TemporaryDirectory() as d:
+            s.write_tecplot(os.path.join(d, "st"))
+            lr = np.loadtxt(os.path.join(d, "st_LR.dat"), delimiter=",", skiprows=2)
+            mean, dev = s.stats(1)
+            assert lr.shape == (24 * 16 * 16, 12)
+            assert np.allclose(lr[:24, 0], 8.25 + 0.5 * np.arange(24))
+            # the file holds 6 significant digits (ostream's default)
+            for col, ref in ((7, mean[0]), (8, mean[1]), (9, dev[0]), (10, dev[1])):
+                assert np.abs(lr[:, col] - ref.reshape(-1)).max() <= 1e-5 * np.abs(ref).max() + 1e-30
+            sl = open(os.path.join(d, "st_slice.dat")).read().split("ZONE")
+            assert len(sl) == 3  # coarse z = 1 and fine k = 3 zones
 
 
 def test_statistics_errors():

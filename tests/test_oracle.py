@@ -339,3 +339,43 @@ def test_symmetry_plane_is_a_mirror_inside_the_plane():
             else:
                 assert g[i] == f[i]
         assert abs((C[:, 0] * g).sum()) < 1e-15
+
+
+def test_row_limited_body_force():
+    """dyn_y1 (MRTtest/DynInlet3.cu:1744, "if(y>DYNY1) dpdy = 0.f"): rows up to
+    dyn_y1 step exactly as with the force everywhere, rows past it exactly
+    as with no force."""
+    rng = np.random.default_rng(3)
+    nx, ny, nz = 8, 10, 6
+    f = rand_pops(rng, nx * ny * nz).T.reshape(19, nz, ny, nx).copy()
+    kw = dict(collision=O.MRT_LES, omega=1.6, cs=0.01)
+    full, _ = O.step(O.params(nx, ny, nz, dpdy=1e-4, **kw), f)
+    none, _ = O.step(O.params(nx, ny, nz, dpdy=0.0, **kw), f)
+    lim, _ = O.step(O.params(nx, ny, nz, dpdy=1e-4, dyn_y1=4, **kw), f)
+    assert np.array_equal(lim[:, :, :5], full[:, :, :5])
+    assert np.array_equal(lim[:, :, 5:], none[:, :, 5:])
+    assert not np.array_equal(full, none)
+
+
+def test_plane_v_sum_against_momentum_definition():
+    """AverageV's restated sum equals sum over the plane of sum_i c_iy f_i with
+    c_y from the velocity table (FP64), so the ten hard-coded indices and
+    signs are pinned."""
+    rng = np.random.default_rng(4)
+    f = rng.standard_normal((19, 5, 7, 9)).astype(np.float32)
+    for y in (0, 3, 6):
+        ref = float(np.einsum("i,izx->", C[:, 1].astype(np.float64), f[:, :, y, :].astype(np.float64)))
+        assert abs(O.plane_v_sum(f, y) - ref) < 1e-5 * np.abs(f).sum()
+
+
+def test_inlet_control_law():
+    """P-controller (MRTtest/DynInlet3.cu:3159-3165): at Av_V = UMAX dpdy is
+    held; at Av_V = 2 UMAX it moves by KP |DPDY|; KP = 0 holds it."""
+    nx, nz, umax = 10, 4, 0.05
+    plane = (nx - 2) * nz
+    av, d = O.inlet_control_update(2e-5, umax * plane, nx, nz, umax, 0.5, 2e-5)
+    assert abs(av - umax) < 1e-8 and d == np.float32(2e-5)
+    av, d = O.inlet_control_update(2e-5, 2 * umax * plane, nx, nz, umax, 0.5, -2e-5)
+    assert abs(d - (2e-5 + 0.5 * 2e-5)) < 1e-11
+    av, d = O.inlet_control_update(3e-5, 2 * umax * plane, nx, nz, umax, 0.0, 2e-5)
+    assert d == np.float32(3e-5)
