@@ -71,20 +71,44 @@ def cpu_baseline(cfg, budget_s=12.0):
     mask = lut[m]
     u_in = O.inlet_profile(c.ny, c.nz, c.umax, c.inlet_amp, c.seed)
     p = orc_params(c)
-    fb = np.empty_like(f)
-    steps, t0 = 0, time.perf_counter()
-    while True:
-        out, _ = O.step(p, f, mask=mask, u_in=u_in)
-        f, fb = out, f
-        steps += 1
-        if time.perf_counter() - t0 > budget_s or steps >= 50:
-            break
-    dt = time.perf_counter() - t0
+
+    def timed(max_steps, budget):
+        nonlocal f
+        steps, t0 = 0, time.perf_counter()
+        while True:
+            f, _ = O.step(p, f, mask=mask, u_in=u_in)
+            steps += 1
+            if time.perf_counter() - t0 > budget or steps >= max_steps:
+                break
+        return steps, time.perf_counter() - t0
+
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     cells = c.nx * c.ny * c.nz
-    return {"value": cells * steps / dt / 1e9, "unit": "GLUPS", "cores": threads, "kind": "port",
-            "sample": f"oracle (C, OpenMP {threads} threads) on the full {c.nx}x{c.ny}x{c.nz} workload, "
-                      f"{steps} steps in {dt:.1f} s"}
+    steps, dt = timed(50, budget_s)
+    out = {"value": cells * steps / dt / 1e9, "unit": "GLUPS", "cores": threads, "kind": "port",
+           "sample": f"oracle (C, OpenMP {threads} threads) on the full {c.nx}x{c.ny}x{c.nz} workload, "
+                     f"{steps} steps in {dt:.1f} s"}
+    # one thread as well (the reference's CPU program is single-threaded), and the host CPU
+    try:
+        import ctypes
+
+        gomp = ctypes.CDLL("libgomp.so.1")
+        gomp.omp_set_num_threads(1)
+        s1, d1 = timed(3, budget_s / 2)
+        gomp.omp_set_num_threads(threads)
+        out["single_thread_value"] = cells * s1 / d1 / 1e9
+        out["sample"] += f"; one thread: {s1} steps in {d1:.1f} s"
+    except OSError:
+        pass
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), None)
+        if model:
+            out["cpu_model"] = model
+            out["nproc"] = os.cpu_count()
+    except OSError:
+        pass
+    return out
 
 
 def copy_calibration(nbytes=637_534_208, reps=20):
