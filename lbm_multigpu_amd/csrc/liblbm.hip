@@ -204,9 +204,21 @@ extern "C" void lbm_inlet_profile(int ny, int nz, float u0, float amp, uint64_t 
 static int padded_nx(int nx, int) { return ((nx + 63) / 64) * 64; }
 
 // Kernels address a direction plane with 32-bit byte offsets.
+// Extra elements between consecutive direction planes.  A wavefront reads
+// and writes the same cell offset in all 19 planes at once; when the plane
+// size is a multiple of a large power of two (256 x 128 x 128 nodes: 16 MiB
+// in FP32) those 38 streams fall on few HBM channels.  7 x 64 elements
+// staggers the plane starts by 7 x 256 B (FP32) and keeps every row on a
+// wavefront boundary; measured +8 % FP32 on 256 x 128 x 128, +1.4 % FP32
+// and +3 % FP16 on 512 x 128 x 128, any non-zero pad doing about as well.
+#ifndef LBM_PLANE_PAD
+#define LBM_PLANE_PAD 448
+#endif
+static_assert(LBM_PLANE_PAD % 64 == 0, "plane padding keeps rows on wavefront boundaries");
+
 static bool plane_fits(int nx, int ny, int nz, int storage)
 {
-    return (long long)padded_nx(nx, storage) * ny * nz * elem_size(storage) <= (1LL << 32) - 64;
+    return ((long long)padded_nx(nx, storage) * ny * nz + LBM_PLANE_PAD) * elem_size(storage) <= (1LL << 32) - 64;
 }
 
 static int validate(const lbm_config* c, std::string& why)
@@ -250,7 +262,7 @@ static int alloc_level(lbm_ctx* ctx, Level& L, int nx, int ny, int nz, int stora
     L.nx = nx; L.ny = ny; L.nz = nz;
     L.nxp = padded_nx(nx, storage);    // rows padded to whole wavefront segments
     L.nsegw = L.nxp / 64;              // 64-node wavefront segments per row
-    L.qs = (long long)L.nxp * ny * nz;
+    L.qs = (long long)L.nxp * ny * nz + LBM_PLANE_PAD;
     L.cells = (size_t)nx * ny * nz;
     L.storage = storage;
     L.esize = elem_size(storage);

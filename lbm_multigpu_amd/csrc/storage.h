@@ -2,7 +2,9 @@
 //
 // Layout: SoA, one plane per direction, plane q holds f_q[z][y][x] with the
 // x stride padded to a multiple of 64 elements so every 64-lane wavefront
-// row access starts on a 256-byte boundary (FP32).  Four element types:
+// row access starts on a 256-byte boundary (FP32); consecutive planes sit
+// 448 elements further apart than a plane's nodes need, which staggers the
+// 19 planes over the HBM channels.  Four element types:
 //   float      FP32 populations, FP32 arithmetic (reference precision)
 //   double     FP64 populations and arithmetic (the FP64 yardstick)
 //   Half16     f_i - w_i as IEEE binary16, FP32 arithmetic
