@@ -199,12 +199,13 @@ __device__ __forceinline__ void chain_stage(MlpChain& s, const MlpChain* g)
     for (int i = t; i < n; i += nt) dst[i] = src[i];
 }
 
-// z = BF16(acc + b), h = BF16(GELU(z)) for the four units this lane holds
-__device__ __forceinline__ void chain_act(const f32x4& acc, const f32x4& b, __bf16 h[4])
+// z = BF16(acc), h = BF16(GELU(z)) for the four units this lane holds; the
+// accumulator started from the layer's bias, so acc = b + W h
+__device__ __forceinline__ void chain_act(const f32x4& acc, __bf16 h[4])
 {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const float z = (float)(__bf16)(acc[j] + b[j]);
+        const float z = (float)(__bf16)acc[j];
         h[j] = (__bf16)gelu(z);
     }
 }
@@ -227,14 +228,12 @@ __device__ __forceinline__ float mlp_chain_infer(const MlpChain& c, const short*
         f32x4 acc[4];
         bf16x8 hb[2];
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt) {
-            const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
-            acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(c.w1[mt][lane], xin, zero, 0, 0, 0);
-        }
+        for (int mt = 0; mt < 4; ++mt)
+            acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(c.w1[mt][lane], xin, c.b1[mt][q], 0, 0, 0);
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
             __bf16 h[4];
-            chain_act(acc[mt], c.b1[mt][q], h);
+            chain_act(acc[mt], h);
 #pragma unroll
             for (int j = 0; j < 4; ++j) hb[mt >> 1][(mt & 1) * 4 + j] = h[j];
         }
@@ -244,15 +243,14 @@ __device__ __forceinline__ float mlp_chain_infer(const MlpChain& c, const short*
             const f32x4(*b)[4] = layer == 0 ? c.b2 : c.b3;
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) {
-                f32x4 a = {0.f, 0.f, 0.f, 0.f};
-                a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[mt][0][lane], hb[0], a, 0, 0, 0);
+                f32x4 a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[mt][0][lane], hb[0], b[mt][q], 0, 0, 0);
                 a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[mt][1][lane], hb[1], a, 0, 0, 0);
                 acc[mt] = a;
             }
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt) {
                 __bf16 h[4];
-                chain_act(acc[mt], b[mt][q], h);
+                chain_act(acc[mt], h);
 #pragma unroll
                 for (int j = 0; j < 4; ++j) hb[mt >> 1][(mt & 1) * 4 + j] = h[j];
             }
