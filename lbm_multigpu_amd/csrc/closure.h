@@ -195,18 +195,13 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float* X, long long n
                                                      MlpNorm nrm, float* nu)
 {
     __shared__ MlpChain s;
-    __shared__ short xb[4][64 * MLP_INP];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     chain_stage(s, chain);
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    float x[MLP_IN];
 #pragma unroll
-    for (int k = 0; k < MLP_INP; ++k) {
-        float v = 0.f;
-        if (i < n && k < MLP_IN) v = (X[i * MLP_IN + k] - nrm.mu[k]) * nrm.inv_sd[k];
-        xb[w][lane * MLP_INP + k] = bf(v);
-    }
+    for (int k = 0; k < MLP_IN; ++k) x[k] = i < n ? (X[i * MLP_IN + k] - nrm.mu[k]) * nrm.inv_sd[k] : 0.f;
     __syncthreads();
-    const float zo = mlp_chain_infer(s, xb[w]);
+    const float zo = mlp_chain_infer(s, mlp_pack_inputs(x));
     if (i < n) nu[i] = nrm.nu_scale * softplus(zo);
 }
 
@@ -312,13 +307,16 @@ struct ClosureArgs {
 // nodes that are not collided feed dummy inputs), then the MRT collide at
 // tau = tau0 + 3 nu_t, or the Smagorinsky rate where an input leaves the
 // training range.
+// Four wavefronts per workgroup for every storage type: the 19.5 KB of
+// staged weights are shared by all of them, and the LDS is what bounds the
+// kernel's occupancy.
+constexpr int CLOSURE_NW = 4;
 template <int MODEL, class S>
-__global__ void __launch_bounds__(Shape<S>::THREADS) k_stream_collide_closure(StepParams p, ClosureArgs ca)
+__global__ void __launch_bounds__(64 * CLOSURE_NW) k_stream_collide_closure(StepParams p, ClosureArgs ca)
 {
     using R = typename Arith<S>::R;
     __shared__ MlpChain s;
-    constexpr int NW = Shape<S>::NW;
-    __shared__ short xb[NW][64 * MLP_INP];
+    constexpr int NW = CLOSURE_NW;
     chain_stage(s, ca.chain);
     const int lb = logical_block();
     int x, y, z;
@@ -352,12 +350,11 @@ __global__ void __launch_bounds__(Shape<S>::THREADS) k_stream_collide_closure(St
         for (int q = 0; q < MLP_IN; ++q) in_range = in_range && feat[q] >= ca.nrm.lo[q] && feat[q] <= ca.nrm.hi[q];
         if (p.cs > 0.f) omega_les = les_omega(m, R(p.omega), R(p.cs));
     }
-    const int w = wave_id();
+    float xin[MLP_IN];
 #pragma unroll
-    for (int q = 0; q < MLP_INP; ++q)
-        xb[w][lane_id() * MLP_INP + q] = q < MLP_IN ? bf((feat[q] - ca.nrm.mu[q]) * ca.nrm.inv_sd[q]) : (short)0;
-    __syncthreads();
-    const float zo = mlp_chain_infer(s, xb[w]);
+    for (int q = 0; q < MLP_IN; ++q) xin[q] = (feat[q] - ca.nrm.mu[q]) * ca.nrm.inv_sd[q];
+    __syncthreads(); // the staged weights
+    const float zo = mlp_chain_infer(s, mlp_pack_inputs(xin));
     if (act != ACT_SKIP) {
         if (act == ACT_COLLIDE) {
             CollideParams<R> cp = collide_params<R>(p, y);

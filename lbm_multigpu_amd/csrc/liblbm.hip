@@ -653,20 +653,25 @@ static void launch_sc(int model, const StepParams& p, int blocks, hipStream_t s)
     else launch_sc_policy<S, false>(model, p, blocks, s);
 }
 
+// Launches the fused closure collide over level L; returns its workgroup
+// count (the number of force partials it wrote).
 template <class S>
-static void launch_closure(lbm_ctx* ctx, const StepParams& p, int blocks, float* omega_out)
+static long long launch_closure(lbm_ctx* ctx, const StepParams& p, const Level& L, float* omega_out)
 {
+    const long long segs = (long long)L.nsegw * L.ny * L.nz;
+    const int blocks = (int)((segs + CLOSURE_NW - 1) / CLOSURE_NW);
     ClosureArgs ca;
     ca.omega_out = omega_out;
     ca.chain = ctx->mlp->chain;
     ca.dwall = ctx->dwall;
     ca.nrm = ctx->mlp->nrm;
     ca.nu_cap = ctx->nu_cap;
-    const dim3 blk(Shape<S>::THREADS), grd(blocks);
+    const dim3 blk(64 * CLOSURE_NW), grd(blocks);
     if (ctx->cfg.model == LBM_MODEL_MRT_DHUMIERES)
         hipLaunchKernelGGL((k_stream_collide_closure<MODEL_MRT_DHUM, S>), grd, blk, 0, ctx->stream, p, ca);
     else
         hipLaunchKernelGGL((k_stream_collide_closure<MODEL_MRT_LES, S>), grd, blk, 0, ctx->stream, p, ca);
+    return blocks;
 }
 
 // One coarse-level stream-collide launch (update_inn/top/bot on one GPU),
@@ -678,14 +683,15 @@ static int coarse_step(lbm_ctx* ctx)
     StepParams p = step_params(ctx, L, c.omega);
     const bool rec = c.force_steps > 0 && ctx->t >= c.startf && (ctx->t - c.startf) < c.force_steps;
     if (rec) p.fpart = ctx->fpart;
+    long long nparts = L.blocks; // force partials written: one per workgroup
     with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         if (!ctx->mlp) {
             launch_sc<S>(c.model, p, L.blocks, ctx->stream);
         } else if (ctx->closure_every <= 1) {
-            launch_closure<S>(ctx, p, L.blocks, nullptr);
+            nparts = launch_closure<S>(ctx, p, L, nullptr);
         } else if ((ctx->t - ctx->closure_t0) % ctx->closure_every == 0) {
-            launch_closure<S>(ctx, p, L.blocks, ctx->omega_field); // refresh: network + rates for the next steps
+            nparts = launch_closure<S>(ctx, p, L, ctx->omega_field); // refresh: network + rates for the next steps
         } else {
             StepParams q = p;
             q.omega_node = ctx->omega_field;
@@ -695,8 +701,8 @@ static int coarse_step(lbm_ctx* ctx)
     });
     if (rec) {
         const long long slot = ctx->t - c.startf;
-        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, ctx->stream, ctx->fpart,
-                           (long long)L.blocks, 3, ctx->fhist + slot * 3); // one partial per workgroup
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, ctx->stream, ctx->fpart, nparts, 3,
+                           ctx->fhist + slot * 3);
         ctx->force_recorded = (int)std::max<long long>(ctx->force_recorded, slot + 1);
     }
     std::swap(L.fa, L.fb);

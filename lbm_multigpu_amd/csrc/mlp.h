@@ -210,11 +210,27 @@ __device__ __forceinline__ void chain_act(const f32x4& acc, __bf16 h[4])
     }
 }
 
-// The network for the 64 nodes of one wavefront.  xb: LDS [64][16] BF16 bits
-// of the standardised inputs, one row per node (lane l's node is row l).
-// Returns the output pre-activation of this lane's node.  All 64 lanes must
-// take part.
-__device__ __forceinline__ float mlp_chain_infer(const MlpChain& c, const short* xb)
+// The standardised inputs of one node as BF16 pairs: word k holds input 2k
+// in its low half and input 2k+1 in its high half (input 7 is padding, 0).
+struct MlpIn {
+    uint32_t d[4];
+};
+__device__ __forceinline__ MlpIn mlp_pack_inputs(const float x[MLP_IN])
+{
+    MlpIn in;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t lo = (uint16_t)bf(x[2 * k]);
+        const uint32_t hi = 2 * k + 1 < MLP_IN ? (uint16_t)bf(x[2 * k + 1]) : 0u;
+        in.d[k] = lo | (hi << 16);
+    }
+    return in;
+}
+
+// The network for the 64 nodes of one wavefront; lane l passes the inputs
+// of its own node.  Returns the output pre-activation of this lane's node.
+// All 64 lanes must take part.
+__device__ __forceinline__ float mlp_chain_infer(const MlpChain& c, const MlpIn& in)
 {
     const int lane = threadIdx.x & 63;
     const int q = lane >> 4, r = lane & 15;
@@ -224,7 +240,15 @@ __device__ __forceinline__ float mlp_chain_infer(const MlpChain& c, const short*
         // keeps the weight reads inside the loop (re-read from LDS per node
         // block) instead of 128 VGPRs of hoisted weights
         asm volatile("" ::: "memory");
-        const bf16x4 xin = *reinterpret_cast<const bf16x4*>(xb + (16 * nt + r) * MLP_INP + 4 * q);
+        // first layer's B operand: lane (q, r) takes inputs 4q..4q+3 of node
+        // 16 nt + r, gathered across lanes (inputs 8..15 are the zero padding)
+        const int src = (16 * nt + r) * 4;
+        const uint32_t a0 = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)in.d[0]);
+        const uint32_t a1 = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)in.d[1]);
+        const uint32_t a2 = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)in.d[2]);
+        const uint32_t a3 = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)in.d[3]);
+        const u32x2 xw = q == 0 ? u32x2{a0, a1} : (q == 1 ? u32x2{a2, a3} : u32x2{0u, 0u});
+        const bf16x4 xin = __builtin_bit_cast(bf16x4, xw);
         f32x4 acc[4];
         bf16x8 hb[2];
 #pragma unroll

@@ -336,6 +336,39 @@ def test_closure_out_of_range_is_smagorinsky():
     assert np.abs(g - h).max() <= 1e-6 * np.abs(h).max()
 
 
+@pytest.mark.parametrize("storage, bar", [("f16", 1e-3), ("bf16", 1e-2)])
+def test_closure_on_16bit_storage_tracks_fp32(storage, bar):
+    """The fused closure collide with 16-bit population storage against the
+    same run in FP32 (pinned against the oracle above): the cylinder channel,
+    20 steps with the network refreshed every step and every 3 steps, per-step
+    forces recorded.  Velocity within the storage's precision bar (as in the
+    storage precision tests); forces within 10 x that bar of max|F|, which
+    also checks that the reduction reads exactly the partials the closure
+    launch wrote (its workgroups differ from the 16-bit plain kernel's)."""
+    say("closure on", storage)
+    m = L.Mlp(seed=1)
+    big = np.full(7, 3e38, np.float32)
+    m.set_norm(np.zeros(7, np.float32), np.ones(7, np.float32), -big, big, 1e-3)
+    for every in (1, 3):
+        out = {}
+        for st in ("f32", storage):
+            cc = cfgmod.channel(nx=64, ny=32, nz=24, side=6, x_center=20, tau=0.52, storage=st, force_steps=20)
+            with L.Solver(cc.to_c()) as s:
+                s.set_geometry(None)
+                s.init_equilibrium(L.INIT_BGK)
+                s.set_closure(m, every=every)
+                s.step(20)
+                rho, u, v, w = s.moments()
+                out[st] = (np.stack([u, v, w]), s.forces())
+        (u32, F32), (u16, F16) = out["f32"], out[storage]
+        du = np.sqrt(((u16 - u32) ** 2).sum() / (u32 ** 2).sum())
+        dF = np.abs(F16 - F32).max() / np.abs(F32).max()
+        say("  every", every, "velocity", du, "forces", dF, "max|F|", np.abs(F32).max())
+        assert F32.shape == (20, 3) and np.abs(F32).max() > 0
+        assert du < bar
+        assert dF < 10 * bar
+
+
 def test_closure_rejects_unsupported_contexts():
     m = L.Mlp()
     c = L.default_config()
