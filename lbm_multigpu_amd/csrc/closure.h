@@ -190,8 +190,8 @@ __global__ void k_mlp_sync_bf16(const float* params, short* wbf)
 }
 
 // Inference on raw inputs X[n][7] -> nu[n] (256 threads: 4 wavefronts of 64),
-// the same register chain the fused collide runs.
-__global__ void __launch_bounds__(256) k_mlp_forward(const float* X, long long n, const MlpChain* chain,
+// the same register chain the fused collide runs (launch bound as there).
+__global__ void __launch_bounds__(256, 2) k_mlp_forward(const float* X, long long n, const MlpChain* chain,
                                                      MlpNorm nrm, float* nu)
 {
     __shared__ MlpChain s;
@@ -309,10 +309,12 @@ struct ClosureArgs {
 // training range.
 // Four wavefronts per workgroup for every storage type: the 19.5 KB of
 // staged weights are shared by all of them, and the LDS is what bounds the
-// kernel's occupancy.
+// kernel's occupancy.  The launch bound of at least two waves per SIMD caps
+// the registers at 256, so the compiler keeps the matrix-core accumulators
+// in ordinary VGPRs (no accumulator-register copy per hidden unit).
 constexpr int CLOSURE_NW = 4;
 template <int MODEL, class S>
-__global__ void __launch_bounds__(64 * CLOSURE_NW) k_stream_collide_closure(StepParams p, ClosureArgs ca)
+__global__ void __launch_bounds__(64 * CLOSURE_NW, 2) k_stream_collide_closure(StepParams p, ClosureArgs ca)
 {
     using R = typename Arith<S>::R;
     __shared__ MlpChain s;

@@ -140,6 +140,8 @@ __device__ __forceinline__ void mlp_stage(MlpLds& s, const short* wbf, const flo
 }
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 // Weights arranged for the register chain (built by k_mlp_pack_chain):
 // A-operand lane l = 16 q + r of output block mt holds
@@ -200,13 +202,17 @@ __device__ __forceinline__ void chain_stage(MlpChain& s, const MlpChain* g)
 }
 
 // z = BF16(acc), h = BF16(GELU(z)) for the four units this lane holds; the
-// accumulator started from the layer's bias, so acc = b + W h
+// accumulator started from the layer's bias, so acc = b + W h.  z is
+// rounded two units at a time (one packed conversion, then each half
+// widened back by a shift or a mask).
 __device__ __forceinline__ void chain_act(const f32x4& acc, __bf16 h[4])
 {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const float z = (float)(__bf16)acc[j];
-        h[j] = (__bf16)gelu(z);
+    for (int j = 0; j < 4; j += 2) {
+        const f32x2 a = {acc[j], acc[j + 1]};
+        const uint32_t zb = __builtin_bit_cast(uint32_t, __builtin_convertvector(a, bf16x2));
+        h[j] = (__bf16)gelu(__builtin_bit_cast(float, zb << 16));
+        h[j + 1] = (__bf16)gelu(__builtin_bit_cast(float, zb & 0xffff0000u));
     }
 }
 
