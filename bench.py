@@ -112,8 +112,9 @@ def cpu_baseline(cfg, budget_s=12.0):
 
 
 def copy_calibration(nbytes=637_534_208, reps=20):
-    """Device-to-device copy of nbytes (one FP32 lattice of the workload: the
-    stream-collide kernel reads and writes this much per launch), timed with
+    """Device-to-device copy of nbytes (the 19 x 4 B of FP32 populations of
+    the workload's 512x128x128 nodes: the stream-collide kernel reads and
+    writes this much per launch), timed with
     HIP events: the practical HBM ceiling, and the reference kernel for
     scaling rocprof's FETCH_SIZE/WRITE_SIZE counters."""
     import ctypes as C

@@ -905,7 +905,7 @@ extern "C" int lbm_step(lbm_ctx* ctx, int nsteps)
     if (!ctx->initialised) return fail(ctx, LBM_ESTATE, "lbm_step: call lbm_init_equilibrium first");
     if (nsteps < 0) return fail(ctx, LBM_EINVAL, "lbm_step: nsteps < 0");
     for (int s = 0; s < nsteps; ++s) {
-        coarse_step(ctx);
+        if (int rc = coarse_step(ctx)) return rc;
         if (int rc = inlet_control(ctx, ctx->t)) return rc;
         if (ctx->nlevels > 1) {
             // coarse t -> t+1 done; fine sub-steps t -> t+1/2 -> t+1 (main(),
