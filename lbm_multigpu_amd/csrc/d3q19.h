@@ -75,11 +75,18 @@ constexpr unsigned char NODE_OUTLET_EAST = 4;
 constexpr unsigned char NODE_LID_WEST_REG = 5;
 constexpr unsigned char NODE_SYM_XMIN = 6;
 constexpr unsigned char NODE_SYM_XMAX = 7;
+// Flags the host adds to a node's code in the device mask: NODE_NEAR, the
+// node has a link that bounces (out through a wall face, or from a solid
+// under link bounce-back); NODE_NEAR_SOLID, one of those links comes from a
+// solid node, so the kernel must read the neighbours' codes (without it the
+// bounced links follow from the domain faces alone).
 constexpr unsigned char NODE_NEAR = 0x80;
+constexpr unsigned char NODE_NEAR_SOLID = 0x40;
+constexpr unsigned char NODE_CODE_MASK = 0x3f;
 
 __host__ __device__ inline bool is_solid(unsigned char code)
 {
-    code &= 0x7f;
+    code &= NODE_CODE_MASK;
     return code == NODE_SOLID || code == NODE_SOLID_FORCE;
 }
 

@@ -210,13 +210,15 @@ __device__ __forceinline__ int node_fix(const StepParams& p, int x, int y, int z
     using R = typename Arith<S>::R;
     const S* __restrict__ fin = static_cast<const S*>(p.fin);
     const long long qs = p.qs;
-    const unsigned char base = code & 0x7f;
+    const unsigned char base = code & NODE_CODE_MASK;
     if (p.halfway && is_solid(base)) return ACT_SKIP; // inert under link bounce-back
 
     if (code & NODE_NEAR) {
         // links that bounce: out of the domain through a wall face, or (link
         // mode) from a solid node; the population comes back reversed from
-        // this node's own previous post-collision value.
+        // this node's own previous post-collision value.  The neighbours'
+        // codes are read only where a solid neighbour exists.
+        const bool solid_nb = p.halfway && (code & NODE_NEAR_SOLID);
         int xm, xp, ym, yp, zm, zp;
         bool oxm, oxp, oym, oyp, ozm, ozp;
         axis_nb(x, p.nx, p.bcx, xm, xp, oxm, oxp);
@@ -230,8 +232,8 @@ __device__ __forceinline__ int node_fix(const StepParams& p, int x, int y, int z
             const bool out = (CX[i] > 0 && oxm) || (CX[i] < 0 && oxp) || (CY[i] > 0 && oym) ||
                              (CY[i] < 0 && oyp) || (CZ[i] > 0 && ozm) || (CZ[i] < 0 && ozp);
             unsigned char sc = 0;
-            if (!out && p.halfway)
-                sc = *at(p.mask, p.nxp * (Y[1 - CY[i]] + p.ny * Z[1 - CZ[i]]) + X[1 - CX[i]]) & 0x7f;
+            if (!out && solid_nb)
+                sc = *at(p.mask, p.nxp * (Y[1 - CY[i]] + p.ny * Z[1 - CZ[i]]) + X[1 - CX[i]]) & NODE_CODE_MASK;
             const bool bounce = out || is_solid(sc);
             if (bounce) {
                 const R v = Store<S>::load(at(fin + OPP[i] * qs, cell), OPP[i], p.sh);

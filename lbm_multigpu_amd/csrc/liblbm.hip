@@ -422,7 +422,8 @@ static unsigned char classify(const lbm_config& c, int x, int y, int z)
 }
 
 // Device mask for one level: codes plus the NEAR bit on nodes with a link
-// that bounces (out through a wall face, or from a solid node in link mode).
+// that bounces (out through a wall face, or from a solid node in link mode),
+// and NEAR_SOLID where one of those links comes from a solid node.
 static void build_device_mask(const Level& L, const int bc[3], int halfway, std::vector<unsigned char>& dev)
 {
     dev.assign((size_t)L.qs, 0);
@@ -432,9 +433,9 @@ static void build_device_mask(const Level& L, const int bc[3], int halfway, std:
             for (int x = 0; x < L.nx; ++x) {
                 const size_t c = (size_t)x + (size_t)L.nx * (y + (size_t)L.ny * z);
                 unsigned char code = L.host_mask[c];
-                bool near = false;
+                bool near = false, near_solid = false;
                 const int p[3] = {x, y, z};
-                for (int i = 1; i < Q && !near; ++i) {
+                for (int i = 1; i < Q && !near_solid; ++i) {
                     const int cc[3] = {CX[i], CY[i], CZ[i]};
                     int s[3];
                     bool out = false;
@@ -449,10 +450,11 @@ static void build_device_mask(const Level& L, const int bc[3], int halfway, std:
                     if (out) near = true;
                     else if (halfway && !is_solid(code)) {
                         const size_t sc = (size_t)s[0] + (size_t)L.nx * (s[1] + (size_t)L.ny * s[2]);
-                        if (is_solid(L.host_mask[sc])) near = true;
+                        if (is_solid(L.host_mask[sc])) near = near_solid = true;
                     }
                 }
-                dev[(size_t)x + (size_t)L.nxp * (y + (size_t)L.ny * z)] = code | (near ? NODE_NEAR : 0);
+                dev[(size_t)x + (size_t)L.nxp * (y + (size_t)L.ny * z)] =
+                    code | (near ? NODE_NEAR : 0) | (near_solid ? NODE_NEAR_SOLID : 0);
             }
 }
 
