@@ -107,6 +107,98 @@ def test_cf_fc_reproduce_uniform_and_linear_fields():
         assert np.all(back[:, :, :, : r.a[0] + 2] == 0)
 
 
+def _moments_with_strain(u, S):
+    """Physical moments (rho = 1, j = u) whose StrainRate is the symmetric,
+    traceless S: the deviatoric non-equilibrium stress -S/1.5 written into
+    3pixx, piww, pixy, piyz, pixz on top of the equilibrium's u u parts."""
+    P = -S / 1.5
+    ux, uy, uz = u
+    return np.array([1.0, ux, uy, uz,
+                     2 * P[0, 0] - P[1, 1] - P[2, 2] + 2 * ux * ux - (uy * uy + uz * uz),
+                     P[1, 1] - P[2, 2] + uy * uy - uz * uz,
+                     P[0, 1] + ux * uy, P[1, 2] + uy * uz, P[0, 2] + ux * uz])
+
+
+def _order2_at(vel, sgrad, pts):
+    """Trilinear interpolation of the 8 corner moments of the unit cube plus
+    the ORDER-2 correction, at fine positions pts; returns (got, exact) j."""
+    L = O._bind_refine()
+    corners = [np.array([k & 1, (k >> 1) & 1, (k >> 2) & 1], float) for k in range(8)]
+    c = np.ascontiguousarray([_moments_with_strain(vel(X), sgrad(X)) for X in corners], np.float32)
+    got, exact = [], []
+    for p in pts:
+        w = np.array([np.prod([p[a] if X[a] else 1 - p[a] for a in range(3)]) for X in corners])
+        mom = (w @ c.astype(np.float64)).astype(np.float32)
+        L.orc_order2_correction_f32(c, mom, *(float(v) for v in p), 0)
+        got.append(mom[1:4].copy())
+        exact.append(vel(np.asarray(p, float)))
+    return np.array(got), np.array(exact)
+
+
+def test_order2_reproduces_divergence_free_quadratic_fields():
+    """ORDER 2 (MRTtest/DynInlet3.cu:2312-2357): the trilinear momentum plus
+    the correction built from the corners' strain rates and velocity
+    differences reproduces every divergence-free quadratic velocity field
+    exactly when the corners carry its exact strain rate.  Each field below
+    exercises one coefficient (cy, cz, dx, dz, ex, ey through the off-
+    diagonal rates and the velocity differences; cx, dy, ez through the
+    diagonal ones), so a wrong sign or index in any of them fails here."""
+    a = 0.02
+    e = np.zeros((3, 3))
+
+    def grad_field(Gfn):
+        return lambda X: 0.5 * (Gfn(X) + Gfn(X).T)
+
+    def G(entries):
+        def f(X):
+            g = e.copy()
+            for (i, j), fn in entries.items():
+                g[i, j] = fn(X)
+            return g
+        return f
+
+    fields = [
+        (lambda X: np.array([a * X[1] ** 2, 0, 0]), G({(0, 1): lambda X: 2 * a * X[1]})),
+        (lambda X: np.array([a * X[2] ** 2, 0, 0]), G({(0, 2): lambda X: 2 * a * X[2]})),
+        (lambda X: np.array([0, a * X[0] ** 2, 0]), G({(1, 0): lambda X: 2 * a * X[0]})),
+        (lambda X: np.array([0, a * X[2] ** 2, 0]), G({(1, 2): lambda X: 2 * a * X[2]})),
+        (lambda X: np.array([0, 0, a * X[0] ** 2]), G({(2, 0): lambda X: 2 * a * X[0]})),
+        (lambda X: np.array([0, 0, a * X[1] ** 2]), G({(2, 1): lambda X: 2 * a * X[1]})),
+        (lambda X: np.array([a * X[0] ** 2, -2 * a * X[0] * X[1], 0]),
+         G({(0, 0): lambda X: 2 * a * X[0], (1, 0): lambda X: -2 * a * X[1], (1, 1): lambda X: -2 * a * X[0]})),
+        (lambda X: np.array([0, a * X[1] ** 2, -2 * a * X[1] * X[2]]),
+         G({(1, 1): lambda X: 2 * a * X[1], (2, 1): lambda X: -2 * a * X[2], (2, 2): lambda X: -2 * a * X[1]})),
+        (lambda X: np.array([-2 * a * X[0] * X[2], 0, a * X[2] ** 2]),
+         G({(2, 2): lambda X: 2 * a * X[2], (0, 2): lambda X: -2 * a * X[0], (0, 0): lambda X: -2 * a * X[2]})),
+    ]
+    pts = [(x, y, z) for x in (0.25, 0.75, 0.4) for y in (0.25, 0.75, 0.6) for z in (0.25, 0.75)]
+    for vel, Gfn in fields:
+        got, exact = _order2_at(vel, grad_field(Gfn), pts)
+        assert np.abs(got - exact).max() < 2e-9
+        # trilinear alone misses the curvature by up to a/4 x (1 - x) terms
+        assert np.abs(exact).max() > 0
+
+
+def test_order2_with_compressible_curvature_recovers_two_thirds():
+    """u = a x^2 alone: the strain rate the corners can carry is deviatoric
+    (StrainRate drops the trace), so S_xx = 2/3 du/dx and the correction
+    supplies 2/3 of the curvature: the residual is exactly a x (1 - x) / 3."""
+    a = 0.02
+
+    def vel(X):
+        return np.array([a * X[0] ** 2, 0, 0])
+
+    def sgrad(X):
+        g = np.diag([2 * a * X[0], 0.0, 0.0])
+        return g - np.trace(g) / 3 * np.eye(3)
+
+    pts = [(x, 0.5, 0.5) for x in (0.25, 0.75, 0.4)]
+    got, exact = _order2_at(vel, sgrad, pts)
+    for p, g, ex in zip(pts, got, exact):
+        assert abs((g[0] - ex[0]) - a * p[0] * (1 - p[0]) / 3) < 2e-9
+        assert abs(g[1]) < 1e-9 and abs(g[2]) < 1e-9
+
+
 def test_temporal_midpoint_is_exact():
     """alpha = 1/2 blends the stored coarse moments (linear in time) before
     the nonlinear rescale: equal to CF of the averaged moments, and different
