@@ -1,0 +1,81 @@
+"""The bench's channel at its full size (512x128x128, square cylinder side 16,
+MRT + Smagorinsky), checked through properties that do not depend on an
+oracle run of that size:
+
+* mirror symmetry: without the inlet perturbation the case is symmetric under
+  y -> ny-1-y and under z -> nz-1-z (rod centred at (ny-1)/2, spanning z;
+  halfway walls on both faces), so after 200 steps u and w are even and v
+  odd in y (u, v even and w odd in z) to FP32 rounding: 1e-5 of max|u|;
+* 16-bit storage: FP16 populations with the FP32 collide against the FP64
+  run of the same case, relative velocity error <= 1e-3 (BF16: 1e-2), the
+  bars the storage format is held to;
+* conservation: with the seeded inlet the FP64 mass sums of the FP32 and
+  FP64 runs differ by less than 1e-6 relative after 200 steps (the open
+  inlet and outlet change the mass of both runs alike; measured 4e-8).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import lbm_multigpu_amd as L
+from lbm_multigpu_amd import config as cfgmod
+
+pytestmark = pytest.mark.gpu
+STEPS = 200
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _run(storage, amp):
+    c = cfgmod.channel(storage=storage, amp=amp).to_c()
+    with L.Solver(c) as s:
+        s.set_geometry(None)
+        mask = s.get_mask()
+        s.init_equilibrium(L.INIT_BGK)
+        s.step(STEPS)
+        rho, u, v, w = s.moments()
+        mass = s.mass_momentum()[0]
+    return mask, rho, u, v, w, mass
+
+
+def _rel_vel(got, ref):
+    scale = np.sqrt(sum(r.astype(np.float64) ** 2 for r in ref)).max()
+    return max(np.abs(g - r).max() for g, r in zip(got, ref)) / scale
+
+
+def test_full_size_channel_is_mirror_symmetric():
+    say("full-size channel, symmetry")
+    mask, rho, u, v, w, _ = _run("f32", 0.0)
+    assert np.array_equal(mask, mask[:, ::-1, :]) and np.array_equal(mask, mask[::-1, :, :])
+    fl = mask == L.NODE_FLUID
+    scale = np.sqrt(u.astype(np.float64) ** 2 + v ** 2 + w ** 2)[fl].max()
+    # y mirror (axis 1 of [z][y][x]): u, w even, v odd
+    ey = max(np.abs(u - u[:, ::-1, :])[fl].max(), np.abs(v + v[:, ::-1, :])[fl].max(),
+             np.abs(w - w[:, ::-1, :])[fl].max()) / scale
+    # z mirror (axis 0): u, v even, w odd
+    ez = max(np.abs(u - u[::-1])[fl].max(), np.abs(v - v[::-1])[fl].max(), np.abs(w + w[::-1])[fl].max()) / scale
+    say(f"  max|u| {scale:.4f}, asymmetry y {ey:.2e}, z {ez:.2e}")
+    assert np.isfinite(scale) and scale > 0.04
+    assert ey < 1e-5 and ez < 1e-5
+
+
+@pytest.mark.parametrize("storage,tol", [("f16", 1e-3), ("bf16", 1e-2)])
+def test_full_size_16bit_storage_vs_fp64(storage, tol):
+    say("full-size channel,", storage, "vs f64")
+    _, _, u6, v6, w6, _ = _run("f64", 1e-3)
+    _, _, u, v, w, _ = _run(storage, 1e-3)
+    err = _rel_vel((u, v, w), (u6, v6, w6))
+    say(f"  {storage}: velocity error vs FP64 {err:.3e} (bar {tol})")
+    assert err < tol
+
+
+def test_full_size_fp32_mass_tracks_fp64():
+    say("full-size channel, mass")
+    *_, m32 = _run("f32", 1e-3)
+    *_, m64 = _run("f64", 1e-3)
+    say(f"  mass FP32 {m32:.9e}, FP64 {m64:.9e}")
+    assert abs(m32 - m64) < 1e-6 * abs(m64)
