@@ -214,21 +214,31 @@ def run(scale=1.0, spinup_fine=16000, snapshots=400, every=100, keep=0.02, train
            "apriori": apriori, "training": {"steps": train_steps, "loss_first": float(hist[:50].mean()),
                                             "loss_last": float(hist[-50:].mean())},
            "norm": norm, "runs": runs, "wall_s": time.time() - t0, "eval_cells": int(region.sum())}
-    # GPU time Smagorinsky needs for the learned run's error (log-log
-    # interpolation of its error against its GPU time)
     pts = sorted((v["gpu_ms"], v["wake_l2"]) for k, v in runs.items()
                  if k.startswith("smagorinsky_r") or k == "coarse_smagorinsky")
     target = runs[cases[0][0]]["wake_l2"]
     tms = [p[0] for p in pts]
     errs = [p[1] for p in pts]
-    res["smagorinsky_ms_at_learned_error"] = None
-    for (t1, e1), (t2, e2) in zip(pts, pts[1:]):  # the cheapest crossing
-        if (e1 - target) * (e2 - target) <= 0 and e1 != e2:
-            a = (math.log(target) - math.log(e1)) / (math.log(e2) - math.log(e1))
-            res["smagorinsky_ms_at_learned_error"] = math.exp(math.log(t1) + a * (math.log(t2) - math.log(t1)))
-            break
+    res["smagorinsky_ms_at_learned_error"], res["smagorinsky_cheapest_meets_learned_error"] = \
+        time_for_error(tms, errs, target)
     res["smagorinsky_curve"] = {"gpu_ms": tms, "wake_l2": errs}
     return res
+
+
+def time_for_error(tms, errs, target):
+    """GPU time a resolution curve (points sorted by GPU time) needs to
+    reach an error of target: the cheapest point's time when that point
+    already meets it (second value True), else the log-log interpolation at
+    the cheapest crossing from above, else None."""
+    if not tms or not all(math.isfinite(e) for e in errs):
+        return None, False
+    if errs[0] <= target:
+        return tms[0], True
+    for (t1, e1), (t2, e2) in zip(zip(tms, errs), zip(tms[1:], errs[1:])):
+        if e1 > target >= e2:
+            a = (math.log(target) - math.log(e1)) / (math.log(e2) - math.log(e1))
+            return math.exp(math.log(t1) + a * (math.log(t2) - math.log(t1))), False
+    return None, False
 
 
 def main(argv=None):

@@ -54,3 +54,24 @@ def test_channel_family_keeps_physical_set_up():
         assert c.obst1_r * 2 == round(16 * r)
         nu = (1 / c.omega - 0.5) / 3
         assert nu == pytest.approx(0.02 / 3 * r, rel=1e-9)
+
+
+def test_time_for_error_on_a_resolution_curve():
+    """Cheapest GPU time at which the curve reaches a target error: a crossing
+    from above is interpolated log-log (exact for a power law), a cheapest
+    point already at or below the target is returned as is, and an unreached
+    target gives None; a non-monotone curve uses its cheapest crossing from
+    above, not a later rise through the target."""
+    tms = [1.0, 2.0, 4.0, 8.0]
+    errs = [0.08 * t ** -0.5 for t in tms]                      # power law e = 0.08 t^-1/2
+    t, cheapest = CL.time_for_error(tms, errs, 0.08 * 3.0 ** -0.5)
+    assert not cheapest and abs(t - 3.0) < 1e-12
+    assert CL.time_for_error(tms, errs, 0.09) == (1.0, True)
+    assert CL.time_for_error(tms, errs, 0.01) == (None, False)
+    # the cheapest point already meets the target; a later bump above it
+    # and back down is not a crossing that counts
+    t, cheapest = CL.time_for_error([1.0, 1.5, 2.4], [0.19, 0.23, 0.14], 0.228)
+    assert cheapest and t == 1.0
+    t, cheapest = CL.time_for_error([1.0, 1.5, 2.4], [0.24, 0.26, 0.14], 0.2)
+    assert not cheapest and 1.5 < t < 2.4
+    assert CL.time_for_error([1.0, 2.0], [float("nan"), 0.1], 0.2) == (None, False)
