@@ -179,6 +179,33 @@ def test_cpu3d_case_matches_reference_program():
     assert rel(rho, r2) < 1e-5 and rel_vel((u, v, w), (u2, v2, w2)) < 1e-5
 
 
+def test_cpu3d_case_1000_steps_matches_oracle():
+    """The same case run 1000 steps (the reference program stops at its
+    TMAX of 100, where the oracle is pinned to its output above): GPU FP32
+    against the FP32 oracle, density and velocity within 1e-5 relative; the
+    FP64 oracle shows how far both FP32 runs are from exact arithmetic."""
+    say("cpu3d case, 1000 steps")
+    cc = cfgmod.cpu3d_case()
+    c = cc.to_c()
+    T = 1000
+    with L.Solver(c) as s:
+        s.set_geometry(None)
+        exp = np.vectorize(NODE_TO_ORC.get)(s.get_mask()).astype(np.uint8)
+        s.init_equilibrium(L.INIT_BGK)
+        f0 = s.populations()
+        s.step(T)
+        rho, u, v, w = s.moments()
+    p = orc_params(c)
+    fo, _ = O.run(p, f0, T, mask=exp)
+    r2, u2, v2, w2 = O.macro(p, fo)
+    fo64, _ = O.run(p, f0.astype(np.float64), T, mask=exp)
+    r6, u6, v6, w6 = O.macro(p, fo64)
+    e_rho, e_u = rel(rho, r2), rel_vel((u, v, w), (u2, v2, w2))
+    say(f"  GPU vs FP32 oracle: rho {e_rho:.2e}, u {e_u:.2e}; vs FP64 oracle: u GPU {rel_vel((u, v, w), (u6, v6, w6)):.2e},"
+        f" oracle FP32 {rel_vel((u2, v2, w2), (u6, v6, w6)):.2e}")
+    assert e_rho < 1e-5 and e_u < 1e-5
+
+
 @pytest.mark.parametrize("rates", ["lrmgpu", "dhumieres"])
 def test_channel_mrt_les_matches_oracle(rates):
     """Channel with a square cylinder, halfway walls at y/z, West_Extrap inlet
