@@ -154,6 +154,9 @@ def main():
     ap.add_argument("--closure-every", type=int, default=1,
                     help="closure workload: evaluate the network every N steps")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the FP16-storage secondary figure")
+    ap.add_argument("--forces", action="store_true",
+                    help="record the momentum-exchange force every step (timed with the steps)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain device copy of one lattice instead (HBM ceiling / counter scale)")
     a = ap.parse_args()
@@ -168,34 +171,48 @@ def main():
     import lbm_multigpu_amd as L
     from lbm_multigpu_amd import config as C
 
-    if a.workload == "refined":
-        cfg = C.refined_channel(storage=a.storage)
-    elif a.workload == "closure":
-        cfg = C.channel(nx=256, ny=64, nz=64, side=8, x_center=64, tau=0.51, storage=a.storage)
-    else:
-        cfg = C.channel(storage=a.storage)
-    mlp = None
-    with L.Solver(cfg.to_c()) as s:
-        s.set_geometry(None)
-        s.init_equilibrium(L.INIT_BGK)
+    def make_cfg(storage):
+        if a.workload == "refined":
+            return C.refined_channel(storage=storage)
         if a.workload == "closure":
-            # a seeded network over the whole input range: every fluid node
-            # runs the network (the cost does not depend on the weights)
-            import numpy as np
-            mlp = L.Mlp(seed=1)
-            big = np.full(7, 3e38, np.float32)
-            mlp.set_norm(np.zeros(7, np.float32), np.ones(7, np.float32), -big, big, 1e-4)
-            s.set_closure(mlp, every=a.closure_every)
-        s.step(a.warmup)
-        s.step_timed(1)  # drains the warm-up
-        t0 = time.perf_counter()
-        ms = s.step_timed(a.steps)
-        wall = time.perf_counter() - t0
-        mm = s.mass_momentum()
-        fine = s.sizes(1) if a.workload == "refined" else None
+            return C.channel(nx=256, ny=64, nz=64, side=8, x_center=64, tau=0.51, storage=storage)
+        return C.channel(storage=storage)
+
+    def measure(cfg, forces):
+        """(ms per step from HIP events on the library's stream, wall s, FP64
+        mass sum, fine-level sizes or None)"""
+        if forces:
+            # record the momentum-exchange force of every step, as the
+            # reference does from STARTF on
+            cfg.force_steps = a.warmup + a.steps + 2
+            cfg.startf = 0
+        with L.Solver(cfg.to_c()) as s:
+            s.set_geometry(None)
+            s.init_equilibrium(L.INIT_BGK)
+            mlp = None
+            if a.workload == "closure":
+                # a seeded network over the whole input range: every fluid node
+                # runs the network (the cost does not depend on the weights)
+                import numpy as np
+                mlp = L.Mlp(seed=1)
+                big = np.full(7, 3e38, np.float32)
+                mlp.set_norm(np.zeros(7, np.float32), np.ones(7, np.float32), -big, big, 1e-4)
+                s.set_closure(mlp, every=a.closure_every)
+            s.step(a.warmup)
+            s.step_timed(1)  # drains the warm-up
+            t0 = time.perf_counter()
+            ms = s.step_timed(a.steps)
+            wall = time.perf_counter() - t0
+            mm = s.mass_momentum()
+            fine = s.sizes(1) if a.workload == "refined" else None
+            if forces:
+                assert len(s.forces()) >= a.steps
+        return ms / a.steps, wall, mm, fine
+
+    cfg = make_cfg(a.storage)
+    ms_step, wall, mm, fine = measure(cfg, a.forces)
     cells = cfg.nx * cfg.ny * cfg.nz
     updates = cells + (2 * fine.nx * fine.ny * fine.nz if fine is not None else 0)
-    ms_step = ms / a.steps
     glups = updates / (ms_step * 1e-3) / 1e9
     traffic = None
     if a.workload == "closure":
@@ -209,6 +226,18 @@ def main():
                 traffic = json.load(fh).get("bytes_per_launch")
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic}
+    # the 16-bit storage variant of the same workload, measured in the same
+    # run as a secondary figure (76 algorithmic bytes per node update)
+    secondary = None
+    if a.workload == "channel" and a.storage == "f32" and not a.no_secondary and not a.forces:
+        sec = {}
+        for st in ("f16",):
+            ms2, _, _, _ = measure(make_cfg(st), False)
+            g2 = cells / (ms2 * 1e-3) / 1e9
+            ach2 = BYTES_PER_NODE[st] * cells / (ms2 * 1e-3) / 1e9
+            sec[st] = {"value": round(g2, 3), "unit": "GLUPS", "ms_per_step": round(ms2, 4),
+                       "roofline_frac": round(ach2 / HBM_PEAK_GBS, 4)}
+        secondary = sec
     cpu = None if (a.no_cpu or a.workload != "channel") else cpu_baseline(cfg)
     line = {
         "metric": METRIC, "value": round(glups, 3), "unit": "GLUPS", "n_gpus": 1, "steps": a.steps,
@@ -218,6 +247,10 @@ def main():
                    "storage": a.storage, "wall_s": round(wall, 3), "mass_sum": mm[0]},
         "roofline": roof, "cpu_baseline": cpu,
     }
+    if secondary:
+        line["secondary_storage"] = secondary
+    if a.forces:
+        line["config"]["forces_recorded_every_step"] = True
     print(json.dumps(line), flush=True)
 
 

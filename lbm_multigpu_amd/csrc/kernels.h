@@ -461,6 +461,32 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const double* part, lon
     }
 }
 
+// First stage of the two-stage reduction of many partials: workgroup b sums
+// partials [b*REDUCE_CHUNK, (b+1)*REDUCE_CHUNK) in a fixed order (strided
+// per thread, then a fixed tree) into out[b*ncomp + c]; k_reduce_partials
+// then sums the gridDim.x results.  A single workgroup over the 32 K force
+// partials of a 512x128x128 step took 93 us; the two stages spread the
+// reads over many CUs.
+constexpr int REDUCE_CHUNK = 512;
+__global__ void __launch_bounds__(256) k_reduce_chunks(const double* part, long long n, int ncomp, double* out)
+{
+    __shared__ double sh[256];
+    const long long base = (long long)blockIdx.x * REDUCE_CHUNK;
+    const long long end = base + REDUCE_CHUNK < n ? base + REDUCE_CHUNK : n;
+    for (int c = 0; c < ncomp; ++c) {
+        double s = 0;
+        for (long long i = base + threadIdx.x; i < end; i += 256) s += part[i * ncomp + c];
+        sh[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[(long long)blockIdx.x * ncomp + c] = sh[0];
+        __syncthreads();
+    }
+}
+
 // AverageV (MRTtest/DynInlet3.cu:2561-2600): v = sum_i c_iy f_i at each node
 // of the plane y, summed in the reference's order, then over the 64-node
 // row segment of the wavefront in FP64; one partial per (segment, z).
@@ -483,44 +509,55 @@ __global__ void __launch_bounds__(64) k_plane_v(const S* f, long long qs, int nx
     if (lane_id() == 0) part[blockIdx.x] = v;
 }
 
-// Per-block FP64 sums of (rho, jx, jy, jz) over non-solid nodes.
+// FP64 sums of (rho, jx, jy, jz) over non-solid nodes, one partial per
+// workgroup.  MM_NW-wavefront workgroups stride over the groups of MM_NW row
+// segments (workgroup g takes groups g, g + G, g + 2G, ... for a launch of G
+// workgroups), each lane accumulating its nodes in that order; then the
+// wavefront sums and the wavefronts in a fixed order.  The partial count is
+// the launch's G whatever the lattice size, and the result is deterministic
+// for a given G.
+constexpr int MM_NW = 4;
+constexpr int MM_GRID_MAX = 2048; // workgroups (partials) of one mass/momentum launch
+__host__ inline int mm_grid(int nsegw, int ny, int nz)
+{
+    const long long groups = ((long long)nsegw * ny * nz + MM_NW - 1) / MM_NW;
+    return (int)(groups < MM_GRID_MAX ? groups : MM_GRID_MAX);
+}
 template <class S>
-__global__ void __launch_bounds__(Shape<S>::THREADS) k_mass_momentum(const S* f, const unsigned char* mask,
-                                                        long long qs, int nx, int ny, int nxp,
-                                                        int nsegw, int nz, double* part, PopShift sh)
+__global__ void __launch_bounds__(64 * MM_NW) k_mass_momentum(const S* f, const unsigned char* mask,
+                                                              long long qs, int nx, int ny, int nxp,
+                                                              int nsegw, int nz, double* part, PopShift sh)
 {
     using R = typename Arith<S>::R;
-    const int lb = blockIdx.x;
-    int x, y, z;
-    constexpr int NW = Shape<S>::NW;
-    const bool row_ok = wave_node<NW>(lb, nsegw, ny, nz, x, y, z);
+    const long long groups = ((long long)nsegw * ny * nz + MM_NW - 1) / MM_NW;
     double acc[4] = {0, 0, 0, 0};
-    if (row_ok && x < nx) {
+    for (long long g = blockIdx.x; g < groups; g += gridDim.x) {
+        int x, y, z;
+        if (!wave_node<MM_NW>((int)g, nsegw, ny, nz, x, y, z) || x >= nx) continue;
         const int cell = x + nxp * (y + ny * z);
-        if (!(mask && is_solid(mask[cell]))) {
-            R r = 0, a = 0, b = 0, c = 0;
+        if (mask && is_solid(mask[cell])) continue;
+        R r = 0, a = 0, b = 0, c = 0;
 #pragma unroll
-            for (int i = 0; i < Q; ++i) {
-                const R v = Store<S>::to_real(f[i * qs + cell], i, sh);
-                r += v;
-                if (CX[i]) a += R(CX[i]) * v;
-                if (CY[i]) b += R(CY[i]) * v;
-                if (CZ[i]) c += R(CZ[i]) * v;
-            }
-            acc[0] = r; acc[1] = a; acc[2] = b; acc[3] = c;
+        for (int i = 0; i < Q; ++i) {
+            const R v = Store<S>::to_real(f[i * qs + cell], i, sh);
+            r += v;
+            if (CX[i]) a += R(CX[i]) * v;
+            if (CY[i]) b += R(CY[i]) * v;
+            if (CZ[i]) c += R(CZ[i]) * v;
         }
+        acc[0] += double(r); acc[1] += double(a); acc[2] += double(b); acc[3] += double(c);
     }
-    __shared__ double red[4][NW];
+    __shared__ double red[4][MM_NW];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const double s = wave_sum(acc[k]);
         if (lane_id() == 0) red[k][wave_id()] = s;
     }
     __syncthreads();
-    if (threadIdx.x < 4 && threadIdx.y == 0) {
+    if (threadIdx.x < 4) {
         double s = 0;
-        for (int w = 0; w < NW; ++w) s += red[threadIdx.x][w];
-        part[(long long)lb * 4 + threadIdx.x] = s;
+        for (int w = 0; w < MM_NW; ++w) s += red[threadIdx.x][w];
+        part[(long long)blockIdx.x * 4 + threadIdx.x] = s;
     }
 }
 

@@ -53,6 +53,7 @@ struct lbm_ctx {
     int nlevels = 1;
     float* u_in = nullptr;     // inlet profile [nz*ny]
     double* fpart = nullptr;   // force partials per block
+    double* fstage = nullptr;  // first-stage sums of the force partials (k_reduce_chunks)
     double* fhist = nullptr;   // [force_steps*3]
     double* red = nullptr;     // small reduction output
     double* mpart = nullptr;   // mass partials
@@ -343,7 +344,8 @@ extern "C" int lbm_create(const lbm_config* cfg, lbm_ctx** out)
     inlet_profile(cfg->ny, cfg->nz, cfg->umax, cfg->inlet_amp, cfg->seed, prof.data());
     if (hipMalloc(&ctx->u_in, prof.size() * sizeof(float)) != hipSuccess ||
         hipMalloc(&ctx->fpart, (size_t)L.blocks * 3 * sizeof(double)) != hipSuccess ||
-        hipMalloc(&ctx->mpart, (size_t)L.blocks * 4 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ctx->fstage, (size_t)(L.blocks / REDUCE_CHUNK + 1) * 3 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ctx->mpart, (size_t)std::max(L.blocks, MM_GRID_MAX) * 4 * sizeof(double)) != hipSuccess ||
         hipMalloc(&ctx->red, 8 * sizeof(double)) != hipSuccess)
         return bail(fail(nullptr, LBM_ENOMEM, "lbm_create: device allocation failed"));
     if (cfg->force_steps > 0) {
@@ -375,7 +377,7 @@ extern "C" void lbm_destroy(lbm_ctx* ctx)
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (int l = 0; l < 2; ++l) free_level(ctx->lv[l]);
-    for (void* p : {(void*)ctx->u_in, (void*)ctx->fpart, (void*)ctx->fhist, (void*)ctx->red, (void*)ctx->mpart,
+    for (void* p : {(void*)ctx->u_in, (void*)ctx->fpart, (void*)ctx->fstage, (void*)ctx->fhist, (void*)ctx->red, (void*)ctx->mpart,
                     (void*)ctx->Mold, (void*)ctx->Mnew, (void*)ctx->Pold, (void*)ctx->Pnew, (void*)ctx->Phalf,
                     (void*)ctx->dwall,
                     (void*)ctx->omega_field})
@@ -676,6 +678,18 @@ static long long launch_closure(lbm_ctx* ctx, const StepParams& p, const Level& 
     return blocks;
 }
 
+// Deterministic FP64 sum of nparts force partials (3 components) into out:
+// chunks of REDUCE_CHUNK partials per workgroup, then one workgroup over
+// the chunk sums.  Replaces the FX/FY/FZ atomicAdd of the reference
+// (MRTtest/DynInlet3.cu:1819-1838).
+static void reduce_force_partials(lbm_ctx* ctx, long long nparts, double* out)
+{
+    const long long chunks = (nparts + REDUCE_CHUNK - 1) / REDUCE_CHUNK;
+    hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)chunks), dim3(256), 0, ctx->stream, ctx->fpart, nparts, 3,
+                       ctx->fstage);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, ctx->stream, ctx->fstage, chunks, 3, out);
+}
+
 // One coarse-level stream-collide launch (update_inn/top/bot on one GPU),
 // plus the deterministic force reduction when this step is recorded.
 static int coarse_step(lbm_ctx* ctx)
@@ -703,8 +717,7 @@ static int coarse_step(lbm_ctx* ctx)
     });
     if (rec) {
         const long long slot = ctx->t - c.startf;
-        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, ctx->stream, ctx->fpart, nparts, 3,
-                           ctx->fhist + slot * 3);
+        reduce_force_partials(ctx, nparts, ctx->fhist + slot * 3);
         ctx->force_recorded = (int)std::max<long long>(ctx->force_recorded, slot + 1);
     }
     std::swap(L.fa, L.fb);
@@ -1059,29 +1072,32 @@ extern "C" int lbm_get_forces(lbm_ctx* ctx, double* fxyz, int max_steps, int* ns
     return LBM_OK;
 }
 
+// FP64 mass and momentum sums of one level, enqueued on stream s: the
+// grid-stride partial kernel (mm_grid workgroups, one partial each, into
+// part) and one workgroup summing them in a fixed order into out_dev[4].
+// Deterministic: the same lattice gives the same bits whatever part is.
+static int mass_momentum_launch(lbm_ctx* ctx, const Level& L, double* part, double* out_dev, hipStream_t s)
+{
+    const int G = mm_grid(L.nsegw, L.ny, L.nz);
+    with_storage(L.storage, [&](auto* tag) {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        hipLaunchKernelGGL(k_mass_momentum<S>, dim3(G), dim3(64 * MM_NW), 0, s, (const S*)L.fa, L.mask, L.qs, L.nx,
+                           L.ny, L.nxp, L.nsegw, L.nz, part, L.shift);
+        return 0;
+    });
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, (const double*)part, (long long)G, 4, out_dev);
+    HIP_TRY(ctx, hipGetLastError());
+    return LBM_OK;
+}
+
 extern "C" int lbm_get_mass_momentum(lbm_ctx* ctx, int level, double out[4])
 {
     if (!ctx || !out) return fail(ctx, LBM_EINVAL, "lbm_get_mass_momentum: null argument");
     if (level < 0 || level >= ctx->nlevels) return fail(ctx, LBM_EINVAL, "lbm_get_mass_momentum: bad level");
-    Level& L = ctx->lv[level];
-    double* part = ctx->mpart;
-    double* tmp = nullptr;
-    if (level > 0) {
-        HIP_TRY(ctx, hipMalloc(&tmp, (size_t)L.blocks * 4 * sizeof(double)));
-        part = tmp;
-    }
-    with_storage(L.storage, [&](auto* tag) {
-        using S = std::remove_pointer_t<decltype(tag)>;
-        hipLaunchKernelGGL(k_mass_momentum<S>, dim3(L.blocks), dim3(Shape<S>::THREADS), 0, ctx->stream, (const S*)L.fa,
-                           L.mask, L.qs, L.nx, L.ny, L.nxp, L.nsegw, L.nz, part, L.shift);
-        return 0;
-    });
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, ctx->stream, part, (long long)L.blocks, 4,
-                       ctx->red);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(out, ctx->red, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    int rc = mass_momentum_launch(ctx, ctx->lv[level], ctx->mpart, ctx->red, ctx->stream);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(out, ctx->red, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (tmp) (void)hipFree(tmp);
     if (e != hipSuccess) return fail(ctx, LBM_EDEVICE, std::string("lbm_get_mass_momentum: ") + hipGetErrorString(e));
     return LBM_OK;
 }
