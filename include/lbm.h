@@ -193,6 +193,20 @@ int lbm_get_forces(lbm_ctx* ctx, double* fxyz, int max_steps, int* nsteps);
  * fixed order (deterministic).  No reference counterpart. */
 int lbm_get_mass_momentum(lbm_ctx* ctx, int level, double out[4]);
 
+/* lbm_get_mass_momentum for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight monitoring loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The same kernels and
+ * the same bits as lbm_get_mass_momentum, but the four FP64 sums go to the
+ * caller's DEVICE buffer out_dev[4], enqueued on `stream` (a hipStream_t;
+ * NULL: the context's stream) without synchronising, and the call allocates
+ * nothing: the per-workgroup partials live in a static device array of
+ * 64 KiB (2048 workgroups x 4 sums; every lattice the library accepts needs
+ * at most that many, and a larger need is refused with LBM_EINVAL) that each
+ * GPU holds for the life of the process and every context on that GPU
+ * shares.  Calls that overlap on one GPU (several contexts or streams) race
+ * on it; serialise them. */
+int lbm_mass_momentum_noalloc(lbm_ctx* ctx, int level, double* out_dev, void* stream);
+
 /* Tecplot ASCII output as WriteResults / WriteResultsLR
  * (MRTtest/DynInlet3.cu:2601-2787): <casename>.dat (coarse level, POINT zone,
  * X,Y,Z,u,v,w,rho, plus velAv[0..2], ufluc, vfluc, wfluc, Smag with running

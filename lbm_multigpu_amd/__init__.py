@@ -68,7 +68,7 @@ EXPORTS = [
     "lbm_mlp_set_params", "lbm_mlp_get_params", "lbm_mlp_set_norm", "lbm_mlp_set_data",
     "lbm_mlp_gradient", "lbm_mlp_train", "lbm_mlp_forward", "lbm_set_closure",
     "lbm_closure_samples", "lbm_stats_enable", "lbm_stats_get", "lbm_set_closure_interval",
-    "lbm_get_inlet_control", "lbm_plane_mean_v", "lbm_get_dpdy",
+    "lbm_get_inlet_control", "lbm_plane_mean_v", "lbm_get_dpdy", "lbm_mass_momentum_noalloc",
 ]
 
 # closure network: 7 -> 64 -> 64 -> 64 -> 1, flat FP32 parameter vector
@@ -106,6 +106,7 @@ def load_library(path=LIB_PATH):
     lib.lbm_set_populations_f64.argtypes = [P, C.c_int, f64]
     lib.lbm_get_forces.argtypes = [P, P, C.c_int, C.POINTER(C.c_int)]
     lib.lbm_get_mass_momentum.argtypes = [P, C.c_int, f64]
+    lib.lbm_mass_momentum_noalloc.argtypes = [P, C.c_int, P, P]
     lib.lbm_write_tecplot.argtypes = [P, C.c_char_p]
     lib.lbm_query_sizes.argtypes = [P, C.c_int, C.POINTER(lbm_sizes)]
     lib.lbm_get_time.argtypes = [P, C.POINTER(C.c_longlong)]

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -57,6 +58,7 @@ struct lbm_ctx {
     double* fhist = nullptr;   // [force_steps*3]
     double* red = nullptr;     // small reduction output
     double* mpart = nullptr;   // mass partials
+    double* scratch = nullptr; // this device's address of the static scratch array (lbm_mass_momentum_noalloc)
     long long t = 0;
     int force_recorded = 0;
     long long stat_av = -1, stat_fluc = -1; // statistics phases (off while stat_av < 0)
@@ -114,6 +116,35 @@ static int fail(lbm_ctx* ctx, int code, const std::string& msg)
     if (ctx) ctx->err = msg;
     g_last_error = msg;
     return code;
+}
+
+// Device memory of lbm_mass_momentum_noalloc: the per-workgroup FP64
+// partials of one mass/momentum launch (at most MM_GRID_MAX workgroups x 4
+// sums = 64 KiB).  Every GPU holds its own copy of this array at its own
+// address for the life of the process; host code uses that address
+// (noalloc_scratch), never the array's name, which on the host is only a
+// stand-in.  Each call overwrites the partial slots it reduces, so nothing
+// has to start at zero; calls that overlap on one GPU race on it.
+constexpr int NOALLOC_SCRATCH_DOUBLES = MM_GRID_MAX * 4;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+// hipGetSymbolAddress of `scratch` once per device, with that device
+// current; the address is kept per device for every later call.
+static int noalloc_scratch(lbm_ctx* ctx, int device, double** out)
+{
+    static std::mutex mu;
+    static std::vector<double*> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)addr.size() <= device) addr.resize(device + 1, nullptr);
+    if (!addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(ctx, LBM_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        addr[device] = static_cast<double*>(p);
+    }
+    *out = addr[device];
+    return LBM_OK;
 }
 
 #define HIP_TRY(ctx, expr)                                                                  \
@@ -348,6 +379,7 @@ extern "C" int lbm_create(const lbm_config* cfg, lbm_ctx** out)
         hipMalloc(&ctx->mpart, (size_t)std::max(L.blocks, MM_GRID_MAX) * 4 * sizeof(double)) != hipSuccess ||
         hipMalloc(&ctx->red, 8 * sizeof(double)) != hipSuccess)
         return bail(fail(nullptr, LBM_ENOMEM, "lbm_create: device allocation failed"));
+    if ((rc = noalloc_scratch(nullptr, dev, &ctx->scratch)) != LBM_OK) return bail(rc);
     if (cfg->force_steps > 0) {
         if (hipMalloc(&ctx->fhist, (size_t)cfg->force_steps * 3 * sizeof(double)) != hipSuccess)
             return bail(fail(nullptr, LBM_ENOMEM, "lbm_create: force history allocation failed"));
@@ -1100,6 +1132,24 @@ extern "C" int lbm_get_mass_momentum(lbm_ctx* ctx, int level, double out[4])
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return fail(ctx, LBM_EDEVICE, std::string("lbm_get_mass_momentum: ") + hipGetErrorString(e));
     return LBM_OK;
+}
+
+// lbm_get_mass_momentum without any allocation or synchronisation: the same
+// kernels, with the partials in this device's static scratch array and the
+// four sums written to the caller's device buffer.
+extern "C" int lbm_mass_momentum_noalloc(lbm_ctx* ctx, int level, double* out_dev, void* stream)
+{
+    if (!ctx || !out_dev) return fail(ctx, LBM_EINVAL, "lbm_mass_momentum_noalloc: null argument");
+    if (level < 0 || level >= ctx->nlevels) return fail(ctx, LBM_EINVAL, "lbm_mass_momentum_noalloc: bad level");
+    const Level& L = ctx->lv[level];
+    const long long need = 4LL * mm_grid(L.nsegw, L.ny, L.nz);
+    if (need > NOALLOC_SCRATCH_DOUBLES)
+        return fail(ctx, LBM_EINVAL, "lbm_mass_momentum_noalloc: " + std::to_string(need) +
+                                         " partials exceed the scratch array of " +
+                                         std::to_string(NOALLOC_SCRATCH_DOUBLES));
+    if (!ctx->scratch) return fail(ctx, LBM_ESTATE, "lbm_mass_momentum_noalloc: no scratch address for this device");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    return mass_momentum_launch(ctx, L, ctx->scratch, out_dev, s);
 }
 
 extern "C" int lbm_query_sizes(lbm_ctx* ctx, int level, lbm_sizes* out)
