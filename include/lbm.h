@@ -52,7 +52,8 @@ extern "C" {
 
 /* population storage in HBM */
 #define LBM_STORE_F32 0  /* reference layout precision */
-#define LBM_STORE_F16 1  /* f_i - f_i^eq(1, (umax,0,0)) in IEEE half, FP32 collide (= f_i - w_i when umax = 0) */
+#define LBM_STORE_F16 1  /* f_i - f_i^eq(1, (umax,0,0)) in IEEE half, FP32 collide (= f_i - w_i when umax = 0,
+                            and with the y-streamwise LBM_GEOM_PRECURSOR) */
 #define LBM_STORE_BF16 2 /* the same offset in bfloat16, FP32 collide */
 #define LBM_STORE_F64 3  /* FP64 storage and arithmetic (the FP64 yardstick) */
 
@@ -63,6 +64,12 @@ extern "C" {
                               (LaminarChannelCylinder1/LRMultiGPU6.cu:90-129) */
 #define LBM_GEOM_CPU3D 2   /* CPU_3D/CPU_3D.cpp:39-75: circular cylinder (im 10),
                               regularised moving-lid x=0 plane (im 53), walls (im 1) */
+#define LBM_GEOM_PRECURSOR 3 /* y-streamwise channel with a periodic inflow precursor
+                              (bgk_channel1/Re3000_Cyl2/DynInlet3.cu:75-94): trip
+                              obstacle 2 (im 1) while t < trip_until, obstacle 1
+                              (im 10), node walls at x = 0 and x = nx-1 (im 1), the
+                              recycling plane y = dyn_y1+1 (im 80) and the north
+                              rho = 1 outlet y = ny-1 (im 100) */
 
 /* node codes of lbm_set_geometry()/lbm_get_mask(), with the reference im code */
 #define LBM_NODE_FLUID 0        /* im 0 */
@@ -73,6 +80,21 @@ extern "C" {
 #define LBM_NODE_LID_WEST_REG 5 /* im 53: DirichletWest_Regularized (CPU_3D/CPU_3D.cpp:974) */
 #define LBM_NODE_SYM_XMIN 6     /* im 26: xsymmetry_bot, x = 0 symmetry plane (MRTtest/DynInlet3.cu:1020-1075) */
 #define LBM_NODE_SYM_XMAX 7     /* im 25: xsymmetry_top, x = nx-1 symmetry plane (MRTtest/DynInlet3.cu:1077-1137) */
+#define LBM_NODE_OUTLET_NORTH 8 /* im 100: North_Extrap rho = 1 from the unstreamed y-1 node
+                                   (MRTtest/DynInlet3.cu:834-879, dispatch :1711-1716) */
+#define LBM_NODE_INLET_SOUTH 9  /* im 200: South_Extrap, velocity (0, v_in, 0) and the other
+                                   moments from the unstreamed y+1 node (:881-924, :1717-1724) */
+#define LBM_NODE_RECYCLE 10     /* im 80: rho from the unstreamed y+1 node, every other moment
+                                   from y-1 (bgk_channel1/Re3000_Cyl2/DynInlet3.cu:1231-1239) */
+#define LBM_NODE_VEL_FROM_BELOW 11 /* im 70: velocity from the unstreamed y-1 node, every other
+                                   moment from y+1 (bgk_channel1/Re3000_Cyl2/DynInlet3.cu:1220-1230) */
+#define LBM_NODE_GHOST_W 12     /* im 50: copy of the unstreamed node (nx-2, y, z)
+                                   (MRTtest/DynInlet3.cu:1747-1751) */
+#define LBM_NODE_GHOST_E 13     /* im 51: copy of (1, y, z) (:1752-1756) */
+#define LBM_NODE_GHOST_S 14     /* im 52: copy of (x, DYNY1-1, z) (:1757-1761) */
+#define LBM_NODE_GHOST_N 15     /* im 53 of MRTtest (not CPU_3D's 53): copy of (x, DYNY2, z) (:1762-1766) */
+#define LBM_NODE_GHOST_DYN 16   /* im 54: copy of (x, 1, z) (:1767-1771) */
+#define LBM_NODE_MAX 16
 
 /* initial state of lbm_init_equilibrium() */
 #define LBM_INIT_BGK 0 /* explicit BGK feq (CPU_3D/CPU_3D.cpp:1218-1254) */
@@ -124,6 +146,19 @@ typedef struct lbm_config {
     float kp;              /* "KP" (0: the plane mean is recorded, dpdy held) */
     int ctrl_every;        /* 100 as the reference; 0 switches the control off */
     int ctrl_start;        /* 10000 as the reference */
+    /* the precursor form of bgk_channel1/Re3000_Cyl2/DynInlet3.cu: with
+       dyn_mode = 1 rows 0 .. dyn_y1 are periodic in y among themselves (row
+       dyn_y1 pulls its +y neighbours from row 0 and row 0 its -y ones from
+       row dyn_y1, :1131-1132) and the body force acts on every row (the
+       "if(y>DYNY1) dpdy = 0.f" of MRTtest is commented out there, :1247);
+       dyn_mode = 0 is MRTtest's form (no wrap, force up to dyn_y1). */
+    int dyn_mode;
+    int dyn_y2;            /* DYNY2: source row of the im 53 ghost copy (MRTtest/DynInlet3.cu:1765) */
+    float obst2_x, obst2_y, obst2_z, obst2_r; /* "Obst2 location" / "Obst2 radius": the trip obstacle */
+    int trip_until;        /* the trip obstacle exists for steps t < trip_until (ImageFcn's
+                              "t < 5000", MRTtest/DynInlet3.cu:75-78); 0: never */
+    int trip_z;            /* z plane of the trip's second, offset block (x-X2-3, y-Y2-3);
+                              -1: none */
 } lbm_config;
 
 typedef struct lbm_sizes {
@@ -154,7 +189,9 @@ int lbm_get_mask(lbm_ctx* ctx, int level, uint8_t* mask);
 
 /* Replaces initialize()/initializeLR() (MRTtest/DynInlet3.cu:1186-1251) and
  * CPU_3D's initial fill (CPU_3D/CPU_3D.cpp:1218-1254).  rho, u, v, w: dense
- * fields, or NULL for rho=1 and u=(umax,0,0).  kind = LBM_INIT_BGK fills
+ * fields, or NULL for rho=1 and u=(umax,0,0) -- (0,umax,0) with
+ * LBM_GEOM_PRECURSOR, whose flow runs along y as initialize's does
+ * (MRTtest/DynInlet3.cu:1200-1203).  kind = LBM_INIT_BGK fills
  * every node (solids included, as CPU_3D); LBM_INIT_MRT gives solids u=0. */
 int lbm_init_equilibrium(lbm_ctx* ctx, int kind, const float* rho, const float* u,
                          const float* v, const float* w);
@@ -186,7 +223,14 @@ int lbm_set_populations_f64(lbm_ctx* ctx, int level, const double* f);
 
 /* Momentum-exchange force per recorded step, FxFyFz (3 doubles per step),
  * un-normalised (WriteForces, :2788-2797, divides by UMAX^2 ZDIM OBSTR1).
- * Replaces the FX/FY/FZ atomicAdd arrays (:1819-1838). */
+ * Replaces the FX/FY/FZ atomicAdd arrays (:1819-1838).  Row s holds step
+ * t = STARTF + s for s >= 1; steps t > STARTF are recorded, as main() sets
+ * flag_F (:3130, :3183), so row 0 stays zero.  Without refinement the
+ * coarse level's force is recorded; with refinement only the fine level's,
+ * on the first of its two sub-steps, in fine lattice units (the reference
+ * turns the coarse recording off under REFINEMENT, :3130, records
+ * update_inn_LR's on sub-step 0, :3183, and WriteForces divides that by a
+ * further LRLEVEL^2, :2790). */
 int lbm_get_forces(lbm_ctx* ctx, double* fxyz, int max_steps, int* nsteps);
 
 /* Global mass and momentum (sum rho, sum u, sum v, sum w) with FP64 sums in a

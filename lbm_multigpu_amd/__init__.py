@@ -20,15 +20,21 @@ LIB_PATH = os.path.join(_HERE, "liblbm.so")
 MODEL_BGK, MODEL_MRT_CPU3D, MODEL_MRT_LES, MODEL_MRT_DHUMIERES = 0, 1, 2, 3
 BC_PERIODIC, BC_CLAMP, BC_WALL = 0, 1, 2
 STORE_F32, STORE_F16, STORE_BF16, STORE_F64 = 0, 1, 2, 3
-GEOM_NONE, GEOM_CHANNEL, GEOM_CPU3D = 0, 1, 2
+GEOM_NONE, GEOM_CHANNEL, GEOM_CPU3D, GEOM_PRECURSOR = 0, 1, 2, 3
 (NODE_FLUID, NODE_SOLID, NODE_SOLID_FORCE, NODE_INLET_WEST, NODE_OUTLET_EAST, NODE_LID_WEST_REG,
- NODE_SYM_XMIN, NODE_SYM_XMAX) = range(8)
+ NODE_SYM_XMIN, NODE_SYM_XMAX, NODE_OUTLET_NORTH, NODE_INLET_SOUTH, NODE_RECYCLE, NODE_VEL_FROM_BELOW,
+ NODE_GHOST_W, NODE_GHOST_E, NODE_GHOST_S, NODE_GHOST_N, NODE_GHOST_DYN) = range(17)
 INIT_BGK, INIT_MRT = 0, 1
 
-# reference `im` code of each node code (MRTtest/DynInlet3.cu:1286-1390, CPU_3D/CPU_3D.cpp:1142)
+# reference `im` code of each node code (MRTtest/DynInlet3.cu:1286-1390, :1711-1771,
+# bgk_channel1/Re3000_Cyl2/DynInlet3.cu:1220-1239, CPU_3D/CPU_3D.cpp:1142).  The
+# reference uses 53 twice: CPU_3D's regularised lid and MRTtest's ghost copy of
+# row DYNY2; IM_TO_NODE maps 53 to the lid (CPU_3D's meaning).
 NODE_TO_IM = {NODE_FLUID: 0, NODE_SOLID: 1, NODE_SOLID_FORCE: 10, NODE_INLET_WEST: 400,
-              NODE_OUTLET_EAST: 300, NODE_LID_WEST_REG: 53, NODE_SYM_XMIN: 26, NODE_SYM_XMAX: 25}
-IM_TO_NODE = {v: k for k, v in NODE_TO_IM.items()}
+              NODE_OUTLET_EAST: 300, NODE_LID_WEST_REG: 53, NODE_SYM_XMIN: 26, NODE_SYM_XMAX: 25,
+              NODE_OUTLET_NORTH: 100, NODE_INLET_SOUTH: 200, NODE_RECYCLE: 80, NODE_VEL_FROM_BELOW: 70,
+              NODE_GHOST_W: 50, NODE_GHOST_E: 51, NODE_GHOST_S: 52, NODE_GHOST_N: 53, NODE_GHOST_DYN: 54}
+IM_TO_NODE = {v: k for k, v in NODE_TO_IM.items() if k != NODE_GHOST_N}
 
 
 class LBMError(RuntimeError):
@@ -49,7 +55,9 @@ class lbm_config(C.Structure):
                 ("force_steps", C.c_int), ("refinement", C.c_int), ("lr_nx", C.c_int),
                 ("lr_ny", C.c_int), ("lr_nz", C.c_int), ("lr_x0", C.c_float), ("lr_y0", C.c_float),
                 ("lr_z0", C.c_float), ("temporal_interp", C.c_int), ("interface_eq", C.c_int),
-                ("dyn_y1", C.c_int), ("kp", C.c_float), ("ctrl_every", C.c_int), ("ctrl_start", C.c_int)]
+                ("dyn_y1", C.c_int), ("kp", C.c_float), ("ctrl_every", C.c_int), ("ctrl_start", C.c_int),
+                ("dyn_mode", C.c_int), ("dyn_y2", C.c_int), ("obst2_x", C.c_float), ("obst2_y", C.c_float),
+                ("obst2_z", C.c_float), ("obst2_r", C.c_float), ("trip_until", C.c_int), ("trip_z", C.c_int)]
 
 
 class lbm_sizes(C.Structure):

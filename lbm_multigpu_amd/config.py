@@ -24,12 +24,12 @@ REFERENCE_KEYS = [
 EXTENSION_KEYS = [
     "MRT rates:", "Boundary x:", "Boundary y:", "Boundary z:", "Halfway bounce-back:", "Storage:",
     "Geometry:", "Inlet perturbation:", "Seed:", "Temporal interpolation:", "Force steps:",
-    "Statistics:", "DYNY1:", "Inlet control:",
+    "Statistics:", "DYNY1:", "Inlet control:", "DYNY2:", "Dynamic inlet mode:", "Trip until:", "Trip z:",
 ]
 
 _BC = {"periodic": 0, "clamp": 1, "wall": 2}
 _STORE = {"f32": 0, "f16": 1, "bf16": 2, "f64": 3}
-_GEOM = {"none": 0, "channel": 1, "cpu3d": 2}
+_GEOM = {"none": 0, "channel": 1, "cpu3d": 2, "precursor": 3}
 _RATES = {"lrmgpu": 2, "dhumieres": 3, "cpu3d": 1}
 
 
@@ -75,6 +75,11 @@ class Config:
     kp: float = 0.0             # "KP": P-control gain on dpdy
     ctrl_every: int = 100       # control interval (steps); 0: off
     ctrl_start: int = 10000     # control from steps t > ctrl_start
+    # bgk_channel1/Re3000_Cyl2 precursor: 1 = rows 0..DYNY1 periodic in y, force on every row
+    dyn_mode: int = 0
+    dyn_y2: int = 1             # DYNY2: source row of MRTtest's im 53 ghost copy
+    trip_until: int = 0         # the trip obstacle (Obst2) exists for steps t < trip_until
+    trip_z: int = -1            # z plane of the trip's offset block; -1: none
 
     def model_code(self):
         if self.model.upper() == "BGK":
@@ -110,6 +115,10 @@ class Config:
         c.interface_eq = int(self.interface_eq)
         c.dyn_y1, c.kp = int(self.dyn_y1), float(self.kp)
         c.ctrl_every, c.ctrl_start = int(self.ctrl_every), int(self.ctrl_start)
+        c.dyn_mode, c.dyn_y2 = int(self.dyn_mode), int(self.dyn_y2)
+        c.obst2_x, c.obst2_y, c.obst2_z = self.obst2
+        c.obst2_r = self.obst2_r
+        c.trip_until, c.trip_z = int(self.trip_until), int(self.trip_z)
         return c
 
 
@@ -184,7 +193,9 @@ def read_inputs(path_or_text):
                            ("START_VELAV:", "start_velav", int), ("START_VELFLUC:", "start_velfluc", int),
                            ("CS:", "cs", float), ("omega in LR", "omega_lr", float),
                            ("LR factor", "lr_factor", float), ("GPUs per node", "gpus", int),
-                           ("KP:", "kp", float), ("DYNY1:", "dyn_y1", int)):
+                           ("KP:", "kp", float), ("DYNY1:", "dyn_y1", int), ("DYNY2:", "dyn_y2", int),
+                           ("Dynamic inlet mode:", "dyn_mode", int), ("Trip until:", "trip_until", int),
+                           ("Trip z:", "trip_z", int)):
         if key in seen:
             setattr(cfg, attr, typ(float(seen[key])) if typ is int else typ(seen[key]))
     if "REFINEMENT:" in seen:
@@ -274,6 +285,10 @@ def write_inputs(cfg, path=None):
         f"DYNY1: \t{cfg.dyn_y1}",
         f"KP: \t{_g(cfg.kp)}",
         f"Inlet control: \t{cfg.ctrl_every}x{cfg.ctrl_start}",
+        f"DYNY2: \t{cfg.dyn_y2}",
+        f"Dynamic inlet mode: \t{cfg.dyn_mode}",
+        f"Trip until: \t{cfg.trip_until}",
+        f"Trip z: \t{cfg.trip_z}",
     ]
     text = "\n".join(lines) + "\n"
     if path:
@@ -343,3 +358,21 @@ def refined_channel(nx=256, ny=128, nz=128, side=16, x_center=64, patch=(128, 96
     c.temporal_interp = temporal
     c.tmax = 2000
     return c
+
+
+def precursor(nx=42, ny=96, nz=8, dyn_y1=24, side=12, obst_y=60, trip=(20, 10, 4), trip_until=200,
+              tau=0.55, cs_smag=0.1, u0=0.06, dpdy=-5e-7, kp=0.3, ctrl=(100, 1000), storage="f32"):
+    """A y-streamwise channel with a periodic inflow precursor in the form of
+    bgk_channel1/Re3000_Cyl2/DynInlet3.cu (its 224x800x4 case scaled down):
+    node walls at x = 0 and x = nx-1, rows 0..DYNY1 periodic in y and driven
+    by the P-controlled body force, the recycling plane y = DYNY1+1 (im 80)
+    feeding the channel above, a force-summed square cylinder of side `side`
+    (Obst1) at y = obst_y, a trip obstacle (Obst2, half-width trip[2] at
+    (trip[0], trip[1])) for t < trip_until, and the rho = 1 north outlet."""
+    half = side / 2.0
+    return Config(nx=nx, ny=ny, nz=nz, bc=("periodic", "periodic", "periodic"), model="MRT", mrt_rates="lrmgpu",
+                  les=cs_smag > 0, cs=cs_smag * cs_smag, omega=1.0 / tau, umax=u0, dpdy=dpdy,
+                  geometry="precursor", obst1=(nx / 2.0 - 0.5, obst_y + 0.5, nz / 2.0), obst1_r=half,
+                  obst2=(trip[0] + 0.5, trip[1] + 0.5, nz / 2.0), obst2_r=float(trip[2]), trip_until=trip_until,
+                  trip_z=0, dyn_y1=dyn_y1, dyn_mode=1, kp=kp, ctrl_every=ctrl[0], ctrl_start=ctrl[1],
+                  storage=storage, tmax=2000)

@@ -372,6 +372,8 @@ __global__ void __launch_bounds__(64 * CLOSURE_NW, 2) k_stream_collide_closure(S
             cp.cs = R(0);
             collide<MODEL>(f, cp);
             if (p.stat_mode) stats_update(p, cell, f);
+        } else if (act == ACT_COPY && p.stat_mode) {
+            stats_update(p, cell, f);
         }
         node_store<S>(p, cell, f);
     }

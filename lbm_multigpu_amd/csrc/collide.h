@@ -347,11 +347,14 @@ __device__ __forceinline__ void collide(R f[Q], const CollideParams<R>& cp)
     else collide_mrt<MODEL, R, FAST>(f, cp);
 }
 
-// West_Extrap (MRTtest/DynInlet3.cu:973-1017) applied to the neighbour's
-// unstreamed populations: f = M^-1 m with m3 = u_in, m5 = m7 = 0 and every
-// other moment from f.  As a correction: f_i += c_ix (u_in - jx)/10 - c_iy jy/10 - c_iz jz/10.
+// Velocity-prescribing Extrap conditions applied to a neighbour's unstreamed
+// populations: f = M^-1 m with j = (ux, uy, uz) and every other moment (rho
+// included) from f; as a correction f_i += c_i . (u - j)/10.  West_Extrap
+// (MRTtest/DynInlet3.cu:973-1017) prescribes (u_in, 0, 0), South_Extrap
+// (:881-924) (0, v_in, 0), and the im 70 plane of bgk_channel1/Re3000_Cyl2/
+// DynInlet3.cu:1220-1230 the momentum of the node below.
 template <class R>
-__device__ __forceinline__ void west_extrap(R f[Q], R u_in)
+__device__ __forceinline__ void vel_extrap(R f[Q], R ux, R uy, R uz)
 {
     R jx = 0, jy = 0, jz = 0;
 #pragma unroll
@@ -360,13 +363,32 @@ __device__ __forceinline__ void west_extrap(R f[Q], R u_in)
         if (CY[i]) jy += R(CY[i]) * f[i];
         if (CZ[i]) jz += R(CZ[i]) * f[i];
     }
-    const R ax = (u_in - jx) * R(0.1), ay = -jy * R(0.1), az = -jz * R(0.1);
+    const R ax = (ux - jx) * R(0.1), ay = (uy - jy) * R(0.1), az = (uz - jz) * R(0.1);
 #pragma unroll
     for (int i = 0; i < Q; ++i) f[i] += R(CX[i]) * ax + R(CY[i]) * ay + R(CZ[i]) * az;
 }
 
-// East_Extrap (MRTtest/DynInlet3.cu:926-971): f = M^-1 m with m0 = rho_out,
-// i.e. f_i += (rho_out - sum f)/19.
+template <class R>
+__device__ __forceinline__ void west_extrap(R f[Q], R u_in)
+{
+    vel_extrap(f, u_in, R(0), R(0));
+}
+
+// Momentum sums of a population set.
+template <class R>
+__device__ __forceinline__ void momentum(const R f[Q], R& jx, R& jy, R& jz)
+{
+    jx = 0; jy = 0; jz = 0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        if (CX[i]) jx += R(CX[i]) * f[i];
+        if (CY[i]) jy += R(CY[i]) * f[i];
+        if (CZ[i]) jz += R(CZ[i]) * f[i];
+    }
+}
+
+// East_Extrap (MRTtest/DynInlet3.cu:926-971) and North_Extrap (:834-879):
+// f = M^-1 m with m0 = rho_out, i.e. f_i += (rho_out - sum f)/19.
 template <class R>
 __device__ __forceinline__ void east_extrap(R f[Q], R rho_out)
 {

@@ -75,6 +75,18 @@ constexpr unsigned char NODE_OUTLET_EAST = 4;
 constexpr unsigned char NODE_LID_WEST_REG = 5;
 constexpr unsigned char NODE_SYM_XMIN = 6;
 constexpr unsigned char NODE_SYM_XMAX = 7;
+// y-streamwise inflow family and ghost copies (codes at and above
+// NODE_OUTLET_NORTH take one branch off the common path)
+constexpr unsigned char NODE_OUTLET_NORTH = 8;   // im 100
+constexpr unsigned char NODE_INLET_SOUTH = 9;    // im 200
+constexpr unsigned char NODE_RECYCLE = 10;       // im 80
+constexpr unsigned char NODE_VEL_FROM_BELOW = 11; // im 70
+constexpr unsigned char NODE_GHOST_W = 12;       // im 50 .. 54
+constexpr unsigned char NODE_GHOST_E = 13;
+constexpr unsigned char NODE_GHOST_S = 14;
+constexpr unsigned char NODE_GHOST_N = 15;
+constexpr unsigned char NODE_GHOST_DYN = 16;
+constexpr unsigned char NODE_MAX = 16;
 // Flags the host adds to a node's code in the device mask: NODE_NEAR, the
 // node has a link that bounces (out through a wall face, or from a solid
 // under link bounce-back); NODE_NEAR_SOLID, one of those links comes from a

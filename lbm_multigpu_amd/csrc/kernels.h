@@ -107,6 +107,11 @@ struct StepParams {
     const float* omega_node;
     PopShift sh;   // 16-bit storage offsets
     int dyn_y1;    // rows y > dyn_y1 get no body force (INT_MAX: all rows get it)
+    const float* v_in;  // south inlet profile [z*nx + x] (NODE_INLET_SOUTH)
+    const int* fslot;   // force partial slot of each workgroup (-1: no force node in it);
+                        // nullptr: slot = workgroup
+    int y_wrap;         // rows 0 .. y_wrap periodic in y among themselves (-1: no wrap)
+    int ghost_y[3];     // source rows of the im 52 / 53 / 54 ghost copies
     int xcd_chunk; // consecutive workgroups per XCD (logical_block)
     int nt_store;  // nontemporal population stores (lattices larger than the MALL)
 };
@@ -173,7 +178,20 @@ __device__ __forceinline__ R wave_sum(R v)
     return v;
 }
 
-enum NodeAction : int { ACT_SKIP = 0, ACT_STORE = 1, ACT_COLLIDE = 2 };
+enum NodeAction : int { ACT_SKIP = 0, ACT_STORE = 1, ACT_COLLIDE = 2, ACT_COPY = 3 };
+
+// Neighbour rows along y: the axis rule, then the precursor wrap of
+// bgk_channel1/Re3000_Cyl2/DynInlet3.cu:1131-1132 (row y_wrap pulls its +y
+// neighbours from row 0, row 0 its -y ones from row y_wrap).  y is
+// wave-uniform, so this is scalar work.
+__device__ __forceinline__ void y_nb(const StepParams& p, int y, int& m, int& pl, bool& om, bool& op)
+{
+    axis_nb(y, p.ny, p.bcy, m, pl, om, op);
+    if (p.y_wrap >= 0) {
+        if (y == p.y_wrap) { pl = 0; op = false; }
+        if (y == 0) { m = p.y_wrap; om = false; }
+    }
+}
 
 // The 19 pulled populations of node (x, y, z): neighbour rows per axis
 // (periodic, clamped or walled), then one load per direction.
@@ -184,7 +202,7 @@ __device__ __forceinline__ void node_load(const StepParams& p, int x, int y, int
     int xm, xp, ym, yp, zm, zp;
     bool oxm, oxp, oym, oyp, ozm, ozp;
     axis_nb(x, p.nx, p.bcx, xm, xp, oxm, oxp);
-    axis_nb(y, p.ny, p.bcy, ym, yp, oym, oyp);
+    y_nb(p, y, ym, yp, oym, oyp);
     axis_nb(z, p.nz, p.bcz, zm, zp, ozm, ozp);
     const int Y[3] = {ym, y, yp};
     const int Z[3] = {zm, z, zp};
@@ -194,6 +212,86 @@ __device__ __forceinline__ void node_load(const StepParams& p, int x, int y, int
         const int src = p.nxp * (Y[1 - CY[i]] + p.ny * Z[1 - CZ[i]]) + X[1 - CX[i]];
         f[i] = Store<S>::load(at(fin + i * p.qs, src), i, p.sh);
     }
+}
+
+// The unstreamed populations of node `cell` (its state at the step's start).
+template <class S>
+__device__ __forceinline__ void node_unstreamed(const StepParams& p, int cell, typename Arith<S>::R f[Q])
+{
+    const S* __restrict__ fin = static_cast<const S*>(p.fin);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = Store<S>::load(at(fin + i * p.qs, cell), i, p.sh);
+}
+
+// Node codes at and above NODE_OUTLET_NORTH: the y-streamwise Extrap family
+// and the ghost copies, one branch off the common path.
+//   im 100 North_Extrap(f, 1) on the node below      (MRTtest/DynInlet3.cu:1711-1716)
+//   im 200 South_Extrap(f, v_in) on the node above   (:1717-1724)
+//   im  80 rho of the node above, moments of the node below
+//          (bgk_channel1/Re3000_Cyl2/DynInlet3.cu:1231-1239)
+//   im  70 velocity of the node below, moments of the node above (:1220-1230)
+//   im 50-54 copies of an unstreamed node            (MRTtest/DynInlet3.cu:1747-1771)
+// The node whose moments are kept is read into f; the Extrap correction
+// f_i += (rho* - rho)/19 (density form) or f_i += c_i . (j* - j)/10
+// (velocity form) then needs only the differences, which for im 80 / im 70
+// start from the kept node's negated sums and take the other node's values
+// as they are read, so at most three sums stay live beside f.  Rows y +- 1
+// wrap periodically, as the reference's f_mem does (MRTtest/DynInlet3.cu:177-185).
+template <class S>
+__device__ __forceinline__ int node_rare(const StepParams& p, int x, int y, int z, unsigned char base,
+                                         typename Arith<S>::R f[Q])
+{
+    using R = typename Arith<S>::R;
+    const int row = p.nxp * p.ny * z;
+    const int below = x + p.nxp * (y > 0 ? y - 1 : p.ny - 1) + row;
+    const int above = x + p.nxp * (y + 1 < p.ny ? y + 1 : 0) + row;
+    const bool ghost = base >= NODE_GHOST_W;
+    int sx = x, sy = y;
+    if (base == NODE_GHOST_W) sx = p.nx - 2;
+    else if (base == NODE_GHOST_E) sx = 1;
+    else if (base == NODE_GHOST_S) sy = p.ghost_y[0];
+    else if (base == NODE_GHOST_N) sy = p.ghost_y[1];
+    else if (base == NODE_GHOST_DYN) sy = p.ghost_y[2];
+    const bool density = base == NODE_OUTLET_NORTH || base == NODE_RECYCLE;
+    node_unstreamed<S>(p, ghost ? sx + p.nxp * sy + row : (density ? below : above), f);
+    if (ghost) return ACT_COPY;
+    // differences to the prescribed values: d0 = rho* - rho (density form),
+    // (dx, dy, dz) = j* - j (velocity form)
+    R d0 = 0, dx = 0, dy = 0, dz = 0;
+    if (density) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) d0 -= f[i];
+        if (base == NODE_OUTLET_NORTH) d0 += R(1);
+    } else {
+        R jx, jy, jz;
+        momentum(f, jx, jy, jz);
+        dx = -jx; dy = -jy; dz = -jz;
+        if (base == NODE_INLET_SOUTH) dy += R(p.v_in[z * p.nx + x]);
+    }
+    if (base == NODE_RECYCLE || base == NODE_VEL_FROM_BELOW) {
+        const S* __restrict__ fin = static_cast<const S*>(p.fin);
+        const int other = base == NODE_RECYCLE ? above : below;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const R v = Store<S>::load(at(fin + i * p.qs, other), i, p.sh);
+            if (density) d0 += v;
+            else {
+                if (CX[i]) dx += R(CX[i]) * v;
+                if (CY[i]) dy += R(CY[i]) * v;
+                if (CZ[i]) dz += R(CZ[i]) * v;
+            }
+        }
+    }
+    if (density) {
+        const R a = d0 * R(1.0 / 19.0);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] += a;
+    } else {
+        const R ax = dx * R(0.1), ay = dy * R(0.1), az = dz * R(0.1);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] += R(CX[i]) * ax + R(CY[i]) * ay + R(CZ[i]) * az;
+    }
+    return ACT_COLLIDE;
 }
 
 // Boundary handling of a pulled node (the work of update_inn before
@@ -222,7 +320,7 @@ __device__ __forceinline__ int node_fix(const StepParams& p, int x, int y, int z
         int xm, xp, ym, yp, zm, zp;
         bool oxm, oxp, oym, oyp, ozm, ozp;
         axis_nb(x, p.nx, p.bcx, xm, xp, oxm, oxp);
-        axis_nb(y, p.ny, p.bcy, ym, yp, oym, oyp);
+        y_nb(p, y, ym, yp, oym, oyp);
         axis_nb(z, p.nz, p.bcz, zm, zp, ozm, ozp);
         const int Y[3] = {ym, y, yp};
         const int Z[3] = {zm, z, zp};
@@ -282,6 +380,8 @@ __device__ __forceinline__ int node_fix(const StepParams& p, int x, int y, int z
         xsym_bot(f, y, z, p.ny, p.nz);
     } else if (base == NODE_SYM_XMAX) {
         xsym_top(f, y, z, p.ny, p.nz);
+    } else if (base >= NODE_OUTLET_NORTH) {
+        return node_rare<S>(p, x, y, z, base, f);
     }
     return ACT_COLLIDE;
 }
@@ -387,6 +487,9 @@ __device__ __forceinline__ void node_finish(const StepParams& p, int y, int cell
 {
     if (act == ACT_SKIP) return;
     node_collide<MODEL, S>(p, y, cell, act, f);
+    // a ghost copy's statistics see the copied populations, as update_inn's
+    // vel_av follows the copies (MRTtest/DynInlet3.cu:1747-1779)
+    if (act == ACT_COPY && p.stat_mode) stats_update(p, cell, f);
     node_store_policy<S, AUX>(p, cell, f);
 }
 
@@ -438,7 +541,11 @@ __global__ void __launch_bounds__(Shape<S>::THREADS, Shape<S>::MIN_WAVES) k_stre
     int x, y, z;
     const bool row_ok = wave_node<NW>(lb, p.nsegw, p.ny, p.nz, x, y, z);
     if (row_ok && x < p.nx) node_update<MODEL, S, NT ? kStoreNT : 0>(p, x, y, z, fx, fy, fz);
-    if (p.fpart) block_force_partial<NW>(p, lb, fx, fy, fz);
+    if (p.fpart) {
+        // only workgroups holding a force node write a partial (wave-uniform)
+        const int slot = p.fslot ? p.fslot[lb] : lb;
+        if (slot >= 0) block_force_partial<NW>(p, slot, fx, fy, fz);
+    }
 }
 
 // Deterministic FP64 reduction of ncomp-component partials (fixed order:
@@ -627,7 +734,8 @@ __global__ void k_unpack(const T* in, long long qs, int nx, int ny, int nz, int 
 }
 
 // Equilibrium initialisation of both lattices.  Fields are dense; nullptr
-// fields mean rho = 1, u = (u0, 0, 0).
+// fields mean rho = 1, u = (u0, 0, 0), or (0, u0, 0) with ydir (the
+// y-streamwise cases' initialize, MRTtest/DynInlet3.cu:1200-1203).
 //  kind 0: BGK feq at every node, solids included (CPU_3D/CPU_3D.cpp:1218-1254)
 //  kind 1: mrt_meq + InvertMoments, solids at u = 0 (initialize,
 //          MRTtest/DynInlet3.cu:1186-1216)
@@ -635,7 +743,7 @@ __global__ void k_unpack(const T* in, long long qs, int nx, int ny, int nz, int 
 template <class S>
 __global__ void k_init(S* fa, S* fb, long long qs, int nx, int ny, int nz, int nxp,
                        const unsigned char* mask, const float* rho_in, const float* u_in,
-                       const float* v_in, const float* w_in, float u0, int kind, PopShift sh)
+                       const float* v_in, const float* w_in, float u0, int ydir, int kind, PopShift sh)
 {
     using R = typename Arith<S>::R;
     const long long n = (long long)nx * ny * nz;
@@ -647,8 +755,8 @@ __global__ void k_init(S* fa, S* fb, long long qs, int nx, int ny, int nz, int n
         const int z = int(r / ny);
         const long long cell = x + (long long)nxp * (y + (long long)ny * z);
         R rho = rho_in ? R(rho_in[c]) : R(1);
-        R u = u_in ? R(u_in[c]) : R(u0);
-        R v = v_in ? R(v_in[c]) : R(0);
+        R u = u_in ? R(u_in[c]) : (ydir ? R(0) : R(u0));
+        R v = v_in ? R(v_in[c]) : (ydir ? R(u0) : R(0));
         R w = w_in ? R(w_in[c]) : R(0);
         if (kind == 2) {
             const double kx = 2.0 * 3.14159265358979323846 * x / nx;

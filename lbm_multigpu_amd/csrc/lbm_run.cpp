@@ -39,7 +39,7 @@ const char* const kRefKeys[] = {
 const char* const kExtKeys[] = {
     "MRT rates:", "Boundary x:", "Boundary y:", "Boundary z:", "Halfway bounce-back:", "Storage:",
     "Geometry:", "Inlet perturbation:", "Seed:", "Temporal interpolation:", "Force steps:", "Statistics:",
-    "DYNY1:", "Inlet control:"};
+    "DYNY1:", "Inlet control:", "DYNY2:", "Dynamic inlet mode:", "Trip until:", "Trip z:"};
 
 struct Run {
     int nx = 64, ny = 64, nz = 64;
@@ -55,6 +55,9 @@ struct Run {
     // dynamic-inlet precursor (MRTtest/DynInlet3.cu:1744, :2561-2600, :3147-3170)
     int dyn_y1 = -1, ctrl_every = 100, ctrl_start = 10000;
     double kp = 0;
+    // the precursor form of bgk_channel1/Re3000_Cyl2 (periodic rows 0..DYNY1),
+    // the im 53 ghost row DYNY2, and the trip obstacle's time window
+    int dyn_mode = 0, dyn_y2 = 1, trip_until = 0, trip_z = -1;
 };
 
 std::string trim(const std::string& s)
@@ -169,6 +172,10 @@ bool parse(const std::string& path, Run& r)
         auto v = numbers(seen["Inlet control:"], "x");
         if (v.size() == 2) { r.ctrl_every = (int)v[0]; r.ctrl_start = (int)v[1]; }
     }
+    if (has("DYNY2:")) r.dyn_y2 = (int)num("DYNY2:");
+    if (has("Dynamic inlet mode:")) r.dyn_mode = (int)num("Dynamic inlet mode:");
+    if (has("Trip until:")) r.trip_until = (int)num("Trip until:");
+    if (has("Trip z:")) r.trip_z = (int)num("Trip z:");
     return true;
 }
 
@@ -232,6 +239,10 @@ std::string inputs_record(const Run& r)
     o << "DYNY1: \t" << r.dyn_y1 << "\n";
     o << "KP: \t" << g(r.kp) << "\n";
     o << "Inlet control: \t" << r.ctrl_every << "x" << r.ctrl_start << "\n";
+    o << "DYNY2: \t" << r.dyn_y2 << "\n";
+    o << "Dynamic inlet mode: \t" << r.dyn_mode << "\n";
+    o << "Trip until: \t" << r.trip_until << "\n";
+    o << "Trip z: \t" << r.trip_z << "\n";
     return o.str();
 }
 
@@ -264,9 +275,9 @@ bool to_config(const Run& r, lbm_config& c, std::string& why)
     const int sc[4] = {LBM_STORE_F32, LBM_STORE_F16, LBM_STORE_BF16, LBM_STORE_F64};
     c.storage = code_of(r.storage, sn, sc, 4, -1);
     if (c.storage < 0) { why = "unknown storage '" + r.storage + "'"; return false; }
-    const char* gn[3] = {"none", "channel", "cpu3d"};
-    const int gc[3] = {LBM_GEOM_NONE, LBM_GEOM_CHANNEL, LBM_GEOM_CPU3D};
-    c.geometry = code_of(r.geometry, gn, gc, 3, -1);
+    const char* gn[4] = {"none", "channel", "cpu3d", "precursor"};
+    const int gc[4] = {LBM_GEOM_NONE, LBM_GEOM_CHANNEL, LBM_GEOM_CPU3D, LBM_GEOM_PRECURSOR};
+    c.geometry = code_of(r.geometry, gn, gc, 4, -1);
     if (c.geometry < 0) { why = "unknown geometry '" + r.geometry + "'"; return false; }
     c.omega = (float)r.omega;
     c.cs = r.les ? (float)r.cs : 0.f;
@@ -288,6 +299,12 @@ bool to_config(const Run& r, lbm_config& c, std::string& why)
     c.kp = (float)r.kp;
     c.ctrl_every = r.ctrl_every;
     c.ctrl_start = r.ctrl_start;
+    c.dyn_mode = r.dyn_mode;
+    c.dyn_y2 = r.dyn_y2;
+    c.obst2_x = (float)r.obst2[0]; c.obst2_y = (float)r.obst2[1]; c.obst2_z = (float)r.obst2[2];
+    c.obst2_r = (float)r.obst2_r;
+    c.trip_until = r.trip_until;
+    c.trip_z = r.trip_z;
     return true;
 }
 
@@ -367,13 +384,16 @@ int main(int argc, char** argv)
     }
     const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (lbm_write_tecplot(ctx, casename.c_str())) return die("lbm_write_tecplot", ctx);
-    // WriteForces: "t, Fx/ref, Fy/ref, Fz/ref", ref = UMAX^2 ZDIM OBSTR1
+    // WriteForces: "t, Fx/ref, Fy/ref, Fz/ref", ref = UMAX^2 ZDIM OBSTR1, times
+    // LRLEVEL^2 = 4 with refinement, where the force is the fine level's
+    // (WriteForces(..., REFINEMENT*LRLEVEL), MRTtest/DynInlet3.cu:2790, :3300)
     int nf = 0;
     if (lbm_get_forces(ctx, nullptr, 0, &nf)) return die("lbm_get_forces", ctx);
     if (nf > 0) {
         std::vector<double> F(3 * (size_t)nf);
         if (lbm_get_forces(ctx, F.data(), nf, &nf)) return die("lbm_get_forces", ctx);
-        const float ref = (float)r.umax * (float)r.umax * (float)r.nz * (float)r.obst1_r;
+        float ref = (float)r.umax * (float)r.umax * (float)r.nz * (float)r.obst1_r;
+        if (nlev == 2) ref *= 4.f;
         std::ofstream o(casename + ".force");
         for (int i = 0; i < nf; ++i)
             o << i + r.startf << ", " << (float)F[3 * i] / ref << ", " << (float)F[3 * i + 1] / ref << ", "

@@ -41,6 +41,8 @@ struct Level {
     PopShift shift{};              // 16-bit storage offsets (equilibrium of the inflow)
     int blocks = 0;    // workgroups of the one-node-per-lane kernels (Shape::NW row segments each)
     size_t bytes = 0;
+    int* fslot = nullptr;  // device: force partial slot per stream-collide workgroup, -1 without force nodes
+    int nfslot = 0;        // workgroups holding a force node (partials one recorded step writes)
 };
 
 } // namespace
@@ -52,7 +54,10 @@ struct lbm_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     Level lv[2];
     int nlevels = 1;
-    float* u_in = nullptr;     // inlet profile [nz*ny]
+    float* u_in = nullptr;     // west inlet profile [nz*ny]
+    float* v_in = nullptr;     // south inlet profile [nz*nx]
+    bool user_mask = false;    // the coarse mask came from the caller (no generator, no trip switch)
+    bool trip_on = false;      // the classified mask holds the trip obstacle
     double* fpart = nullptr;   // force partials per block
     double* fstage = nullptr;  // first-stage sums of the force partials (k_reduce_chunks)
     double* fhist = nullptr;   // [force_steps*3]
@@ -109,7 +114,8 @@ static void classify_fine(lbm_ctx* ctx, bool user_mask);
 static int refine_extract(lbm_ctx* ctx, const float* Mold, float* M, float* Pnew, float* Phalf);
 static int refine_cf(lbm_ctx* ctx, const float* P);
 static int refine_fc(lbm_ctx* ctx);
-static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f);
+static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f, bool record);
+static int set_geometry_at(lbm_ctx* ctx, const uint8_t* mask, long long t);
 
 static int fail(lbm_ctx* ctx, int code, const std::string& msg)
 {
@@ -193,6 +199,10 @@ extern "C" void lbm_config_default(lbm_config* c)
     c->kp = 0.0f;
     c->ctrl_every = 100;
     c->ctrl_start = 10000;
+    c->dyn_mode = 0;
+    c->dyn_y2 = 1; // DYNY2 of the precursor programs (bgk_channel1/Re3000_Cyl2/DynInlet3.cu:26)
+    c->trip_until = 0;
+    c->trip_z = -1;
 }
 
 // ---------------------------------------------------------------- helpers
@@ -266,8 +276,14 @@ static int validate(const lbm_config* c, std::string& why)
     if (c->cs < 0.0f) { why = "CS must be >= 0"; return 0; }
     if (c->cs > 0.0f && c->model == LBM_MODEL_MRT_CPU3D) { why = "MRT_CPU3D has no LES"; return 0; }
     if (c->force_steps < 0 || c->startf < 0) { why = "force window must be >= 0"; return 0; }
-    if (c->geometry < 0 || c->geometry > 2) { why = "unknown geometry"; return 0; }
+    if (c->geometry < 0 || c->geometry > 3) { why = "unknown geometry"; return 0; }
     if (c->dyn_y1 < -1 || c->dyn_y1 >= c->ny) { why = "DYNY1 must be -1 or a row of the domain"; return 0; }
+    if (c->dyn_mode < 0 || c->dyn_mode > 1) { why = "the dynamic-inlet mode must be 0 or 1"; return 0; }
+    if (c->dyn_mode == 1 && c->dyn_y1 < 1) { why = "the periodic precursor (mode 1) needs DYNY1 >= 1"; return 0; }
+    if (c->dyn_y2 < 0 || c->dyn_y2 >= c->ny) { why = "DYNY2 must be a row of the domain"; return 0; }
+    if (c->trip_until < 0) { why = "the trip window must be >= 0"; return 0; }
+    if (c->geometry == LBM_GEOM_PRECURSOR && (c->dyn_y1 < 0 || c->dyn_y1 + 2 >= c->ny))
+        { why = "the precursor geometry needs 0 <= DYNY1 < ny - 2 (recycling plane DYNY1+1 below the outlet)"; return 0; }
     if (c->ctrl_every < 0 || c->ctrl_start < 0) { why = "the inlet control interval and start must be >= 0"; return 0; }
     if (c->kp != 0.0f && !(c->umax > 0.0f)) { why = "the inlet P-control (KP) needs UMAX > 0"; return 0; }
     if (c->model == LBM_MODEL_MRT_DHUMIERES)
@@ -302,7 +318,8 @@ static int alloc_level(lbm_ctx* ctx, Level& L, int nx, int ny, int nz, int stora
     // inflow-driven run the stored deviations are small, so the 11 (FP16) or
     // 8 (BF16) significant bits go to the flow's variation, not its mean
     {
-        const float U = ctx->cfg.umax;
+        // (the offset table is keyed on c_x: y-streamwise geometries keep the plain f_i - w_i)
+        const float U = ctx->cfg.geometry == LBM_GEOM_PRECURSOR ? 0.f : ctx->cfg.umax;
         for (int i = 0; i < Q; ++i) {
             const float w = float(WNUM[i]) * (1.0f / 36.0f);
             const float cu = float(CX[i]) * U;
@@ -319,6 +336,8 @@ static int alloc_level(lbm_ctx* ctx, Level& L, int nx, int ny, int nz, int stora
     HIP_TRY(ctx, hipMemsetAsync(L.fa, 0, lattice, ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(L.fb, 0, lattice, ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(L.mask, 0, (size_t)L.qs, ctx->stream));
+    HIP_TRY(ctx, hipMalloc(&L.fslot, (size_t)L.blocks * sizeof(int)));
+    HIP_TRY(ctx, hipMemsetAsync(L.fslot, 0xff, (size_t)L.blocks * sizeof(int), ctx->stream));
     L.host_mask.assign(L.cells, 0);
     L.bytes = 2 * lattice + (size_t)L.qs;
     return LBM_OK;
@@ -330,9 +349,11 @@ static void free_level(Level& L)
     if (L.fb) (void)hipFree(L.fb);
     if (L.mask) (void)hipFree(L.mask);
     if (L.stat) (void)hipFree(L.stat);
+    if (L.fslot) (void)hipFree(L.fslot);
     L.fa = L.fb = nullptr;
     L.mask = nullptr;
     L.stat = nullptr;
+    L.fslot = nullptr;
 }
 
 extern "C" int lbm_create(const lbm_config* cfg, lbm_ctx** out)
@@ -371,22 +392,10 @@ extern "C" int lbm_create(const lbm_config* cfg, lbm_ctx** out)
         return bail(rc);
     }
     Level& L = ctx->lv[0];
-    std::vector<float> prof((size_t)cfg->ny * cfg->nz);
+    std::vector<float> prof((size_t)cfg->ny * cfg->nz), vprof((size_t)cfg->nx * cfg->nz);
     inlet_profile(cfg->ny, cfg->nz, cfg->umax, cfg->inlet_amp, cfg->seed, prof.data());
-    if (hipMalloc(&ctx->u_in, prof.size() * sizeof(float)) != hipSuccess ||
-        hipMalloc(&ctx->fpart, (size_t)L.blocks * 3 * sizeof(double)) != hipSuccess ||
-        hipMalloc(&ctx->fstage, (size_t)(L.blocks / REDUCE_CHUNK + 1) * 3 * sizeof(double)) != hipSuccess ||
-        hipMalloc(&ctx->mpart, (size_t)std::max(L.blocks, MM_GRID_MAX) * 4 * sizeof(double)) != hipSuccess ||
-        hipMalloc(&ctx->red, 8 * sizeof(double)) != hipSuccess)
-        return bail(fail(nullptr, LBM_ENOMEM, "lbm_create: device allocation failed"));
-    if ((rc = noalloc_scratch(nullptr, dev, &ctx->scratch)) != LBM_OK) return bail(rc);
-    if (cfg->force_steps > 0) {
-        if (hipMalloc(&ctx->fhist, (size_t)cfg->force_steps * 3 * sizeof(double)) != hipSuccess)
-            return bail(fail(nullptr, LBM_ENOMEM, "lbm_create: force history allocation failed"));
-        (void)hipMemsetAsync(ctx->fhist, 0, (size_t)cfg->force_steps * 3 * sizeof(double), ctx->stream);
-    }
-    (void)hipMemcpyAsync(ctx->u_in, prof.data(), prof.size() * sizeof(float), hipMemcpyHostToDevice,
-                         ctx->stream);
+    // the south inlet's table: the same generator over (x, z)
+    inlet_profile(cfg->nx, cfg->nz, cfg->umax, cfg->inlet_amp, cfg->seed, vprof.data());
     if (cfg->refinement) {
         ctx->nlevels = 2;
         if ((rc = alloc_level(ctx, ctx->lv[1], cfg->lr_nx, cfg->lr_ny, cfg->lr_nz, cfg->storage)) != LBM_OK) {
@@ -398,6 +407,26 @@ extern "C" int lbm_create(const lbm_config* cfg, lbm_ctx** out)
             return bail(rc);
         }
     }
+    // force partials: at most one per workgroup of either level's stream-collide
+    // (or of the closure collide, which uses four-wavefront workgroups)
+    const int maxblocks = std::max(L.blocks, ctx->nlevels > 1 ? ctx->lv[1].blocks : 0);
+    if (hipMalloc(&ctx->u_in, prof.size() * sizeof(float)) != hipSuccess ||
+        hipMalloc(&ctx->v_in, vprof.size() * sizeof(float)) != hipSuccess ||
+        hipMalloc(&ctx->fpart, (size_t)maxblocks * 3 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ctx->fstage, (size_t)(maxblocks / REDUCE_CHUNK + 1) * 3 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ctx->mpart, (size_t)std::max(L.blocks, MM_GRID_MAX) * 4 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ctx->red, 8 * sizeof(double)) != hipSuccess)
+        return bail(fail(nullptr, LBM_ENOMEM, "lbm_create: device allocation failed"));
+    if ((rc = noalloc_scratch(nullptr, dev, &ctx->scratch)) != LBM_OK) return bail(rc);
+    if (cfg->force_steps > 0) {
+        if (hipMalloc(&ctx->fhist, (size_t)cfg->force_steps * 3 * sizeof(double)) != hipSuccess)
+            return bail(fail(nullptr, LBM_ENOMEM, "lbm_create: force history allocation failed"));
+        (void)hipMemsetAsync(ctx->fhist, 0, (size_t)cfg->force_steps * 3 * sizeof(double), ctx->stream);
+    }
+    (void)hipMemcpyAsync(ctx->u_in, prof.data(), prof.size() * sizeof(float), hipMemcpyHostToDevice,
+                         ctx->stream);
+    (void)hipMemcpyAsync(ctx->v_in, vprof.data(), vprof.size() * sizeof(float), hipMemcpyHostToDevice,
+                         ctx->stream);
     if (hipStreamSynchronize(ctx->stream) != hipSuccess)
         return bail(fail(nullptr, LBM_EDEVICE, "lbm_create: setup failed on the device"));
     *out = ctx;
@@ -409,7 +438,7 @@ extern "C" void lbm_destroy(lbm_ctx* ctx)
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (int l = 0; l < 2; ++l) free_level(ctx->lv[l]);
-    for (void* p : {(void*)ctx->u_in, (void*)ctx->fpart, (void*)ctx->fstage, (void*)ctx->fhist, (void*)ctx->red, (void*)ctx->mpart,
+    for (void* p : {(void*)ctx->u_in, (void*)ctx->v_in, (void*)ctx->fpart, (void*)ctx->fstage, (void*)ctx->fhist, (void*)ctx->red, (void*)ctx->mpart,
                     (void*)ctx->Mold, (void*)ctx->Mnew, (void*)ctx->Pold, (void*)ctx->Pnew, (void*)ctx->Phalf,
                     (void*)ctx->dwall,
                     (void*)ctx->omega_field})
@@ -427,8 +456,8 @@ extern "C" const char* lbm_last_error(lbm_ctx* ctx)
 
 // ---------------------------------------------------------------- geometry
 
-// ImageFcn stand-ins (see LBM_GEOM_* in lbm.h).
-static unsigned char classify(const lbm_config& c, int x, int y, int z)
+// ImageFcn stand-ins (see LBM_GEOM_* in lbm.h), at step t.
+static unsigned char classify(const lbm_config& c, int x, int y, int z, long long t)
 {
     if (c.geometry == LBM_GEOM_CHANNEL) {
         // LaminarChannelCylinder1/LRMultiGPU6.cu:90-129: obstacle first, then
@@ -452,13 +481,52 @@ static unsigned char classify(const lbm_config& c, int x, int y, int z)
         if (x == c.nx - 1 || y == 0 || y == c.ny - 1 || z == 0 || z == c.nz - 1) return LBM_NODE_SOLID;
         return LBM_NODE_FLUID;
     }
+    if (c.geometry == LBM_GEOM_PRECURSOR) {
+        // bgk_channel1/Re3000_Cyl2/DynInlet3.cu:75-94 in its order: the trip
+        // blocks while t < trip_until, obstacle 1, the x walls, then the
+        // recycling plane and the north outlet
+        unsigned char v = LBM_NODE_FLUID;
+        if (t < c.trip_until) {
+            if (std::fabs((float)x - c.obst2_x) < c.obst2_r && std::fabs((float)y - c.obst2_y) < c.obst2_r)
+                v = LBM_NODE_SOLID;
+            if (std::fabs((float)x - c.obst2_x - 3.f) < c.obst2_r && std::fabs((float)y - c.obst2_y - 3.f) < c.obst2_r &&
+                z == c.trip_z)
+                v = LBM_NODE_SOLID;
+        }
+        if (std::fabs((float)x - c.obst_x) < c.obst_r && std::fabs((float)y - c.obst_y) < c.obst_r)
+            v = LBM_NODE_SOLID_FORCE;
+        if (x == 0) v = LBM_NODE_SOLID;
+        else if (x == c.nx - 1) v = LBM_NODE_SOLID;
+        else if (y == c.dyn_y1 + 1) v = LBM_NODE_RECYCLE;
+        else if (y == c.ny - 1) v = LBM_NODE_OUTLET_NORTH;
+        return v;
+    }
     return LBM_NODE_FLUID;
+}
+
+// Neighbour coordinate along one axis under the level's rule; false when the
+// pull leaves the domain through a wall face.  ywrap: the precursor's
+// periodic rows 0 .. ywrap (y axis only; -1: none).
+static bool nb_coord(int c, int d, int n, int bc, int ywrap, int& out)
+{
+    if (ywrap >= 0) {
+        if (c == ywrap && d > 0) { out = 0; return true; }
+        if (c == 0 && d < 0) { out = ywrap; return true; }
+    }
+    int s = c + d;
+    if (s < 0 || s >= n) {
+        if (bc == LBM_BC_PERIODIC) s = (s + n) % n;
+        else if (bc == LBM_BC_CLAMP) s = s < 0 ? 0 : n - 1;
+        else return false;
+    }
+    out = s;
+    return true;
 }
 
 // Device mask for one level: codes plus the NEAR bit on nodes with a link
 // that bounces (out through a wall face, or from a solid node in link mode),
 // and NEAR_SOLID where one of those links comes from a solid node.
-static void build_device_mask(const Level& L, const int bc[3], int halfway, std::vector<unsigned char>& dev)
+static void build_device_mask(const Level& L, const int bc[3], int halfway, int ywrap, std::vector<unsigned char>& dev)
 {
     dev.assign((size_t)L.qs, 0);
     const int n[3] = {L.nx, L.ny, L.nz};
@@ -473,14 +541,8 @@ static void build_device_mask(const Level& L, const int bc[3], int halfway, std:
                     const int cc[3] = {CX[i], CY[i], CZ[i]};
                     int s[3];
                     bool out = false;
-                    for (int a = 0; a < 3; ++a) {
-                        s[a] = p[a] - cc[a];
-                        if (s[a] < 0 || s[a] >= n[a]) {
-                            if (bc[a] == LBM_BC_PERIODIC) s[a] = (s[a] + n[a]) % n[a];
-                            else if (bc[a] == LBM_BC_CLAMP) s[a] = s[a] < 0 ? 0 : n[a] - 1;
-                            else out = true;
-                        }
-                    }
+                    for (int a = 0; a < 3; ++a)
+                        if (!nb_coord(p[a], -cc[a], n[a], bc[a], a == 1 ? ywrap : -1, s[a])) out = true;
                     if (out) near = true;
                     else if (halfway && !is_solid(code)) {
                         const size_t sc = (size_t)s[0] + (size_t)L.nx * (s[1] + (size_t)L.ny * s[2]);
@@ -492,36 +554,93 @@ static void build_device_mask(const Level& L, const int bc[3], int halfway, std:
             }
 }
 
-extern "C" int lbm_set_geometry(lbm_ctx* ctx, const uint8_t* mask)
+// Force slots of one level's stream-collide workgroups: a workgroup gets the
+// next slot (in workgroup order) when one of its nodes is a force-summed
+// solid or pulls from one, else -1; returns the slot count.  Only those
+// workgroups write a momentum-exchange partial, and the reduction sums just
+// them (deterministic, fixed order).
+static int build_force_slots(const Level& L, const int bc[3], int ywrap, std::vector<int>& slot)
 {
-    if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_set_geometry: null context");
+    const int nw = nwave_of_storage(L.storage);
+    std::vector<char> has((size_t)L.blocks, 0);
+    const int n[3] = {L.nx, L.ny, L.nz};
+    for (int z = 0; z < L.nz; ++z)
+        for (int y = 0; y < L.ny; ++y)
+            for (int x = 0; x < L.nx; ++x) {
+                if (L.host_mask[(size_t)x + (size_t)L.nx * (y + (size_t)L.ny * z)] != LBM_NODE_SOLID_FORCE) continue;
+                const int p[3] = {x, y, z};
+                for (int i = 0; i < Q; ++i) {
+                    const int cc[3] = {CX[i], CY[i], CZ[i]};
+                    int s[3];
+                    bool in = true;
+                    for (int a = 0; a < 3; ++a)
+                        if (!nb_coord(p[a], cc[a], n[a], bc[a], a == 1 ? ywrap : -1, s[a])) in = false;
+                    if (!in) continue;
+                    // the node that pulls along c_i from this solid (or the solid itself)
+                    const long long seg = ((long long)s[2] * L.ny + s[1]) * L.nsegw + s[0] / 64;
+                    has[(size_t)(seg / nw)] = 1;
+                }
+            }
+    slot.assign((size_t)L.blocks, -1);
+    int k = 0;
+    for (int b = 0; b < L.blocks; ++b)
+        if (has[(size_t)b]) slot[(size_t)b] = k++;
+    return k;
+}
+
+// The rows 0 .. ywrap of the precursor wrap (dyn_mode 1), or -1.
+static int y_wrap_of(const lbm_config& c) { return c.dyn_mode == 1 ? c.dyn_y1 : -1; }
+
+static int upload_level_mask(lbm_ctx* ctx, Level& L, const int bc[3], int ywrap)
+{
+    std::vector<unsigned char> dev;
+    build_device_mask(L, bc, ctx->cfg.halfway, ywrap, dev);
+    HIP_TRY(ctx, hipMemcpyAsync(L.mask, dev.data(), dev.size(), hipMemcpyHostToDevice, ctx->stream));
+    std::vector<int> slot;
+    L.nfslot = build_force_slots(L, bc, ywrap, slot);
+    HIP_TRY(ctx, hipMemcpyAsync(L.fslot, slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    // both copies read host vectors that end with this call
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return LBM_OK;
+}
+
+// Classify (from the generator at step t, or the caller's mask) and upload
+// every level's mask and force slots.
+static int set_geometry_at(lbm_ctx* ctx, const uint8_t* mask, long long t)
+{
     Level& L = ctx->lv[0];
     const lbm_config& c = ctx->cfg;
     if (mask) {
         for (size_t i = 0; i < L.cells; ++i)
-            if (mask[i] > LBM_NODE_SYM_XMAX)
+            if (mask[i] > LBM_NODE_MAX)
                 return fail(ctx, LBM_EINVAL, "lbm_set_geometry: unknown node code " + std::to_string(mask[i]));
+        for (size_t i = 0; i < L.cells; ++i)
+            if (mask[i] == LBM_NODE_GHOST_S && c.dyn_y1 < 1)
+                return fail(ctx, LBM_EINVAL, "lbm_set_geometry: the im 52 ghost copy reads row DYNY1-1: DYNY1 must be >= 1");
         std::memcpy(L.host_mask.data(), mask, L.cells);
     } else {
         for (int z = 0; z < L.nz; ++z)
             for (int y = 0; y < L.ny; ++y)
                 for (int x = 0; x < L.nx; ++x)
-                    L.host_mask[(size_t)x + (size_t)L.nx * (y + (size_t)L.ny * z)] = classify(c, x, y, z);
+                    L.host_mask[(size_t)x + (size_t)L.nx * (y + (size_t)L.ny * z)] = classify(c, x, y, z, t);
     }
-    std::vector<unsigned char> dev;
-    build_device_mask(L, c.bc, c.halfway, dev);
-    HIP_TRY(ctx, hipMemcpyAsync(L.mask, dev.data(), dev.size(), hipMemcpyHostToDevice, ctx->stream));
+    ctx->user_mask = mask != nullptr;
+    ctx->trip_on = !mask && c.geometry == LBM_GEOM_PRECURSOR && t < c.trip_until;
+    if (int rc = upload_level_mask(ctx, L, c.bc, y_wrap_of(c))) return rc;
     if (ctx->nlevels > 1) {
         Level& F = ctx->lv[1];
         classify_fine(ctx, mask != nullptr);
         const int fbc[3] = {LBM_BC_CLAMP, LBM_BC_CLAMP, LBM_BC_CLAMP};
-        std::vector<unsigned char> fdev;
-        build_device_mask(F, fbc, c.halfway, fdev);
-        HIP_TRY(ctx, hipMemcpyAsync(F.mask, fdev.data(), fdev.size(), hipMemcpyHostToDevice, ctx->stream));
+        if (int rc = upload_level_mask(ctx, F, fbc, -1)) return rc;
     }
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->geometry_set = true;
     return LBM_OK;
+}
+
+extern "C" int lbm_set_geometry(lbm_ctx* ctx, const uint8_t* mask)
+{
+    if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_set_geometry: null context");
+    return set_geometry_at(ctx, mask, ctx->t);
 }
 
 extern "C" int lbm_get_mask(lbm_ctx* ctx, int level, uint8_t* mask)
@@ -567,7 +686,8 @@ static int init_level(lbm_ctx* ctx, int level, int kind, const float* rho, const
     int rc = with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         hipLaunchKernelGGL(k_init<S>, dim3(grid_for(L.cells)), dim3(256), 0, ctx->stream, (S*)L.fa, (S*)L.fb,
-                           L.qs, L.nx, L.ny, L.nz, L.nxp, L.mask, d[0], d[1], d[2], d[3], u0, kind, L.shift);
+                           L.qs, L.nx, L.ny, L.nz, L.nxp, L.mask, d[0], d[1], d[2], d[3], u0,
+                           (int)(ctx->cfg.geometry == LBM_GEOM_PRECURSOR), kind, L.shift);
         return LBM_OK;
     });
     HIP_TRY(ctx, hipGetLastError());
@@ -644,10 +764,17 @@ static StepParams step_params(lbm_ctx* ctx, const Level& L, float omega)
     // as a body force of 0.5 per fine step; that is not reproduced.
     const bool fine = &L != &ctx->lv[0];
     p.dpdy = fine ? 0.5f * ctx->dpdy : ctx->dpdy;
-    // rows past DYNY1 get no force (:1744); fine row j sits at coarse y = lr_y0 + j/2
+    // rows past DYNY1 get no force (:1744); fine row j sits at coarse y = lr_y0 + j/2.
+    // The periodic precursor (dyn_mode 1) forces every row, as bgk_channel1 does.
     p.dyn_y1 = INT_MAX;
-    if (c.dyn_y1 >= 0)
+    if (c.dyn_y1 >= 0 && c.dyn_mode == 0)
         p.dyn_y1 = fine ? (int)std::floor(2.0f * ((float)c.dyn_y1 - c.lr_y0)) : c.dyn_y1;
+    p.v_in = ctx->v_in;
+    p.fslot = nullptr;
+    p.y_wrap = fine ? -1 : y_wrap_of(c);
+    // ghost sources: im 52 row DYNY1-1, im 53 row DYNY2, im 54 row 1 (clamped into the level)
+    const int gy[3] = {c.dyn_y1 - 1, c.dyn_y2, 1};
+    for (int k = 0; k < 3; ++k) p.ghost_y[k] = std::min(std::max(gy[k], 0), L.ny - 1);
     p.u_lid = c.umax;
     p.s1 = c.s_dhum[1]; p.s2 = c.s_dhum[2]; p.s4 = c.s_dhum[4]; p.s10 = c.s_dhum[10]; p.s16 = c.s_dhum[16];
     p.omega_node = nullptr;
@@ -716,10 +843,23 @@ static long long launch_closure(lbm_ctx* ctx, const StepParams& p, const Level& 
 // (MRTtest/DynInlet3.cu:1819-1838).
 static void reduce_force_partials(lbm_ctx* ctx, long long nparts, double* out)
 {
+    if (nparts <= REDUCE_CHUNK) { // (no partials: the sum writes zeros)
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, ctx->stream, (const double*)ctx->fpart, nparts,
+                           3, out);
+        return;
+    }
     const long long chunks = (nparts + REDUCE_CHUNK - 1) / REDUCE_CHUNK;
     hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)chunks), dim3(256), 0, ctx->stream, ctx->fpart, nparts, 3,
                        ctx->fstage);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, ctx->stream, ctx->fstage, chunks, 3, out);
+}
+
+// Whether step t records its force: steps t > STARTF into slot t - STARTF
+// (main() sets flag_F for t > STARTF, MRTtest/DynInlet3.cu:3130, :3183).
+static bool records_force(const lbm_ctx* ctx, long long t)
+{
+    const lbm_config& c = ctx->cfg;
+    return c.force_steps > 0 && t > c.startf && (t - c.startf) < c.force_steps;
 }
 
 // One coarse-level stream-collide launch (update_inn/top/bot on one GPU),
@@ -729,9 +869,14 @@ static int coarse_step(lbm_ctx* ctx)
     Level& L = ctx->lv[0];
     const lbm_config& c = ctx->cfg;
     StepParams p = step_params(ctx, L, c.omega);
-    const bool rec = c.force_steps > 0 && ctx->t >= c.startf && (ctx->t - c.startf) < c.force_steps;
-    if (rec) p.fpart = ctx->fpart;
-    long long nparts = L.blocks; // force partials written: one per workgroup
+    // with refinement the fine level records the force, as main() turns the
+    // coarse flag_F off under REFINEMENT (MRTtest/DynInlet3.cu:3130)
+    const bool rec = ctx->nlevels == 1 && records_force(ctx, ctx->t);
+    if (rec) {
+        p.fpart = ctx->fpart;
+        p.fslot = L.fslot;
+    }
+    long long nparts = L.nfslot; // force partials written: one per workgroup holding a force node
     with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         if (!ctx->mlp) {
@@ -757,17 +902,28 @@ static int coarse_step(lbm_ctx* ctx)
 }
 
 // Fine-level sub-step (update_inn_LR, MRTtest/DynInlet3.cu:2051-2201).
-static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f)
+// With record set (the first sub-step of a recorded coarse step, flag_F at
+// :3183) the fine level's momentum exchange goes to the force history.
+static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f, bool record)
 {
     Level& F = ctx->lv[1];
     const lbm_config& c = ctx->cfg;
     StepParams p = step_params(ctx, F, omega_f);
     p.bcx = p.bcy = p.bcz = LBM_BC_CLAMP;
+    if (record) {
+        p.fpart = ctx->fpart;
+        p.fslot = F.fslot;
+    }
     with_storage(F.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         launch_sc<S>(c.model, p, F.blocks, ctx->stream);
         return 0;
     });
+    if (record) {
+        const long long slot = ctx->t - c.startf;
+        reduce_force_partials(ctx, F.nfslot, ctx->fhist + slot * 3);
+        ctx->force_recorded = (int)std::max<long long>(ctx->force_recorded, slot + 1);
+    }
     std::swap(F.fa, F.fb);
     return LBM_OK;
 }
@@ -815,7 +971,7 @@ static void classify_fine(lbm_ctx* ctx, bool user_mask)
             for (int i = 0; i < F.nx; ++i) {
                 const float X = c.lr_x0 + 0.5f * i, Y = c.lr_y0 + 0.5f * j, Z = c.lr_z0 + 0.5f * k;
                 unsigned char v = LBM_NODE_FLUID;
-                if (user_mask || c.geometry != LBM_GEOM_CHANNEL) {
+                if (user_mask || (c.geometry != LBM_GEOM_CHANNEL && c.geometry != LBM_GEOM_PRECURSOR)) {
                     const int xi = std::min(std::max((int)std::lround(X), 0), L.nx - 1);
                     const int yi = std::min(std::max((int)std::lround(Y), 0), L.ny - 1);
                     const int zi = std::min(std::max((int)std::lround(Z), 0), L.nz - 1);
@@ -951,7 +1107,11 @@ extern "C" int lbm_step(lbm_ctx* ctx, int nsteps)
     if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_step: null context");
     if (!ctx->initialised) return fail(ctx, LBM_ESTATE, "lbm_step: call lbm_init_equilibrium first");
     if (nsteps < 0) return fail(ctx, LBM_EINVAL, "lbm_step: nsteps < 0");
+    const lbm_config& c = ctx->cfg;
     for (int s = 0; s < nsteps; ++s) {
+        // ImageFcn(x, y, z, t): the trip obstacle exists for t < trip_until
+        if (!ctx->user_mask && c.geometry == LBM_GEOM_PRECURSOR && ctx->trip_on != (ctx->t < c.trip_until))
+            if (int rc = set_geometry_at(ctx, nullptr, ctx->t)) return rc;
         if (int rc = coarse_step(ctx)) return rc;
         if (int rc = inlet_control(ctx, ctx->t)) return rc;
         if (ctx->nlevels > 1) {
@@ -960,9 +1120,9 @@ extern "C" int lbm_step(lbm_ctx* ctx, int nsteps)
             const bool temporal = ctx->cfg.temporal_interp;
             int rc = refine_extract(ctx, ctx->Mold, ctx->Mnew, ctx->Pnew, temporal ? ctx->Phalf : nullptr);
             if (!rc) rc = refine_cf(ctx, ctx->Pold);
-            if (!rc) rc = lbm_internal_fine_step(ctx, ctx->rg.omega_f);
+            if (!rc) rc = lbm_internal_fine_step(ctx, ctx->rg.omega_f, records_force(ctx, ctx->t));
             if (!rc && temporal) rc = refine_cf(ctx, ctx->Phalf);
-            if (!rc) rc = lbm_internal_fine_step(ctx, ctx->rg.omega_f);
+            if (!rc) rc = lbm_internal_fine_step(ctx, ctx->rg.omega_f, false);
             if (!rc) rc = refine_fc(ctx);
             if (rc) return rc;
             std::swap(ctx->Mold, ctx->Mnew);

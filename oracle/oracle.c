@@ -119,6 +119,8 @@ void orc_default_params(orc_params* p)
     p->s_dhum[10] = p->s_dhum[12] = 1.4;
     p->s_dhum[16] = p->s_dhum[17] = p->s_dhum[18] = 1.98;
     p->dyn_y1 = -1;
+    p->y_wrap = -1;
+    p->ghost_y[0] = p->ghost_y[1] = p->ghost_y[2] = 0;
 }
 
 #define FN(name) name##_f32

@@ -46,7 +46,16 @@ enum {
     ORC_OUTLET_EAST = 4,    /* im 300: East_Extrap rho=1 outlet */
     ORC_LID_WEST_REG = 5,   /* CPU_3D im 53: DirichletWest_Regularized */
     ORC_SYM_XMIN = 6,       /* im 26: xsymmetry_bot */
-    ORC_SYM_XMAX = 7        /* im 25: xsymmetry_top */
+    ORC_SYM_XMAX = 7,       /* im 25: xsymmetry_top */
+    ORC_OUTLET_NORTH = 8,   /* im 100: North_Extrap rho = 1 from the unstreamed y-1 node */
+    ORC_INLET_SOUTH = 9,    /* im 200: South_Extrap (0, v_in, 0) from the unstreamed y+1 node */
+    ORC_RECYCLE = 10,       /* im 80: rho of y+1, moments of y-1 (North_Extrap(f(y-1), rho(y+1))) */
+    ORC_VEL_FROM_BELOW = 11, /* im 70: velocity of y-1, moments of y+1 (South_Extrap(f(y+1), j(y-1))) */
+    ORC_GHOST_W = 12,       /* im 50..54: copies of an unstreamed node */
+    ORC_GHOST_E = 13,
+    ORC_GHOST_S = 14,
+    ORC_GHOST_N = 15,
+    ORC_GHOST_DYN = 16
 };
 
 typedef struct {
@@ -62,6 +71,10 @@ typedef struct {
     double s_dhum[19];  /* rates for ORC_MRT_DHUM (entries 9,11,13,14,15 ignored: omega) */
     int dyn_y1;         /* rows y > dyn_y1 get no body force, "if(y>DYNY1) dpdy = 0.f"
                            (MRTtest/DynInlet3.cu:1744); -1: every row gets it */
+    int y_wrap;         /* rows 0 .. y_wrap periodic in y among themselves
+                           (bgk_channel1/Re3000_Cyl2/DynInlet3.cu:1131-1132); -1: none */
+    int ghost_y[3];     /* source rows of the im 52 / 53 / 54 ghost copies */
+    const float* v_in;  /* south inlet profile [z*nx + x] (NULL: 0) */
 } orc_params;
 
 void orc_default_params(orc_params* p);
@@ -77,6 +90,7 @@ void orc_xsym_bot_f32(float f[19], int y, int z, int ny, int nz);
 void orc_xsym_top_f32(float f[19], int y, int z, int ny, int nz);
 void orc_les_omega_f32(const orc_params* p, const float f[19], float* omega_out, float* q_out);
 void orc_west_extrap_f32(float f[19], float u);
+void orc_vel_extrap_f32(float f[19], float u, float v, float w);
 void orc_east_extrap_f32(float f[19], float rho);
 void orc_lid_west_reg_f32(float f[19], float u_lid);
 void orc_feq_bgk_f32(float rho, float u, float v, float w, float f[19]);
@@ -96,6 +110,7 @@ void orc_xsym_bot_f64(double f[19], int y, int z, int ny, int nz);
 void orc_xsym_top_f64(double f[19], int y, int z, int ny, int nz);
 void orc_les_omega_f64(const orc_params* p, const double f[19], double* omega_out, double* q_out);
 void orc_west_extrap_f64(double f[19], double u);
+void orc_vel_extrap_f64(double f[19], double u, double v, double w);
 void orc_east_extrap_f64(double f[19], double rho);
 void orc_lid_west_reg_f64(double f[19], double u_lid);
 void orc_feq_bgk_f64(double rho, double u, double v, double w, double f[19]);
@@ -150,10 +165,12 @@ void orc_interp_cf_f32(const orc_refine* r, const float* Mold, const float* Mnew
 void orc_interp_fc_f32(const orc_params* pc, const orc_refine* r, const float* ff, const uint8_t* cmask,
                        float* fc);
 /* one coarse step with two fine sub-steps; Mold holds the coarse moments at
- * the step's start and is replaced by those at its end */
+ * the step's start and is replaced by those at its end; fine_force (may be
+ * NULL) receives the fine level's momentum exchange of the first sub-step */
 void orc_refine_step_f32(const orc_params* pc, const orc_params* pf, const orc_refine* r,
                          const uint8_t* cmask, const uint8_t* fmask, const float* u_in,
-                         float* ca, float* cb, float* fa, float* fb, float* Mold, float* Mnew);
+                         float* ca, float* cb, float* fa, float* fb, float* Mold, float* Mnew,
+                         double* fine_force);
 
 #ifdef __cplusplus
 }

@@ -208,8 +208,21 @@ void FN(orc_west_extrap)(R f[19], R u)
     FN(invert_moments)(m, f);
 }
 
-/* East_Extrap (MRTtest/DynInlet3.cu:926-971): rho prescribed, momentum and
- * non-conserved moments copied from the neighbour. */
+/* South_Extrap (MRTtest/DynInlet3.cu:881-924: j = (0, v, 0)) and its
+ * four-argument form of bgk_channel1/Re3000_Cyl2/DynInlet3.cu (j = (u, v, w),
+ * the im 70 plane): rho from the neighbour's sum, j prescribed, every
+ * non-conserved moment copied from the neighbour. */
+void FN(orc_vel_extrap)(R f[19], R u, R v, R w)
+{
+    R m[19];
+    FN(orc_moments)(f, m);
+    m[3] = u; m[5] = v; m[7] = w;
+    FN(invert_moments)(m, f);
+}
+
+/* East_Extrap (MRTtest/DynInlet3.cu:926-971) and North_Extrap (:834-879):
+ * rho prescribed, momentum and non-conserved moments copied from the
+ * neighbour. */
 void FN(orc_east_extrap)(R f[19], R rho)
 {
     R m[19];
@@ -330,6 +343,12 @@ void FN(orc_step)(const orc_params* p, const uint8_t* mask, const R* u_in,
                     int wall = 0;
                     for (int a = 0; a < 3; ++a) {
                         s[a] = xyz[a] - g_c[i][a];
+                        /* the precursor's periodic rows 0 .. y_wrap
+                           (bgk_channel1/Re3000_Cyl2/DynInlet3.cu:1131-1132) */
+                        if (a == 1 && p->y_wrap >= 0) {
+                            if (y == p->y_wrap && g_c[i][1] < 0) { s[1] = 0; continue; }
+                            if (y == 0 && g_c[i][1] > 0) { s[1] = p->y_wrap; continue; }
+                        }
                         if (s[a] < 0 || s[a] >= n[a]) {
                             if (p->bc[a] == ORC_PERIODIC) s[a] = (s[a] + n[a]) % n[a];
                             else if (p->bc[a] == ORC_CLAMP) s[a] = s[a] < 0 ? 0 : n[a] - 1;
@@ -386,6 +405,41 @@ void FN(orc_step)(const orc_params* p, const uint8_t* mask, const R* u_in,
                     FN(orc_xsym_bot)(f, y, z, ny, nz);
                 } else if (code == ORC_SYM_XMAX) {
                     FN(orc_xsym_top)(f, y, z, ny, nz);
+                } else if (code >= ORC_OUTLET_NORTH && code <= ORC_VEL_FROM_BELOW) {
+                    /* update_inn's y-streamwise conditions (MRTtest/DynInlet3.cu:1711-1724;
+                       bgk_channel1/Re3000_Cyl2/DynInlet3.cu:1220-1239) on the
+                       unstreamed rows y-1 / y+1 (f_mem wraps y, :177-185) */
+                    const size_t bc_ = (size_t)x + (size_t)nx * ((y > 0 ? y - 1 : ny - 1) + (size_t)ny * z);
+                    const size_t ac_ = (size_t)x + (size_t)nx * ((y + 1 < ny ? y + 1 : 0) + (size_t)ny * z);
+                    if (code == ORC_OUTLET_NORTH) {
+                        for (int i = 0; i < 19; ++i) f[i] = fin[i * N + bc_];
+                        FN(orc_east_extrap)(f, (R)1);
+                    } else if (code == ORC_INLET_SOUTH) {
+                        for (int i = 0; i < 19; ++i) f[i] = fin[i * N + ac_];
+                        FN(orc_vel_extrap)(f, (R)0, p->v_in ? (R)p->v_in[(size_t)z * nx + x] : (R)0, (R)0);
+                    } else if (code == ORC_RECYCLE) {
+                        R rho = 0;
+                        for (int i = 0; i < 19; ++i) rho += fin[i * N + ac_];
+                        for (int i = 0; i < 19; ++i) f[i] = fin[i * N + bc_];
+                        FN(orc_east_extrap)(f, rho);
+                    } else {
+                        R j[3] = {0, 0, 0};
+                        for (int i = 0; i < 19; ++i)
+                            for (int a = 0; a < 3; ++a) j[a] += (R)g_c[i][a] * fin[i * N + bc_];
+                        for (int i = 0; i < 19; ++i) f[i] = fin[i * N + ac_];
+                        FN(orc_vel_extrap)(f, j[0], j[1], j[2]);
+                    }
+                } else if (code >= ORC_GHOST_W && code <= ORC_GHOST_DYN) {
+                    /* the ghost copies follow the collide and overwrite it with
+                       an unstreamed node (MRTtest/DynInlet3.cu:1747-1771) */
+                    int sx = x, sy = y;
+                    if (code == ORC_GHOST_W) sx = nx - 2;
+                    else if (code == ORC_GHOST_E) sx = 1;
+                    else sy = p->ghost_y[code - ORC_GHOST_S];
+                    const size_t sc_ = (size_t)sx + (size_t)nx * (sy + (size_t)ny * z);
+                    for (int i = 0; i < 19; ++i) fout[i * N + cell] = fin[i * N + sc_];
+                    F0 += F[0]; F1 += F[1]; F2 += F[2];
+                    continue;
                 }
                 FN(orc_collide)((p->dyn_y1 >= 0 && y > p->dyn_y1) ? &p_noforce : p, f);
                 for (int i = 0; i < 19; ++i) fout[i * N + cell] = f[i];

@@ -269,7 +269,8 @@ void orc_interp_fc_f32(const orc_params* pc, const orc_refine* r, const float* f
 
 void orc_refine_step_f32(const orc_params* pc, const orc_params* pf, const orc_refine* r,
                          const uint8_t* cmask, const uint8_t* fmask, const float* u_in,
-                         float* ca, float* cb, float* fa, float* fb, float* Mold, float* Mnew)
+                         float* ca, float* cb, float* fa, float* fb, float* Mold, float* Mnew,
+                         double* fine_force)
 {
     const size_t N = (size_t)pc->nx * pc->ny * pc->nz;
     const size_t NF = (size_t)pf->nx * pf->ny * pf->nz;
@@ -280,7 +281,9 @@ void orc_refine_step_f32(const orc_params* pc, const orc_params* pf, const orc_r
     for (int s = 0; s < 2; ++s) {
         if (s == 0) orc_interp_cf_f32(r, Mold, Mnew, 0.f, fmask, fa);
         else if (r->temporal) orc_interp_cf_f32(r, Mold, Mnew, 0.5f, fmask, fa);
-        orc_step_f32(pf, fmask, 0, fa, fb, 0);
+        /* the fine level's momentum exchange of its first sub-step
+           (update_inn_LR with flag_F on i == 0, MRTtest/DynInlet3.cu:3183) */
+        orc_step_f32(pf, fmask, 0, fa, fb, s == 0 ? fine_force : 0);
         memcpy(fa, fb, sizeof(float) * 19 * NF);
     }
     orc_interp_fc_f32(pc, r, fa, cmask, ca);

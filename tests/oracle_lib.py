@@ -15,14 +15,16 @@ REF_EXE = os.path.join(ROOT, "oracle", "_ref", "cpu3d_ref")
 
 BGK, MRT_CPU3D, MRT_LES, MRT_DHUM = 0, 1, 2, 3
 PERIODIC, CLAMP, WALL = 0, 1, 2
-FLUID, SOLID, SOLID_FORCE, INLET_WEST, OUTLET_EAST, LID_WEST_REG, SYM_XMIN, SYM_XMAX = 0, 1, 2, 3, 4, 5, 6, 7
+(FLUID, SOLID, SOLID_FORCE, INLET_WEST, OUTLET_EAST, LID_WEST_REG, SYM_XMIN, SYM_XMAX, OUTLET_NORTH, INLET_SOUTH,
+ RECYCLE, VEL_FROM_BELOW, GHOST_W, GHOST_E, GHOST_S, GHOST_N, GHOST_DYN) = range(17)
 
 
 class OrcParams(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
                 ("bc", C.c_int * 3), ("collision", C.c_int), ("halfway", C.c_int),
                 ("omega", C.c_double), ("cs", C.c_double), ("dpdy", C.c_double),
-                ("u_lid", C.c_double), ("s_dhum", C.c_double * 19), ("dyn_y1", C.c_int)]
+                ("u_lid", C.c_double), ("s_dhum", C.c_double * 19), ("dyn_y1", C.c_int),
+                ("y_wrap", C.c_int), ("ghost_y", C.c_int * 3), ("v_in", C.c_void_p)]
 
 
 _lib = None
@@ -46,6 +48,8 @@ def lib():
                                                                  C.c_float, C.c_float]
             getattr(_lib, "orc_west_extrap_" + suf).argtypes = [fp, C.c_float if suf == "f32" else C.c_double]
             getattr(_lib, "orc_east_extrap_" + suf).argtypes = [fp, C.c_float if suf == "f32" else C.c_double]
+            sct = C.c_float if suf == "f32" else C.c_double
+            getattr(_lib, "orc_vel_extrap_" + suf).argtypes = [fp, sct, sct, sct]
             getattr(_lib, "orc_lid_west_reg_" + suf).argtypes = [fp, C.c_float if suf == "f32" else C.c_double]
             getattr(_lib, "orc_xsym_bot_" + suf).argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int]
             getattr(_lib, "orc_xsym_top_" + suf).argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int]
@@ -79,7 +83,9 @@ def ref():
 
 
 def params(nx=1, ny=1, nz=1, bc=(PERIODIC,) * 3, collision=BGK, halfway=0, omega=1.0,
-           cs=0.0, dpdy=0.0, u_lid=0.0, dyn_y1=-1):
+           cs=0.0, dpdy=0.0, u_lid=0.0, dyn_y1=-1, y_wrap=-1, ghost_y=(0, 0, 0), v_in=None):
+    """Oracle parameters.  v_in (float32 [nz*nx], the south inlet table) is
+    kept alive on the returned object."""
     p = OrcParams()
     lib().orc_default_params(C.byref(p))
     p.nx, p.ny, p.nz = nx, ny, nz
@@ -88,6 +94,12 @@ def params(nx=1, ny=1, nz=1, bc=(PERIODIC,) * 3, collision=BGK, halfway=0, omega
     p.collision, p.halfway = collision, halfway
     p.omega, p.cs, p.dpdy, p.u_lid = omega, cs, dpdy, u_lid
     p.dyn_y1 = dyn_y1
+    p.y_wrap = y_wrap
+    for k in range(3):
+        p.ghost_y[k] = ghost_y[k]
+    if v_in is not None:
+        p._v_in = np.ascontiguousarray(v_in, np.float32)
+        p.v_in = p._v_in.ctypes.data
     return p
 
 
@@ -208,7 +220,8 @@ def _bind_refine():
     L.orc_interp_cf_f32.argtypes = [C.POINTER(OrcRefine), f32p, f32p, C.c_float, C.c_void_p, f32p]
     L.orc_interp_fc_f32.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcRefine), f32p, C.c_void_p, f32p]
     L.orc_refine_step_f32.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcParams), C.POINTER(OrcRefine),
-                                      C.c_void_p, C.c_void_p, C.c_void_p, f32p, f32p, f32p, f32p, f32p, f32p]
+                                      C.c_void_p, C.c_void_p, C.c_void_p, f32p, f32p, f32p, f32p, f32p, f32p,
+                                      C.c_void_p]
     L._refine_bound = True
     return L
 
@@ -245,18 +258,21 @@ def interp_fc(pc, r, ff, fc, cmask=None):
     return fc
 
 
-def refine_run(pc, pf, r, ca, fa, nsteps, cmask=None, fmask=None, u_in=None):
-    """Advance coarse (ca) and fine (fa) lattices nsteps coarse steps."""
+def refine_run(pc, pf, r, ca, fa, nsteps, cmask=None, fmask=None, u_in=None, forces=False):
+    """Advance coarse (ca) and fine (fa) lattices nsteps coarse steps; with
+    forces, also the fine level's first-sub-step momentum exchange of each
+    coarse step, [nsteps, 3]."""
     L = _bind_refine()
     ca = np.ascontiguousarray(ca, np.float32).copy()
     fa = np.ascontiguousarray(fa, np.float32).copy()
     cb, fb = np.empty_like(ca), np.empty_like(fa)
     Mold = extract(pc, r, ca)
     Mnew = np.empty_like(Mold)
-    for _ in range(nsteps):
+    F = np.zeros((max(nsteps, 1), 3), np.float64)
+    for t in range(nsteps):
         L.orc_refine_step_f32(C.byref(pc), C.byref(pf), C.byref(r), _ptr(cmask), _ptr(fmask), _ptr(u_in),
-                              ca, cb, fa, fb, Mold, Mnew)
-    return ca, fa
+                              ca, cb, fa, fb, Mold, Mnew, F[t].ctypes.data)
+    return (ca, fa, F) if forces else (ca, fa)
 
 
 def stats_update(p, f, stat, mode, n, w=1.0, mask=None):

@@ -110,3 +110,78 @@ def test_driver_run_matches_solver(tmp_path):
     assert force.shape == (40, 4)
     assert np.array_equal(force[:, 0], np.arange(20, 60))
     assert np.abs(force[:, 1:] - F / ref).max() < 1e-5 * np.abs(F / ref).max()
+    assert np.all(force[0, 1:] == 0)  # step STARTF itself is not recorded (t > STARTF)
+
+
+POST = os.path.join(ROOT, "lbm_multigpu_amd", "lbm_post")
+
+
+@pytest.mark.gpu
+def test_precursor_runs_end_to_end_through_lbm_run_and_lbm_post(tmp_path):
+    """A small bgk_channel1/Re3000_Cyl2-style case: the periodic precursor
+    rows 0..DYNY1 under the P-controlled body force, the im 80 recycling
+    plane, a trip obstacle removed at step 100, the force-summed cylinder and
+    the north outlet; lbm_run writes .dat/.force/.vel/_slice.dat equal to the
+    Solver's run, and lbm_post profiles (VelProf2) reads the slice."""
+    cfg = C.precursor(nx=42, ny=96, nz=4, dyn_y1=24, side=4, obst_y=44, trip=(14, 8, 3), trip_until=100,
+                      ctrl=(50, 100))
+    cfg.tmax, cfg.startf, cfg.force_steps = 400, 200, 200
+    cfg.velav, cfg.start_velav, cfg.start_velfluc = True, 100, 250
+    path = tmp_path / "pre.inputs"
+    C.write_inputs(cfg, str(path))
+    print("precursor driver run", flush=True)
+    r = _driver(str(path), cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    dat = np.loadtxt(tmp_path / "pre.dat", delimiter=",", skiprows=2)
+    force = np.loadtxt(tmp_path / "pre.force", delimiter=",")
+    with L.Solver(C.read_inputs(str(path))) as s:
+        s.set_geometry()
+        s.init_equilibrium(L.INIT_MRT)
+        s.step(400)
+        rho, u, v, w = s.moments()
+        F = s.forces()
+        ctrl = s.inlet_control()
+        m = s.get_mask()
+    assert (m == L.NODE_RECYCLE).any() and not (m[:, 5:12, 10:18] == L.NODE_SOLID).any()  # trip gone
+    scale = np.abs(v).max()
+    assert np.abs(dat[:, 4] - v.reshape(-1)).max() < 1e-5 * scale
+    assert np.abs(dat[:, 6] - rho.reshape(-1)).max() < 1e-5
+    ref = np.float32(cfg.umax) ** 2 * cfg.nz * np.float32(cfg.obst1_r)
+    assert force.shape == (200, 4) and np.array_equal(force[:, 0], np.arange(200, 400))
+    assert np.abs(force[:, 1:] - F / ref).max() < 1e-5 * np.abs(F / ref).max()
+    # control events at t = 150, 200, ... (t % 50 == 0, t > 100), the body force moved
+    assert len(ctrl) == 5 and ctrl[0, 0] == 150 and ctrl[-1, 2] != cfg.dpdy
+    # the flow runs along y through the recycling plane into the channel
+    assert v[:, cfg.dyn_y1 + 2:-1, 1:-1].mean() > 0.5 * cfg.umax
+    prof = tmp_path / "pre.prof2"
+    p = subprocess.run([POST, "profiles", str(tmp_path / "pre_slice.dat"), "--nx", "42", "--ny", "96",
+                        "--obst-y", str(cfg.obst1[1]), "--height", "40", "--diam", "4", "--umax", str(cfg.umax),
+                        "--out", str(prof)], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 0, p.stderr
+    lines = prof.read_text().splitlines()
+    assert lines[0] == "x=1d" and "wake profile" in lines
+    vals = np.array([[float(a) for a in ln.split(",")] for ln in lines if "," in ln])
+    assert vals.shape[1] == 5 and np.all(np.isfinite(vals))
+
+
+@pytest.mark.gpu
+def test_refined_driver_normalises_the_fine_force_by_lrlevel_squared(tmp_path):
+    """With refinement the .force file holds the fine level's force over
+    UMAX^2 ZDIM OBSTR1 LRLEVEL^2 (WriteForces(..., REFINEMENT*LRLEVEL),
+    MRTtest/DynInlet3.cu:2788-2796, :3300)."""
+    cfg = C.channel(nx=64, ny=32, nz=32, side=6, x_center=22, tau=0.55, rates="lrmgpu", force_steps=20)
+    cfg.refinement, cfg.lr_size, cfg.lr_loc = True, (32, 24, 24), (12.25, 4.25, 4.25)
+    cfg.tmax, cfg.startf = 20, 0
+    path = tmp_path / "lr.inputs"
+    C.write_inputs(cfg, str(path))
+    r = _driver(str(path), cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    force = np.loadtxt(tmp_path / "lr.force", delimiter=",")
+    with L.Solver(C.read_inputs(str(path))) as s:
+        s.set_geometry()
+        s.init_equilibrium(L.INIT_MRT)
+        s.step(20)
+        F = s.forces()
+    ref = np.float32(cfg.umax) ** 2 * cfg.nz * np.float32(cfg.obst1_r) * 4
+    assert np.abs(F).max() > 0
+    assert np.abs(force[:, 1:] - F / ref).max() < 1e-5 * np.abs(F / ref).max()

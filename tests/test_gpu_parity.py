@@ -21,9 +21,14 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 MODEL_TO_ORC = {L.MODEL_BGK: O.BGK, L.MODEL_MRT_CPU3D: O.MRT_CPU3D, L.MODEL_MRT_LES: O.MRT_LES,
                 L.MODEL_MRT_DHUMIERES: O.MRT_DHUM}
+# the library's node codes and the oracle's are the same numbers (both
+# follow include/lbm.h); the table keeps the correspondence explicit
 NODE_TO_ORC = {L.NODE_FLUID: O.FLUID, L.NODE_SOLID: O.SOLID, L.NODE_SOLID_FORCE: O.SOLID_FORCE,
                L.NODE_INLET_WEST: O.INLET_WEST, L.NODE_OUTLET_EAST: O.OUTLET_EAST,
-               L.NODE_LID_WEST_REG: O.LID_WEST_REG, L.NODE_SYM_XMIN: O.SYM_XMIN, L.NODE_SYM_XMAX: O.SYM_XMAX}
+               L.NODE_LID_WEST_REG: O.LID_WEST_REG, L.NODE_SYM_XMIN: O.SYM_XMIN, L.NODE_SYM_XMAX: O.SYM_XMAX,
+               L.NODE_OUTLET_NORTH: O.OUTLET_NORTH, L.NODE_INLET_SOUTH: O.INLET_SOUTH, L.NODE_RECYCLE: O.RECYCLE,
+               L.NODE_VEL_FROM_BELOW: O.VEL_FROM_BELOW, L.NODE_GHOST_W: O.GHOST_W, L.NODE_GHOST_E: O.GHOST_E,
+               L.NODE_GHOST_S: O.GHOST_S, L.NODE_GHOST_N: O.GHOST_N, L.NODE_GHOST_DYN: O.GHOST_DYN}
 
 
 def say(*a):
@@ -32,9 +37,16 @@ def say(*a):
 
 
 def orc_params(c):
+    """Oracle parameters of a library configuration: the precursor wrap and
+    the ghost rows as the library derives them, and the south inlet table
+    (the library's generator over (x, z))."""
+    wrap = c.dyn_y1 if c.dyn_mode == 1 else -1
+    gy = tuple(min(max(v, 0), c.ny - 1) for v in (c.dyn_y1 - 1, c.dyn_y2, 1))
+    v_in = O.inlet_profile(c.nx, c.nz, c.umax, c.inlet_amp, c.seed)
     p = O.params(c.nx, c.ny, c.nz, bc=tuple(c.bc), collision=MODEL_TO_ORC[c.model], halfway=c.halfway,
                  omega=float(np.float32(c.omega)), cs=float(np.float32(c.cs)), dpdy=float(np.float32(c.dpdy)),
-                 u_lid=float(np.float32(c.umax)))
+                 u_lid=float(np.float32(c.umax)), dyn_y1=c.dyn_y1 if c.dyn_mode == 0 else -1, y_wrap=wrap,
+                 ghost_y=gy, v_in=v_in)
     for k in range(19):
         p.s_dhum[k] = float(np.float32(c.s_dhum[k]))
     return p
@@ -255,8 +267,9 @@ def test_channel_mrt_les_matches_oracle(rates):
     fl = mask == O.FLUID
     assert rel(rho[fl], r2[fl]) < 1e-5
     assert rel_vel((u[fl], v[fl], w[fl]), (u2[fl], v2[fl], w2[fl])) < 1e-5
-    assert F.shape == (T, 3)
-    assert np.abs(F - Fo).max() < 1e-4 * np.abs(Fo).max()
+    # steps t > STARTF = 0 are recorded into row t; row 0 stays zero
+    assert F.shape == (T, 3) and np.all(F[0] == 0)
+    assert np.abs(F[1:] - Fo[1:]).max() < 1e-4 * np.abs(Fo).max()
     assert np.abs(Fo).max() > 0
 
 
@@ -369,6 +382,11 @@ def _bc_case(case):
         mask[:, :, -1] = L.NODE_OUTLET_EAST
         mask[2:4, 3:6, 30:33] = L.NODE_SOLID_FORCE
         mask[0, :, 1:68] = L.NODE_SOLID
+    elif case == "inflow_family":
+        from test_gpu_inflow import inflow_box
+        c, mask = inflow_box()
+        c.umax = 0.05
+        return c, mask
     else:  # symmetry planes
         c.nx, c.ny, c.nz = 256, 6, 5
         c.bc[0], c.bc[1], c.bc[2] = L.BC_CLAMP, L.BC_WALL, L.BC_WALL
@@ -381,7 +399,8 @@ def _bc_case(case):
 
 
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
-@pytest.mark.parametrize("case", ["periodic_odd", "walls_halfway_solid", "inlet_outlet_node_bb", "symmetry"])
+@pytest.mark.parametrize("case", ["periodic_odd", "walls_halfway_solid", "inlet_outlet_node_bb", "symmetry",
+                                  "inflow_family"])
 def test_16bit_steps_match_oracle_on_the_16bit_grid(storage, case):
     """Three single steps of 16-bit storage, each from the GPU's own previous
     state (as stored: 16-bit value plus offset), against the FP32 oracle step
@@ -429,7 +448,7 @@ def test_bad_mask_code_rejected():
     c = L.default_config()
     c.nx, c.ny, c.nz = 8, 4, 4
     with L.Solver(c) as s:
-        bad = np.full((4, 4, 8), 9, np.uint8)
+        bad = np.full((4, 4, 8), L.NODE_GHOST_DYN + 1, np.uint8)
         with pytest.raises(L.LBMError) as e:
             s.set_geometry(bad)
         assert e.value.code == -22
