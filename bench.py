@@ -100,15 +100,133 @@ def cpu_baseline(cfg, budget_s=12.0):
         out["sample"] += f"; one thread: {s1} steps in {d1:.1f} s"
     except OSError:
         pass
+    cpu = host_cpu()
+    out["cpu_model"] = cpu.get("model name")
+    out["cpu"] = cpu
+    out["nproc"] = cpu["nproc"]
+    return out
+
+
+def host_cpu():
+    """The host CPU as /proc/cpuinfo names it, and the threads this process may use."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as fh:
-            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), None)
-        if model:
-            out["cpu_model"] = model
-            out["nproc"] = os.cpu_count()
+            for ln in fh:
+                k, _, v = ln.partition(":")
+                k, v = k.strip(), v.strip()
+                if k in ("vendor_id", "model name", "cpu family", "model", "stepping", "cpu MHz", "cache size") \
+                        and k not in info:
+                    info[k] = v
+                if k == "processor":
+                    info["processors"] = info.get("processors", 0) + 1
     except OSError:
         pass
-    return out
+    info["nproc"] = os.cpu_count()
+    info["affinity"] = len(os.sched_getaffinity(0))
+    info["omp_threads"] = int(os.environ.get("OMP_NUM_THREADS", info["affinity"]))
+    return info
+
+
+def cpu_plan(out_path=None):
+    """The CPU baseline plan: the oracle (C, OpenMP) on each workload, on all
+    the host threads this process may use and on one thread: the 64^3
+    Taylor-Green case in full (1000 steps), the 512x128x128 channel, the
+    refined channel (256x128x128 + 128x96x96 fine, two fine sub-steps per
+    coarse step) and the 1024x256x256 channel for 100 steps on all threads
+    (fewer on one thread), each with its full run time extrapolated from the
+    measured rate (5000, 2000 and 2000 steps).  Prints one JSON line per
+    measurement and the whole record at the end."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ctypes
+    import numpy as np
+    import oracle_lib as O
+    import lbm_multigpu_amd as L
+    from lbm_multigpu_amd import config as C
+    from test_gpu_parity import NODE_TO_ORC, orc_params
+
+    gomp = ctypes.CDLL("libgomp.so.1")
+    cpu = host_cpu()
+    nthr = cpu["omp_threads"]
+    lut = np.zeros(256, np.uint8)
+    for k, v in NODE_TO_ORC.items():
+        lut[k] = v
+    rec = {"cpu": cpu, "runs": []}
+    print(json.dumps({"cpu": cpu}), flush=True)
+
+    def state(cfg, tgv=False):
+        with L.Solver(cfg.to_c()) as s:
+            s.set_geometry(None)
+            if tgv:
+                s.init_taylor_green(0.05)
+            else:
+                s.init_equilibrium(L.INIT_BGK)
+            out = [lut[s.get_mask(0)], s.populations(0)]
+            if cfg.refinement:
+                out += [lut[s.get_mask(1)], s.populations(1)]
+        return out
+
+    def runner(p, f, m, u_in=None):
+        """The oracle's time loop on preallocated lattices (no copies timed)."""
+        fa = np.ascontiguousarray(f).copy()
+        fb = np.empty_like(fa)
+        fn = O.lib().orc_run_f32
+        uin = None if u_in is None else u_in.ctypes.data
+        return lambda k: fn(ctypes.byref(p), m.ctypes.data, uin, fa, fb, k, None)
+
+    def timed(fn, steps, threads):
+        gomp.omp_set_num_threads(threads)
+        t0 = time.perf_counter()
+        fn(steps)
+        dt = time.perf_counter() - t0
+        gomp.omp_set_num_threads(nthr)
+        return dt
+
+    def report(name, updates, steps, threads, dt, full_steps):
+        r = {"workload": name, "threads": threads, "steps": steps, "seconds": round(dt, 3),
+             "mlups": round(updates * steps / dt / 1e6, 2)}
+        if full_steps != steps:
+            r["full_steps"] = full_steps
+            r["extrapolated_full_run_s"] = round(dt / steps * full_steps, 1)
+        rec["runs"].append(r)
+        print(json.dumps(r), flush=True)
+
+    # config 1: Taylor-Green 64^3, BGK tau 0.6, 1000 steps, in full
+    cfg = C.taylor_green()
+    m, f = state(cfg, tgv=True)
+    p = orc_params(cfg.to_c())
+    n1 = cfg.nx * cfg.ny * cfg.nz
+    for th in (nthr, 1):
+        report("taylor_green 64^3 BGK", n1, 1000, th, timed(runner(p, f, m), 1000, th), 1000)
+    # configs 2 and 4: the channel at 512x128x128 and 1024x256x256
+    for name, cfg, full, one in (("channel 512x128x128", C.channel(), 5000, 10),
+                                 ("channel 1024x256x256", C.channel(nx=1024, ny=256, nz=256, side=32, x_center=256), 2000, 2)):
+        m, f = state(cfg)
+        c = cfg.to_c()
+        p = orc_params(c)
+        u_in = O.inlet_profile(c.ny, c.nz, c.umax, c.inlet_amp, c.seed)
+        n = cfg.nx * cfg.ny * cfg.nz
+        for th, k in ((nthr, 100), (1, one)):
+            report(name, n, k, th, timed(runner(p, f, m, u_in), k, th), full)
+        del f
+    # config 3: the refined channel, coarse plus both fine sub-steps per step
+    cfg = C.refined_channel()
+    cm, cf, fm, ff = state(cfg)
+    c = cfg.to_c()
+    pc = orc_params(c)
+    r = O.refine([int(round(x - 0.25)) for x in cfg.lr_loc], cfg.lr_size, float(np.float32(c.omega)),
+                 cs=float(np.float32(c.cs)), temporal=1, eq_kind=O.EQ_BGK_MEQ)
+    pf = O.params(*cfg.lr_size, bc=(O.CLAMP,) * 3, collision=pc.collision, halfway=1, omega=r.omega_f,
+                  cs=pc.cs, dpdy=0.5 * pc.dpdy)
+    u_in = O.inlet_profile(c.ny, c.nz, c.umax, c.inlet_amp, c.seed)
+    n3 = cfg.nx * cfg.ny * cfg.nz + 2 * int(np.prod(cfg.lr_size))
+    for th, k in ((nthr, 100), (1, 3)):
+        report("refined channel 256x128x128 + 128x96x96", n3, k, th,
+               timed(lambda s: O.refine_run(pc, pf, r, cf, ff, s, cmask=cm, fmask=fm, u_in=u_in), k, th), 2000)
+    print(json.dumps(rec), flush=True)
+    if out_path:
+        with open(out_path, "w") as fh:
+            json.dump(rec, fh, indent=1)
 
 
 def copy_calibration(nbytes=637_534_208, reps=20):
@@ -159,9 +277,14 @@ def main():
                     help="record the momentum-exchange force every step (timed with the steps)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain device copy of one lattice instead (HBM ceiling / counter scale)")
+    ap.add_argument("--cpu-plan", metavar="OUT", nargs="?", const="",
+                    help="run only the CPU baseline plan over every workload (several minutes)")
     a = ap.parse_args()
     if a.calibrate_copy:
         copy_calibration()
+        return
+    if a.cpu_plan is not None:
+        cpu_plan(a.cpu_plan or None)
         return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the sharded path is a design, not a build.",

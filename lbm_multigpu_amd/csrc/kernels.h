@@ -325,6 +325,9 @@ __device__ __forceinline__ int node_fix(const StepParams& p, int x, int y, int z
         const int Y[3] = {ym, y, yp};
         const int Z[3] = {zm, z, zp};
         const int X[3] = {xm, x, xp};
+        // links bounced from a force-summed solid, as bits: the momentum
+        // exchange is summed after the loop, off the loads' dependency chain
+        unsigned flinks = 0;
 #pragma unroll
         for (int i = 1; i < Q; ++i) {
             const bool out = (CX[i] > 0 && oxm) || (CX[i] < 0 && oxp) || (CY[i] > 0 && oym) ||
@@ -333,15 +336,17 @@ __device__ __forceinline__ int node_fix(const StepParams& p, int x, int y, int z
             if (!out && solid_nb)
                 sc = *at(p.mask, p.nxp * (Y[1 - CY[i]] + p.ny * Z[1 - CZ[i]]) + X[1 - CX[i]]) & NODE_CODE_MASK;
             const bool bounce = out || is_solid(sc);
-            if (bounce) {
-                const R v = Store<S>::load(at(fin + OPP[i] * qs, cell), OPP[i], p.sh);
-                f[i] = v;
-                if (p.fpart && sc == NODE_SOLID_FORCE) {
-                    fx += R(2 * CX[OPP[i]]) * v;
-                    fy += R(2 * CY[OPP[i]]) * v;
-                    fz += R(2 * CZ[OPP[i]]) * v;
+            if (bounce) f[i] = Store<S>::load(at(fin + OPP[i] * qs, cell), OPP[i], p.sh);
+            if (sc == NODE_SOLID_FORCE) flinks |= 1u << i;
+        }
+        if (p.fpart && flinks) {
+#pragma unroll
+            for (int i = 1; i < Q; ++i)
+                if (flinks & (1u << i)) {
+                    fx += R(2 * CX[OPP[i]]) * f[i];
+                    fy += R(2 * CY[OPP[i]]) * f[i];
+                    fz += R(2 * CZ[OPP[i]]) * f[i];
                 }
-            }
         }
     }
 
@@ -529,6 +534,47 @@ __device__ __forceinline__ void block_force_partial(const StepParams& p, int lb,
     }
 }
 
+// Momentum-exchange partial of one wavefront of a force-holding workgroup:
+// its FP64 sum (fixed shuffle tree) into wave slot slot*NW + wave of this
+// step's partials.  No barrier: the other wavefronts of the workgroup need
+// not wait for the slow near-solid one.
+template <int NW, class R>
+__device__ __forceinline__ void wave_force_partial(const StepParams& p, int slot, R fx, R fy, R fz)
+{
+    const double sx = wave_sum(double(fx)), sy = wave_sum(double(fy)), sz = wave_sum(double(fz));
+    if (lane_id() == 0) {
+        double* o = p.fpart + ((long long)slot * NW + wave_id()) * 3;
+        o[0] = sx;
+        o[1] = sy;
+        o[2] = sz;
+    }
+}
+
+// Force of each step of a batch: workgroup s sums step s's n wave partials
+// (3 components) in a fixed order (strided per thread, then a fixed tree)
+// into out[3 s + c].
+__global__ void __launch_bounds__(256) k_reduce_steps(const double* part, long long n, double* out)
+{
+    __shared__ double sh[3][256];
+    const double* q = part + (long long)blockIdx.x * n * 3;
+    double a = 0, b = 0, c = 0;
+    for (long long i = threadIdx.x; i < n; i += 256) {
+        a += q[i * 3];
+        b += q[i * 3 + 1];
+        c += q[i * 3 + 2];
+    }
+    sh[0][threadIdx.x] = a;
+    sh[1][threadIdx.x] = b;
+    sh[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) out[(long long)blockIdx.x * 3 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
 // The stream-collide step; NT selects nontemporal population stores (the
 // level's p.nt_store, as a template argument so each variant is straight-line).
 template <int MODEL, class S, bool NT>
@@ -537,15 +583,14 @@ __global__ void __launch_bounds__(Shape<S>::THREADS, Shape<S>::MIN_WAVES) k_stre
     using R = typename Arith<S>::R;
     constexpr int NW = Shape<S>::NW;
     const int lb = logical_block(p.xcd_chunk);
+    // only workgroups holding a force node write a partial; the slot is read
+    // (a scalar load) before the pull, so its latency hides under the loads
+    const int slot = p.fpart ? (p.fslot ? p.fslot[lb] : lb) : -1;
     R fx = 0, fy = 0, fz = 0;
     int x, y, z;
     const bool row_ok = wave_node<NW>(lb, p.nsegw, p.ny, p.nz, x, y, z);
     if (row_ok && x < p.nx) node_update<MODEL, S, NT ? kStoreNT : 0>(p, x, y, z, fx, fy, fz);
-    if (p.fpart) {
-        // only workgroups holding a force node write a partial (wave-uniform)
-        const int slot = p.fslot ? p.fslot[lb] : lb;
-        if (slot >= 0) block_force_partial<NW>(p, slot, fx, fy, fz);
-    }
+    if (slot >= 0) wave_force_partial<NW>(p, slot, fx, fy, fz);
 }
 
 // Deterministic FP64 reduction of ncomp-component partials (fixed order:

@@ -58,8 +58,13 @@ struct lbm_ctx {
     float* v_in = nullptr;     // south inlet profile [nz*nx]
     bool user_mask = false;    // the coarse mask came from the caller (no generator, no trip switch)
     bool trip_on = false;      // the classified mask holds the trip obstacle
-    double* fpart = nullptr;   // force partials per block
-    double* fstage = nullptr;  // first-stage sums of the force partials (k_reduce_chunks)
+    double* fpart = nullptr;   // force partials: a batch of steps' wave partials, or a closure step's block partials
+    size_t fpart_cap = 0;      // doubles allocated in fpart
+    double* fstage = nullptr;  // first-stage sums of the closure's block partials (k_reduce_chunks)
+    // recorded steps whose wave partials wait in fpart for one batched
+    // reduction: fb_n steps from history row fb_first, fb_w wave slots each
+    int fb_n = 0, fb_w = 0;
+    long long fb_first = 0;
     double* fhist = nullptr;   // [force_steps*3]
     double* red = nullptr;     // small reduction output
     double* mpart = nullptr;   // mass partials
@@ -116,6 +121,7 @@ static int refine_cf(lbm_ctx* ctx, const float* P);
 static int refine_fc(lbm_ctx* ctx);
 static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f, bool record);
 static int set_geometry_at(lbm_ctx* ctx, const uint8_t* mask, long long t);
+static int flush_forces(lbm_ctx* ctx);
 
 static int fail(lbm_ctx* ctx, int code, const std::string& msg)
 {
@@ -407,13 +413,11 @@ extern "C" int lbm_create(const lbm_config* cfg, lbm_ctx** out)
             return bail(rc);
         }
     }
-    // force partials: at most one per workgroup of either level's stream-collide
-    // (or of the closure collide, which uses four-wavefront workgroups)
-    const int maxblocks = std::max(L.blocks, ctx->nlevels > 1 ? ctx->lv[1].blocks : 0);
+    // force partials are sized when the geometry (and so the force slots) is
+    // known; the closure's first-stage sums cover one per workgroup of level 0
     if (hipMalloc(&ctx->u_in, prof.size() * sizeof(float)) != hipSuccess ||
         hipMalloc(&ctx->v_in, vprof.size() * sizeof(float)) != hipSuccess ||
-        hipMalloc(&ctx->fpart, (size_t)maxblocks * 3 * sizeof(double)) != hipSuccess ||
-        hipMalloc(&ctx->fstage, (size_t)(maxblocks / REDUCE_CHUNK + 1) * 3 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ctx->fstage, (size_t)(L.blocks / REDUCE_CHUNK + 1) * 3 * sizeof(double)) != hipSuccess ||
         hipMalloc(&ctx->mpart, (size_t)std::max(L.blocks, MM_GRID_MAX) * 4 * sizeof(double)) != hipSuccess ||
         hipMalloc(&ctx->red, 8 * sizeof(double)) != hipSuccess)
         return bail(fail(nullptr, LBM_ENOMEM, "lbm_create: device allocation failed"));
@@ -610,6 +614,7 @@ static int set_geometry_at(lbm_ctx* ctx, const uint8_t* mask, long long t)
 {
     Level& L = ctx->lv[0];
     const lbm_config& c = ctx->cfg;
+    if (int rc = flush_forces(ctx)) return rc; // the pending partials follow the old force slots
     if (mask) {
         for (size_t i = 0; i < L.cells; ++i)
             if (mask[i] > LBM_NODE_MAX)
@@ -717,6 +722,7 @@ extern "C" int lbm_init_equilibrium(lbm_ctx* ctx, int kind, const float* rho, co
     }
     ctx->t = 0;
     ctx->force_recorded = 0;
+    ctx->fb_n = 0;
     ctx->dpdy = ctx->cfg.dpdy;
     ctx->ctrl.clear();
     ctx->initialised = true;
@@ -734,6 +740,7 @@ extern "C" int lbm_init_taylor_green(lbm_ctx* ctx, float u0)
     if (rc) return rc;
     ctx->t = 0;
     ctx->force_recorded = 0;
+    ctx->fb_n = 0;
     ctx->dpdy = ctx->cfg.dpdy;
     ctx->ctrl.clear();
     ctx->initialised = true;
@@ -837,10 +844,69 @@ static long long launch_closure(lbm_ctx* ctx, const StepParams& p, const Level& 
     return blocks;
 }
 
-// Deterministic FP64 sum of nparts force partials (3 components) into out:
-// chunks of REDUCE_CHUNK partials per workgroup, then one workgroup over
-// the chunk sums.  Replaces the FX/FY/FZ atomicAdd of the reference
-// (MRTtest/DynInlet3.cu:1819-1838).
+// Force partials, replacing the FX/FY/FZ atomicAdd of the reference
+// (MRTtest/DynInlet3.cu:1819-1838) with deterministic FP64 sums.  A recorded
+// stream-collide step writes one partial per wavefront of each workgroup
+// that holds a force node; FORCE_BATCH consecutive steps collect in fpart
+// and one launch (k_reduce_steps, a workgroup per step, fixed order) sums
+// them into the history.  The batch is flushed when full and before
+// anything reads the history or changes the slot layout.
+constexpr int FORCE_BATCH = 32;
+constexpr size_t FORCE_BATCH_BYTES = 256u << 20; // the batch shrinks for larger slot counts
+
+// fpart holds at least n doubles (synchronises before replacing it).
+static int ensure_fpart(lbm_ctx* ctx, size_t n)
+{
+    if (ctx->fpart_cap >= n) return LBM_OK;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->fpart) (void)hipFree(ctx->fpart);
+    ctx->fpart = nullptr;
+    ctx->fpart_cap = 0;
+    HIP_TRY(ctx, hipMalloc(&ctx->fpart, n * sizeof(double)));
+    ctx->fpart_cap = n;
+    return LBM_OK;
+}
+
+// Steps of one batch for w wave slots per step.
+static int force_batch_steps(int w)
+{
+    const size_t per = (size_t)std::max(w, 1) * 3 * sizeof(double);
+    return (int)std::max<size_t>(1, std::min<size_t>(FORCE_BATCH, FORCE_BATCH_BYTES / per));
+}
+
+// Reduce the pending batch into the history.
+static int flush_forces(lbm_ctx* ctx)
+{
+    if (ctx->fb_n == 0) return LBM_OK;
+    hipLaunchKernelGGL(k_reduce_steps, dim3(ctx->fb_n), dim3(256), 0, ctx->stream, (const double*)ctx->fpart,
+                       (long long)ctx->fb_w, ctx->fhist + ctx->fb_first * 3);
+    ctx->fb_n = 0;
+    HIP_TRY(ctx, hipGetLastError());
+    return LBM_OK;
+}
+
+// Partials buffer of the recorded step t of a level with w wave slots per
+// step: the next position of the batch (flushing first when the batch is
+// full, or does not continue it).
+static int batch_slot(lbm_ctx* ctx, long long t, int w, double** out)
+{
+    const long long row = t - ctx->cfg.startf;
+    if (ctx->fb_n > 0 && (w != ctx->fb_w || row != ctx->fb_first + ctx->fb_n || ctx->fb_n >= force_batch_steps(w)))
+        if (int rc = flush_forces(ctx)) return rc;
+    if (int rc = ensure_fpart(ctx, (size_t)force_batch_steps(w) * std::max(w, 1) * 3)) return rc;
+    if (ctx->fb_n == 0) {
+        ctx->fb_first = row;
+        ctx->fb_w = w;
+    }
+    *out = ctx->fpart + (size_t)ctx->fb_n * w * 3;
+    ctx->fb_n += 1;
+    ctx->force_recorded = (int)std::max<long long>(ctx->force_recorded, row + 1);
+    return LBM_OK;
+}
+
+// Deterministic FP64 sum of nparts block partials (3 components) into out,
+// for the closure collide's one partial per workgroup: chunks of
+// REDUCE_CHUNK partials per workgroup, then one workgroup over the chunk sums.
 static void reduce_force_partials(lbm_ctx* ctx, long long nparts, double* out)
 {
     if (nparts <= REDUCE_CHUNK) { // (no partials: the sum writes zeros)
@@ -872,18 +938,25 @@ static int coarse_step(lbm_ctx* ctx)
     // with refinement the fine level records the force, as main() turns the
     // coarse flag_F off under REFINEMENT (MRTtest/DynInlet3.cu:3130)
     const bool rec = ctx->nlevels == 1 && records_force(ctx, ctx->t);
-    if (rec) {
-        p.fpart = ctx->fpart;
+    const bool closure = ctx->mlp && (ctx->closure_every <= 1 || (ctx->t - ctx->closure_t0) % ctx->closure_every == 0);
+    if (rec && !closure) {
+        // wave partials of the force workgroups, batched
+        if (int rc = batch_slot(ctx, ctx->t, L.nfslot * nwave_of_storage(L.storage), &p.fpart)) return rc;
         p.fslot = L.fslot;
+    } else if (rec) {
+        // the closure collide writes one partial per workgroup, reduced at once
+        if (int rc = flush_forces(ctx)) return rc;
+        if (int rc = ensure_fpart(ctx, (size_t)L.blocks * 3)) return rc;
+        p.fpart = ctx->fpart;
     }
-    long long nparts = L.nfslot; // force partials written: one per workgroup holding a force node
+    long long nparts = 0;
     with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         if (!ctx->mlp) {
             launch_sc<S>(c.model, p, L.blocks, ctx->stream);
         } else if (ctx->closure_every <= 1) {
             nparts = launch_closure<S>(ctx, p, L, nullptr);
-        } else if ((ctx->t - ctx->closure_t0) % ctx->closure_every == 0) {
+        } else if (closure) {
             nparts = launch_closure<S>(ctx, p, L, ctx->omega_field); // refresh: network + rates for the next steps
         } else {
             StepParams q = p;
@@ -892,10 +965,10 @@ static int coarse_step(lbm_ctx* ctx)
         }
         return 0;
     });
-    if (rec) {
-        const long long slot = ctx->t - c.startf;
-        reduce_force_partials(ctx, nparts, ctx->fhist + slot * 3);
-        ctx->force_recorded = (int)std::max<long long>(ctx->force_recorded, slot + 1);
+    if (rec && closure) {
+        const long long row = ctx->t - c.startf;
+        reduce_force_partials(ctx, nparts, ctx->fhist + row * 3);
+        ctx->force_recorded = (int)std::max<long long>(ctx->force_recorded, row + 1);
     }
     std::swap(L.fa, L.fb);
     return LBM_OK;
@@ -911,7 +984,7 @@ static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f, bool record)
     StepParams p = step_params(ctx, F, omega_f);
     p.bcx = p.bcy = p.bcz = LBM_BC_CLAMP;
     if (record) {
-        p.fpart = ctx->fpart;
+        if (int rc = batch_slot(ctx, ctx->t, F.nfslot * nwave_of_storage(F.storage), &p.fpart)) return rc;
         p.fslot = F.fslot;
     }
     with_storage(F.storage, [&](auto* tag) {
@@ -919,11 +992,6 @@ static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f, bool record)
         launch_sc<S>(c.model, p, F.blocks, ctx->stream);
         return 0;
     });
-    if (record) {
-        const long long slot = ctx->t - c.startf;
-        reduce_force_partials(ctx, F.nfslot, ctx->fhist + slot * 3);
-        ctx->force_recorded = (int)std::max<long long>(ctx->force_recorded, slot + 1);
-    }
     std::swap(F.fa, F.fb);
     return LBM_OK;
 }
@@ -1254,6 +1322,7 @@ extern "C" int lbm_set_populations_f64(lbm_ctx* ctx, int level, const double* f)
 extern "C" int lbm_get_forces(lbm_ctx* ctx, double* fxyz, int max_steps, int* nsteps)
 {
     if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_get_forces: null context");
+    if (int rc = flush_forces(ctx)) return rc;
     const int n = std::min(ctx->force_recorded, std::max(max_steps, 0));
     if (nsteps) *nsteps = ctx->force_recorded;
     if (n > 0 && fxyz) {

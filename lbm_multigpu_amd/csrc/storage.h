@@ -98,25 +98,6 @@ __device__ __forceinline__ void buffer_put(u32x2 w, __amdgpu_buffer_rsrc_t r, ui
     __builtin_amdgcn_raw_buffer_store_b64(w, r, (int)off, 0, AUX);
 }
 
-// Buffer load of element byte offset off of a direction plane (descriptor in
-// scalar registers, one 32-bit offset register shared by every plane), as
-// the storage's arithmetic type: the cold boundary paths that read another
-// node's unstreamed populations use it, so they hold one offset instead of
-// a 64-bit address per plane.
-template <class S>
-__device__ __forceinline__ typename Arith<S>::R buffer_pop(__amdgpu_buffer_rsrc_t r, uint32_t off, int I,
-                                                           const PopShift& sh);
-template <>
-__device__ __forceinline__ float buffer_pop<float>(__amdgpu_buffer_rsrc_t r, uint32_t off, int, const PopShift&)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
-}
-template <>
-__device__ __forceinline__ double buffer_pop<double>(__amdgpu_buffer_rsrc_t r, uint32_t off, int, const PopShift&)
-{
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
-}
-
 template <>
 struct Store<float> {
     __device__ static __forceinline__ float load(const float* p, int, const PopShift&) { return ld(p); }
@@ -225,19 +206,5 @@ struct Store<BHalf16> {
         return BHalf16{float_to_bf16_bits(v - sh[i])};
     }
 };
-
-template <>
-__device__ __forceinline__ float buffer_pop<Half16>(__amdgpu_buffer_rsrc_t r, uint32_t off, int I, const PopShift& sh)
-{
-    const unsigned h = __builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0);
-    float v;
-    asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(v) : "v"(h), "s"(sh[I]));
-    return v;
-}
-template <>
-__device__ __forceinline__ float buffer_pop<BHalf16>(__amdgpu_buffer_rsrc_t r, uint32_t off, int I, const PopShift& sh)
-{
-    return bf16_bits_to_float(__builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0)) + sh[I];
-}
 
 } // namespace lbm

@@ -79,3 +79,35 @@ def test_full_size_fp32_mass_tracks_fp64():
     *_, m64 = _run("f64", 1e-3)
     say(f"  mass FP32 {m32:.9e}, FP64 {m64:.9e}")
     assert abs(m32 - m64) < 1e-6 * abs(m64)
+
+
+def test_config4_1024_storage_formats_against_fp64():
+    """The channel scaled to 1024x256x256 (cylinder side 32), 2000 steps from
+    the same seeded state in FP64, FP32, FP16 and BF16 storage: velocity
+    error against the FP64 run, L2 over the fluid and Linf relative to
+    max|u|.  Bars: FP16 <= 1e-3 in both (the storage target), FP32 <= 1e-5,
+    BF16 (8 significant bits, reported beside) L2 <= 2e-3 and Linf <= 3e-2;
+    every format's FP64 mass sum within 1e-6 (FP32, FP16) or 1e-4 (BF16) of
+    the FP64 run's."""
+    from lbm_multigpu_amd import precision as P
+
+    ref, solid = None, None
+    bars = {"f32": (1e-5, 1e-5, 1e-6), "f16": (1e-3, 1e-3, 1e-6), "bf16": (2e-3, 3e-2, 1e-4)}
+    for st in ("f64", "f32", "f16", "bf16"):
+        say("config 4 storage", st)
+        cfg = P.workload(1.0, st)
+        assert (cfg.nx, cfg.ny, cfg.nz) == (1024, 256, 256)
+        r = P.run_one(cfg, 2000)
+        if st == "f64":
+            ref = r
+            with L.Solver(cfg) as s:
+                s.set_geometry()
+                solid = np.isin(s.get_mask(), (L.NODE_SOLID, L.NODE_SOLID_FORCE))
+            continue
+        e = P.errors(r, ref, solid)
+        dm = abs((r["mass1"] - ref["mass1"]) / ref["mass1"])
+        say(f"  {st}: u L2 {e['u_l2']:.2e}, Linf {e['u_linf']:.2e}, mass vs FP64 {dm:.1e}, "
+            f"{cfg.nx * cfg.ny * cfg.nz * 2000 / (r['ms'] * 1e-3) / 1e9:.1f} GLUPS")
+        l2, linf, mass = bars[st]
+        assert e["u_l2"] <= l2 and e["u_linf"] <= linf and dm <= mass
+        del r
