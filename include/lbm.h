@@ -339,7 +339,9 @@ int lbm_set_closure_interval(lbm_ctx* ctx, int every);
 /* training samples from a context's current state, box-filtered by 2 (the
  * solver's own finer run): features X[n][7] and the SGS eddy viscosity
  * y[n] = max(0, -tau_dev:S/(2 S:S)) of a hashed fraction `keep` of the
- * filtered cells, in cell order */
+ * filtered cells, in cell order.  When more than max_samples cells qualify
+ * the call returns LBM_EINVAL with *count set to the number needed (a
+ * truncated selection would depend on the order of the GPU's atomics). */
 int lbm_closure_samples(lbm_ctx* ctx, uint64_t seed, float keep, float* X, float* y, int max_samples,
                         int* count);
 

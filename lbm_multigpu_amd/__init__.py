@@ -331,7 +331,10 @@ class Solver:
         X = np.empty((max_samples, MLP_IN), np.float32)
         y = np.empty(max_samples, np.float32)
         n = C.c_int()
-        self._check(self.lib.lbm_closure_samples(self.h, seed, keep, X.reshape(-1), y, max_samples, C.byref(n)))
+        rc = self.lib.lbm_closure_samples(self.h, seed, keep, X.reshape(-1), y, max_samples, C.byref(n))
+        if rc == -22 and n.value > max_samples:  # more qualify: fetch them all with a buffer of that size
+            return self.closure_samples(seed, keep, n.value)
+        self._check(rc)
         return X[: n.value].copy(), y[: n.value].copy()
 
 

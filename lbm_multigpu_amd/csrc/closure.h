@@ -298,7 +298,11 @@ struct ClosureArgs {
     const float* dwall;    // per node (padded layout)
     MlpNorm nrm;
     float nu_cap;       // upper bound on nu_t (stability)
-    float* omega_out;   // if set: each collided node's rate (0 = Smagorinsky) for the steps until the next refresh
+    // per-node rate field: read first, as the rate the node was last relaxed
+    // at (|value|; 0: none yet, the base rate), then overwritten with this
+    // step's rate -- the network's (> 0, held by the steps until the next
+    // refresh) or the Smagorinsky rate negated (< 0: those steps recompute it)
+    float* omega_out;
 };
 
 // Stream-collide with the learned closure: pull and boundary conditions per
@@ -338,7 +342,10 @@ __global__ void __launch_bounds__(64 * CLOSURE_NW, 2) k_stream_collide_closure(S
         const Macro<R> m = macro_from_pairs(f[0], sp, dp);
         const float rho = float(m.rho);
         const float third = rho * (1.f / 3.f);
-        const float k = -1.5f * p.omega / rho;
+        // S = -3 omega/(2 rho) Pi_neq with the rate that shaped Pi_neq: the
+        // node's last applied one (tau0 + 3 nu_t), the base rate before any
+        const float wlast = ca.omega_out ? fabsf(*at(ca.omega_out, cell)) : 0.f;
+        const float k = -1.5f * (wlast > 0.f ? wlast : p.omega) / rho;
         const float sxx = k * (float(m.pxx) - third - float(m.jx * m.jx));
         const float syy = k * (float(m.pyy) - third - float(m.jy * m.jy));
         const float szz = k * (float(m.pzz) - third - float(m.jz * m.jz));
@@ -367,7 +374,7 @@ __global__ void __launch_bounds__(64 * CLOSURE_NW, 2) k_stream_collide_closure(S
                 if (ca.omega_out) *at(ca.omega_out, cell) = wr;
             } else {
                 cp.omega = omega_les;
-                if (ca.omega_out) *at(ca.omega_out, cell) = 0.f;
+                if (ca.omega_out) *at(ca.omega_out, cell) = -float(omega_les);
             }
             cp.cs = R(0);
             collide<MODEL>(f, cp);

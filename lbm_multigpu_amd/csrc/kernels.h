@@ -103,7 +103,7 @@ struct StepParams {
     int stat_mode;
     float stat_n, stat_w; // samples so far (t - start) and this sample's weight
     // per-node collision rate left by the last learned-closure step (padded
-    // layout): > 0 replaces the LES rate, 0 means the Smagorinsky rate
+    // layout): > 0 replaces the LES rate, <= 0 means the Smagorinsky rate
     const float* omega_node;
     PopShift sh;   // 16-bit storage offsets
     int dyn_y1;    // rows y > dyn_y1 get no body force (INT_MAX: all rows get it)

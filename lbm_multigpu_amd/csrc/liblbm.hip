@@ -702,6 +702,18 @@ static int init_level(lbm_ctx* ctx, int level, int kind, const float* rho, const
     return rc;
 }
 
+// A new initial state restarts the closure's refresh cycle at t = 0 and
+// clears the per-node rates the previous state's network left.
+static int reset_closure_state(lbm_ctx* ctx)
+{
+    ctx->closure_t0 = 0;
+    if (ctx->omega_field) {
+        HIP_TRY(ctx, hipMemsetAsync(ctx->omega_field, 0, (size_t)ctx->lv[0].qs * sizeof(float), ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return LBM_OK;
+}
+
 extern "C" int lbm_init_equilibrium(lbm_ctx* ctx, int kind, const float* rho, const float* u, const float* v,
                                     const float* w)
 {
@@ -714,6 +726,7 @@ extern "C" int lbm_init_equilibrium(lbm_ctx* ctx, int kind, const float* rho, co
     }
     int rc = init_level(ctx, 0, kind, rho, u, v, w, ctx->cfg.umax);
     if (rc) return rc;
+    if ((rc = reset_closure_state(ctx))) return rc;
     if (ctx->nlevels > 1) {
         rc = init_level(ctx, 1, kind, nullptr, nullptr, nullptr, nullptr, ctx->cfg.umax);
         if (rc) return rc;
@@ -738,6 +751,7 @@ extern "C" int lbm_init_taylor_green(lbm_ctx* ctx, float u0)
     }
     int rc = init_level(ctx, 0, 2, nullptr, nullptr, nullptr, nullptr, u0);
     if (rc) return rc;
+    if ((rc = reset_closure_state(ctx))) return rc;
     ctx->t = 0;
     ctx->force_recorded = 0;
     ctx->fb_n = 0;
@@ -955,7 +969,7 @@ static int coarse_step(lbm_ctx* ctx)
         if (!ctx->mlp) {
             launch_sc<S>(c.model, p, L.blocks, ctx->stream);
         } else if (ctx->closure_every <= 1) {
-            nparts = launch_closure<S>(ctx, p, L, nullptr);
+            nparts = launch_closure<S>(ctx, p, L, ctx->omega_field);
         } else if (closure) {
             nparts = launch_closure<S>(ctx, p, L, ctx->omega_field); // refresh: network + rates for the next steps
         } else {
@@ -1824,6 +1838,10 @@ extern "C" int lbm_set_closure(lbm_ctx* ctx, lbm_mlp* m, float nu_cap)
                 pad[(size_t)x + (size_t)L.nxp * (y + (size_t)L.ny * z)] = d[(size_t)x + (size_t)L.nx * (y + (size_t)L.ny * z)];
     if (!ctx->dwall) HIP_TRY(ctx, hipMalloc(&ctx->dwall, pad.size() * 4));
     HIP_TRY(ctx, hipMemcpy(ctx->dwall, pad.data(), pad.size() * 4, hipMemcpyHostToDevice));
+    // the per-node rate field: the rate each node was last relaxed at (the
+    // closure's strain estimate) and the held rates between refreshes
+    if (!ctx->omega_field) HIP_TRY(ctx, hipMalloc(&ctx->omega_field, (size_t)L.qs * sizeof(float)));
+    HIP_TRY(ctx, hipMemset(ctx->omega_field, 0, (size_t)L.qs * sizeof(float)));
     ctx->mlp = m;
     ctx->nu_cap = nu_cap > 0.f ? nu_cap : 0.05f;
     ctx->closure_t0 = ctx->t;
@@ -1837,7 +1855,7 @@ extern "C" int lbm_set_closure_interval(lbm_ctx* ctx, int every)
 {
     if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_set_closure_interval: null context");
     if (every < 1) return fail(ctx, LBM_EINVAL, "lbm_set_closure_interval: every must be >= 1");
-    if (every > 1 && !ctx->omega_field) {
+    if (!ctx->omega_field) {
         Level& L = ctx->lv[0];
         HIP_TRY(ctx, hipMalloc(&ctx->omega_field, (size_t)L.qs * sizeof(float)));
         HIP_TRY(ctx, hipMemset(ctx->omega_field, 0, (size_t)L.qs * sizeof(float)));
@@ -1903,6 +1921,14 @@ extern "C" int lbm_closure_samples(lbm_ctx* ctx, uint64_t seed, float keep, floa
     unsigned long long n = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&n, dcount, 8, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && n > (unsigned long long)max_samples) {
+        // the kernel kept the first max_samples candidates in atomic order,
+        // which is not deterministic: refuse, and report how many are needed
+        cleanup();
+        *count = (int)std::min<unsigned long long>(n, INT_MAX);
+        return fail(ctx, LBM_EINVAL, "lbm_closure_samples: " + std::to_string(n) + " samples qualify, more than max_samples " +
+                                         std::to_string(max_samples) + " (*count holds the number needed)");
+    }
     const int got = (int)std::min<unsigned long long>(n, (unsigned long long)max_samples);
     std::vector<float> hx((size_t)got * MLP_IN), hy(got);
     std::vector<unsigned int> hi(got);

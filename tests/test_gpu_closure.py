@@ -184,15 +184,17 @@ def _pre_collision(c, f):
     return fs
 
 
-def _node_features(c, fs):
-    """Closure inputs from the pre-collision populations: S = -3 omega0 /
-    (2 rho) (P - rho/3 I - j j), wall distance, |j|."""
+def _node_features(c, fs, omega_last=None):
+    """Closure inputs from the pre-collision populations: S = -3 omega /
+    (2 rho) (P - rho/3 I - j j) with omega the rate each node was last
+    relaxed at (the base rate omega0 before any), wall distance, |j|."""
     Cv = CV
     f = fs.astype(np.float64)
     rho = f.sum(0)
     j = np.einsum("ia,i...->a...", Cv, f)
     P = np.einsum("ia,ib,i...->ab...", Cv, Cv, f)
-    k = -1.5 * float(np.float32(c.omega)) / rho
+    om = float(np.float32(c.omega)) if omega_last is None else omega_last.reshape(rho.shape).astype(np.float64)
+    k = -1.5 * om / rho
     S = k * (P - rho / 3.0 * np.eye(3)[:, :, None, None, None] - j[:, None] * j[None, :])
     S = np.moveaxis(S, (0, 1), (-2, -1))
     nz, ny, nx = rho.shape
@@ -230,10 +232,10 @@ def _closure_setup(seed):
     return c, f, m, net
 
 
-def _reference_rates(c, fs, net):
+def _reference_rates(c, fs, net, omega_last=None):
     """(in-range flags, per-node rate) the NumPy network gives for the
     pre-collision populations fs."""
-    Xf = _node_features(c, fs).reshape(-1, 7)
+    Xf = _node_features(c, fs, omega_last).reshape(-1, 7)
     zb, _ = M.forward_bf16(net["p"], M.normalise(Xf, net["mu"], net["sd"]))
     nut = np.minimum(net["nu_scale"] * M.softplus(zb.astype(np.float64)), 0.05).astype(np.float32)
     inr = np.all((Xf >= net["lo"]) & (Xf <= net["hi"]), axis=1)
@@ -286,6 +288,45 @@ def test_fused_closure_collide_matches_reference():
     # nodes outside the range take the Smagorinsky rate: identical to the plain solver there
     out = ~inr.reshape(fs.shape[1:])
     assert np.abs(g[:, out] - g_smag[:, out]).max() < 1e-6 * scale
+
+
+def test_closure_strain_uses_the_last_applied_rate():
+    """Pi_neq before a collision scales with the rate the node was last
+    relaxed at (tau0 + 3 nu_t), so from the second step on the closure's
+    strain estimate S = -3 omega/(2 rho) Pi_neq takes that rate: two steps
+    against the NumPy network and the oracle's collide with the step-1 rates
+    (the network's, or the Smagorinsky rate where the inputs left the range)
+    in the step-2 features.  With the base rate instead the two differ
+    measurably."""
+    import ctypes
+    c, f, m, net = _closure_setup(9)
+    say("closure strain from the last rate")
+    with L.Solver(c) as s:
+        s.set_populations(f)
+        s.set_closure(m)
+        s.step(2)
+        g = s.populations()
+    fs1 = _pre_collision(c, f)
+    inr1, om1, _ = _reference_rates(c, fs1, net)
+    # the Smagorinsky rate of the out-of-range nodes, from the oracle's tau(Q) law
+    pl = orc_params(c)
+    fr = fs1.reshape(19, -1)
+    w = ctypes.c_float()
+    last = om1.copy()
+    for n in np.flatnonzero(~inr1):
+        O.lib().orc_les_omega_f32(ctypes.byref(pl), np.ascontiguousarray(fr[:, n]), ctypes.byref(w), None)
+        last[n] = w.value
+    f1 = _reference_collide(c, fs1, inr1, om1)
+    fs2 = _pre_collision(c, f1)
+    inr2, om2, _ = _reference_rates(c, fs2, net, last)
+    ref = _reference_collide(c, fs2, inr2, om2)
+    inr_b, om_b, _ = _reference_rates(c, fs2, net)  # the base rate in the features
+    ref_base = _reference_collide(c, fs2, inr_b, om_b)
+    scale = np.abs(ref).max()
+    err = np.abs(g - ref).max() / scale
+    gap = np.abs(ref_base - ref).max() / scale
+    say("  err", err, "base-rate reference differs by", gap)
+    assert err < 1e-5 and gap > 5e-5
 
 
 def test_closure_refresh_interval_matches_reference():
