@@ -33,7 +33,12 @@ HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 MLP_FLOP_PER_NODE = 2 * (7 * 64 + 64 * 64 + 64 * 64 + 64)  # algorithmic, padding excluded
 BYTES_PER_NODE = {"f32": 152, "f16": 76, "bf16": 76, "f64": 304}
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stream_collide_channel512_traffic.json")
+# L2-to-memory request counters of one pass each (rocprofv3 --pmc allows 4
+# TCC counters per pass): FETCH_SIZE's three, then WRITE_SIZE's two
+TRAFFIC_PASSES = (("read", ("TCC_EA0_RDREQ_sum", "TCC_EA0_RDREQ_32B_sum", "TCC_BUBBLE_sum")),
+                  ("write", ("TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum")))
+ESIZE = {"f32": 4, "f16": 2, "bf16": 2, "f64": 8}
+Q_DIRS = 19
 
 
 WORKLOADS = {
@@ -262,6 +267,112 @@ def copy_calibration(nbytes=637_534_208, reps=20):
                       "frac_of_8TBps": round(gbs / HBM_PEAK_GBS, 4)}), flush=True)
 
 
+def traffic_probe(storage):
+    """The program the counter passes run (under rocprofv3 --pmc): three
+    device copies of the bytes the kernel reads per launch (the counters'
+    reference of known size), then the channel's stream-collide in the given
+    storage, 3 steps after 5 of warm-up."""
+    import lbm_multigpu_amd as L
+    from lbm_multigpu_amd import config as C
+
+    cfg = C.channel(storage=storage)
+    copy_calibration(Q_DIRS * ESIZE[storage] * cfg.nx * cfg.ny * cfg.nz, reps=3)
+    with L.Solver(cfg.to_c()) as s:
+        s.set_geometry(None)
+        s.init_equilibrium(L.INIT_BGK)
+        s.step(5)
+        s.step_timed(3)
+
+
+def _counter_rows(out_dir):
+    """(kernel name, dispatch id, counter, value) rows of a rocprofv3
+    counter-collection CSV under out_dir."""
+    import csv
+    import glob
+
+    files = glob.glob(os.path.join(out_dir, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise RuntimeError("no counter_collection.csv under " + out_dir)
+    rows = []
+    with open(files[0]) as fh:
+        for r in csv.DictReader(fh):
+            rows.append((r["Kernel_Name"], int(r["Dispatch_Id"]), r["Counter_Name"], float(r["Counter_Value"])))
+    return rows
+
+
+def _per_dispatch(rows, match):
+    """{counter: value} of each dispatch whose kernel name contains match, in dispatch order."""
+    d = {}
+    for name, disp, ctr, val in rows:
+        if match in name:
+            d.setdefault(disp, {})[ctr] = d.get(disp, {}).get(ctr, 0.0) + val
+    return [d[k] for k in sorted(d)]
+
+
+def measure_traffic(storage, timeout_s=180):
+    """HBM bytes per stream-collide launch from the L2-to-memory request
+    counters, in two rocprofv3 --pmc passes over traffic_probe() run as child
+    processes (before this process touches the GPU).
+
+    Reads: FETCH_SIZE's terms, 32 B x TCC_EA0_RDREQ_32B + 128 B x TCC_BUBBLE
+    + b x the remaining TCC_EA0_RDREQ, where FETCH_SIZE takes b = 64 B; b is
+    instead solved from the copy of known size in the same pass (it reads
+    128 B per request on this part, where FETCH_SIZE reads half the bytes).
+    Writes: WRITE_SIZE's terms, 64 B x TCC_EA0_WRREQ_64B + 32 B x the rest,
+    checked against the copy's known bytes (the ratio is reported).
+    Returns (result dict, None) or (None, reason)."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None, "rocprofv3 not on PATH"
+    nodes = 512 * 128 * 128
+    copy_bytes = Q_DIRS * ESIZE[storage] * nodes
+    res = {"kernel": f"k_stream_collide ({storage} channel 512x128x128)", "copy_bytes": copy_bytes,
+           "algorithmic_bytes": BYTES_PER_NODE[storage] * nodes, "commands": []}
+    with tempfile.TemporaryDirectory() as tmp:
+        for tag, ctrs in TRAFFIC_PASSES:
+            out = os.path.join(tmp, tag)
+            cmd = [prof, "--pmc", *ctrs, "--output-format", "csv", "-d", out, "-o", tag, "--",
+                   sys.executable, os.path.abspath(__file__), "--traffic-probe", "--storage", storage]
+            res["commands"].append(" ".join(["rocprofv3", "--pmc", *ctrs, "--output-format", "csv", "-d", "<dir>",
+                                             "-o", tag, "--", "python3", "bench.py", "--traffic-probe",
+                                             "--storage", storage]))
+            try:
+                r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s, cwd=ROOT)
+            except subprocess.TimeoutExpired:
+                return None, f"{tag} pass timed out"
+            try:
+                rows = _counter_rows(out)
+            except (RuntimeError, KeyError, ValueError) as e:
+                return None, f"{tag} pass (rc {r.returncode}): {e}: {r.stderr[-300:]}"
+            copies = _per_dispatch(rows, "copyBuffer")
+            kern = _per_dispatch(rows, "k_stream_collide")
+            if not copies or len(kern) < 3:
+                return None, f"{tag} pass: {len(copies)} copy and {len(kern)} kernel dispatches"
+            c, k = copies[-1], kern[-1]
+            if tag == "read":
+                R, R32, B = (ctrs[0], ctrs[1], ctrs[2])
+                other_c = c[R] - c[R32] - c[B]
+                b = (copy_bytes - 32 * c[R32] - 128 * c[B]) / other_c if other_c > 0 else 64.0
+                rd = 32 * k[R32] + 128 * k[B] + b * (k[R] - k[R32] - k[B])
+                res.update(read_bytes=round(rd), read_request_bytes=round(b, 2),
+                           fetch_size_kib=round((32 * k[R32] + 128 * k[B] + 64 * (k[R] - k[R32] - k[B])) / 1024, 1),
+                           read_counters={n: k[n] for n in ctrs}, copy_read_counters={n: c[n] for n in ctrs})
+            else:
+                W, W64 = ctrs
+                wr = 64 * k[W64] + 32 * (k[W] - k[W64])
+                wc = 64 * c[W64] + 32 * (c[W] - c[W64])
+                res.update(write_bytes=round(wr), write_size_kib=round(wr / 1024, 1),
+                           copy_write_ratio=round(wc / copy_bytes, 4),
+                           write_counters={n: k[n] for n in ctrs}, copy_write_counters={n: c[n] for n in ctrs})
+    res["bytes_per_launch"] = res["read_bytes"] + res["write_bytes"]
+    res["ratio_to_algorithmic"] = round(res["bytes_per_launch"] / res["algorithmic_bytes"], 4)
+    return res, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -275,11 +386,17 @@ def main():
     ap.add_argument("--no-secondary", action="store_true", help="skip the FP16-storage secondary figure")
     ap.add_argument("--forces", action="store_true",
                     help="record the momentum-exchange force every step (timed with the steps)")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure the kernel's HBM traffic")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain device copy of one lattice instead (HBM ceiling / counter scale)")
     ap.add_argument("--cpu-plan", metavar="OUT", nargs="?", const="",
                     help="run only the CPU baseline plan over every workload (several minutes)")
     a = ap.parse_args()
+    if a.traffic_probe:
+        traffic_probe(a.storage)
+        return
     if a.calibrate_copy:
         copy_calibration()
         return
@@ -291,6 +408,12 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    # the counter passes run first, as child processes, before this process
+    # initialises the GPU
+    traffic, traffic_err = None, None
+    if a.workload == "channel" and not a.no_traffic:
+        traffic, traffic_err = measure_traffic(a.storage)
+
     import lbm_multigpu_amd as L
     from lbm_multigpu_amd import config as C
 
@@ -301,9 +424,10 @@ def main():
             return C.channel(nx=256, ny=64, nz=64, side=8, x_center=64, tau=0.51, storage=storage)
         return C.channel(storage=storage)
 
-    def measure(cfg, forces):
+    def measure(cfg, forces, warmup=None):
         """(ms per step from HIP events on the library's stream, wall s, FP64
         mass sum, fine-level sizes or None)"""
+        warmup = a.warmup if warmup is None else warmup
         if forces:
             # record the momentum-exchange force of every step, as the
             # reference does from STARTF on
@@ -321,7 +445,7 @@ def main():
                 big = np.full(7, 3e38, np.float32)
                 mlp.set_norm(np.zeros(7, np.float32), np.ones(7, np.float32), -big, big, 1e-4)
                 s.set_closure(mlp, every=a.closure_every)
-            s.step(a.warmup)
+            s.step(warmup)
             s.step_timed(1)  # drains the warm-up
             t0 = time.perf_counter()
             ms = s.step_timed(a.steps)
@@ -337,25 +461,25 @@ def main():
     cells = cfg.nx * cfg.ny * cfg.nz
     updates = cells + (2 * fine.nx * fine.ny * fine.nz if fine is not None else 0)
     glups = updates / (ms_step * 1e-3) / 1e9
-    traffic = None
     if a.workload == "closure":
         ach = MLP_FLOP_PER_NODE * cells / (ms_step * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(ach / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None}
     else:
         ach = BYTES_PER_NODE[a.storage] * updates / (ms_step * 1e-3) / 1e9
-        if os.path.exists(TRAFFIC_FILE) and a.storage == "f32" and a.workload == "channel":
-            with open(TRAFFIC_FILE) as fh:
-                traffic = json.load(fh).get("bytes_per_launch")
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic}
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic["bytes_per_launch"] if traffic else None}
     # the 16-bit storage variant of the same workload, measured in the same
     # run as a secondary figure (76 algorithmic bytes per node update)
     secondary = None
     if a.workload == "channel" and a.storage == "f32" and not a.no_secondary and not a.forces:
         sec = {}
         for st in ("f16",):
-            ms2, _, _, _ = measure(make_cfg(st), False)
+            # past the start-up transient: the 16-bit kernels, bound by the
+            # vector ALUs, run up to 25 % slower over the first ~150 steps
+            # from the uniform start (127 us/step after it on the 512x128x128
+            # channel, 133-160 us within it); FP32 does not vary
+            ms2, _, _, _ = measure(make_cfg(st), False, warmup=max(a.warmup, 200))
             g2 = cells / (ms2 * 1e-3) / 1e9
             ach2 = BYTES_PER_NODE[st] * cells / (ms2 * 1e-3) / 1e9
             sec[st] = {"value": round(g2, 3), "unit": "GLUPS", "ms_per_step": round(ms2, 4),
@@ -372,6 +496,8 @@ def main():
     }
     if secondary:
         line["secondary_storage"] = secondary
+    if traffic or traffic_err:
+        line["traffic_counters"] = traffic if traffic else {"error": traffic_err}
     if a.forces:
         line["config"]["forces_recorded_every_step"] = True
     print(json.dumps(line), flush=True)

@@ -193,10 +193,11 @@ __device__ __forceinline__ void y_nb(const StepParams& p, int y, int& m, int& pl
     }
 }
 
-// The 19 pulled populations of node (x, y, z): neighbour rows per axis
-// (periodic, clamped or walled), then one load per direction.
+// The 19 pulled populations of node (x, y, z) as stored (raw words, decoded
+// once the boundary handling has replaced what it replaces): neighbour rows
+// per axis (periodic, clamped or walled), then one load per direction.
 template <class S>
-__device__ __forceinline__ void node_load(const StepParams& p, int x, int y, int z, typename Arith<S>::R f[Q])
+__device__ __forceinline__ void node_load(const StepParams& p, int x, int y, int z, typename Store<S>::Raw r[Q])
 {
     const S* __restrict__ fin = static_cast<const S*>(p.fin);
     int xm, xp, ym, yp, zm, zp;
@@ -210,7 +211,7 @@ __device__ __forceinline__ void node_load(const StepParams& p, int x, int y, int
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const int src = p.nxp * (Y[1 - CY[i]] + p.ny * Z[1 - CZ[i]]) + X[1 - CX[i]];
-        f[i] = Store<S>::load(at(fin + i * p.qs, src), i, p.sh);
+        r[i] = Store<S>::load_raw(at(fin + i * p.qs, src));
     }
 }
 
@@ -219,8 +220,11 @@ template <class S>
 __device__ __forceinline__ void node_unstreamed(const StepParams& p, int cell, typename Arith<S>::R f[Q])
 {
     const S* __restrict__ fin = static_cast<const S*>(p.fin);
+    typename Store<S>::Raw r[Q]; // every load issued before the first decode
 #pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = Store<S>::load(at(fin + i * p.qs, cell), i, p.sh);
+    for (int i = 0; i < Q; ++i) r[i] = Store<S>::load_raw(at(fin + i * p.qs, cell));
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = Store<S>::decode(r[i], i, p.sh);
 }
 
 // Node codes at and above NODE_OUTLET_NORTH: the y-streamwise Extrap family
@@ -302,20 +306,30 @@ __device__ __forceinline__ int node_rare(const StepParams& p, int x, int y, int 
 // bounce-back).
 template <class S>
 __device__ __forceinline__ int node_fix(const StepParams& p, int x, int y, int z, int cell, unsigned char code,
-                                        typename Arith<S>::R f[Q], typename Arith<S>::R& fx,
-                                        typename Arith<S>::R& fy, typename Arith<S>::R& fz)
+                                        typename Store<S>::Raw r[Q], typename Arith<S>::R f[Q],
+                                        typename Arith<S>::R& fx, typename Arith<S>::R& fy,
+                                        typename Arith<S>::R& fz)
 {
     using R = typename Arith<S>::R;
     const S* __restrict__ fin = static_cast<const S*>(p.fin);
     const long long qs = p.qs;
     const unsigned char base = code & NODE_CODE_MASK;
-    if (p.halfway && is_solid(base)) return ACT_SKIP; // inert under link bounce-back
+    // a solid under link bounce-back is inert; it is dropped only after the
+    // decode below, which every path passes, so the compiler cannot move the
+    // pulled loads behind the wait for the node code
+    const bool inert = p.halfway && is_solid(base);
 
-    if (code & NODE_NEAR) {
+    // links bounced from a force-summed solid, as bits: the momentum exchange
+    // is summed after the loads, off their dependency chain
+    unsigned flinks = 0;
+    if ((code & NODE_NEAR) && !inert) {
         // links that bounce: out of the domain through a wall face, or (link
         // mode) from a solid node; the population comes back reversed from
         // this node's own previous post-collision value.  The neighbours'
-        // codes are read only where a solid neighbour exists.
+        // codes are read only where a solid neighbour exists, all of them
+        // requested before any is tested (each load on its own would wait a
+        // memory round trip), and the bounced words replace the pulled ones
+        // before anything is decoded, so those loads are in flight together.
         const bool solid_nb = p.halfway && (code & NODE_NEAR_SOLID);
         int xm, xp, ym, yp, zm, zp;
         bool oxm, oxp, oym, oyp, ozm, ozp;
@@ -325,29 +339,53 @@ __device__ __forceinline__ int node_fix(const StepParams& p, int x, int y, int z
         const int Y[3] = {ym, y, yp};
         const int Z[3] = {zm, z, zp};
         const int X[3] = {xm, x, xp};
-        // links bounced from a force-summed solid, as bits: the momentum
-        // exchange is summed after the loop, off the loads' dependency chain
-        unsigned flinks = 0;
+        unsigned bounce = 0;
 #pragma unroll
         for (int i = 1; i < Q; ++i) {
             const bool out = (CX[i] > 0 && oxm) || (CX[i] < 0 && oxp) || (CY[i] > 0 && oym) ||
                              (CY[i] < 0 && oyp) || (CZ[i] > 0 && ozm) || (CZ[i] < 0 && ozp);
-            unsigned char sc = 0;
-            if (!out && solid_nb)
-                sc = *at(p.mask, p.nxp * (Y[1 - CY[i]] + p.ny * Z[1 - CZ[i]]) + X[1 - CX[i]]) & NODE_CODE_MASK;
-            const bool bounce = out || is_solid(sc);
-            if (bounce) f[i] = Store<S>::load(at(fin + OPP[i] * qs, cell), OPP[i], p.sh);
-            if (sc == NODE_SOLID_FORCE) flinks |= 1u << i;
+            if (out) bounce |= 1u << i;
         }
-        if (p.fpart && flinks) {
+        if (solid_nb) {
+            // every neighbour index is inside the level (axis_nb clamps or wraps)
+            unsigned char sc[Q];
 #pragma unroll
             for (int i = 1; i < Q; ++i)
-                if (flinks & (1u << i)) {
-                    fx += R(2 * CX[OPP[i]]) * f[i];
-                    fy += R(2 * CY[OPP[i]]) * f[i];
-                    fz += R(2 * CZ[OPP[i]]) * f[i];
-                }
+                sc[i] = *at(p.mask, p.nxp * (Y[1 - CY[i]] + p.ny * Z[1 - CZ[i]]) + X[1 - CX[i]]);
+            const unsigned out = bounce;
+#pragma unroll
+            for (int i = 1; i < Q; ++i) {
+                if (out & (1u << i)) continue;
+                const unsigned char b = sc[i] & NODE_CODE_MASK;
+                if (is_solid(b)) bounce |= 1u << i;
+                if (b == NODE_SOLID_FORCE) flinks |= 1u << i;
+            }
         }
+        // (through per-plane buffer descriptors instead, the descriptors'
+        // scalar registers spill: 140 vs 126 us per 16-bit channel step)
+#pragma unroll
+        for (int i = 1; i < Q; ++i)
+            if (bounce & (1u << i)) r[i] = Store<S>::load_raw(at(fin + OPP[i] * qs, cell));
+        // a bounced word comes from plane OPP[i] and decodes with that
+        // plane's 16-bit offset, which differs from plane i's where c_x != 0
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            if (CX[i] == 0) f[i] = Store<S>::decode(r[i], i, p.sh);
+            else f[i] = Store<S>::decode_shift(r[i], (bounce & (1u << i)) ? p.sh[OPP[i]] : p.sh[i]);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] = Store<S>::decode(r[i], i, p.sh);
+    }
+    if (inert) return ACT_SKIP;
+    if (p.fpart && flinks) {
+#pragma unroll
+        for (int i = 1; i < Q; ++i)
+            if (flinks & (1u << i)) {
+                fx += R(2 * CX[OPP[i]]) * f[i];
+                fy += R(2 * CY[OPP[i]]) * f[i];
+                fz += R(2 * CZ[OPP[i]]) * f[i];
+            }
     }
 
     if (is_solid(base)) {
@@ -375,8 +413,7 @@ __device__ __forceinline__ int node_fix(const StepParams& p, int x, int y, int z
         // Extrap conditions start from the unstreamed neighbour (:1725-1746)
         const int nb = base == NODE_INLET_WEST ? (x + 1 < p.nx ? cell + 1 : cell)
                                                : (x > 0 ? cell - 1 : cell);
-#pragma unroll
-        for (int i = 0; i < Q; ++i) f[i] = Store<S>::load(at(fin + i * qs, nb), i, p.sh);
+        node_unstreamed<S>(p, nb, f);
         if (base == NODE_INLET_WEST) west_extrap(f, R(p.u_in[z * p.ny + y]));
         else east_extrap(f, R(1));
     } else if (base == NODE_LID_WEST_REG) {
@@ -402,8 +439,9 @@ __device__ __forceinline__ int node_pull(const StepParams& p, int x, int y, int 
                                          typename Arith<S>::R& fy, typename Arith<S>::R& fz)
 {
     const unsigned char code = *at(p.mask, cell);
-    node_load<S>(p, x, y, z, f);
-    return node_fix<S>(p, x, y, z, cell, code, f, fx, fy, fz);
+    typename Store<S>::Raw r[Q];
+    node_load<S>(p, x, y, z, r);
+    return node_fix<S>(p, x, y, z, cell, code, r, f, fx, fy, fz);
 }
 
 template <class R>

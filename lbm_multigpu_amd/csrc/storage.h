@@ -100,6 +100,10 @@ __device__ __forceinline__ void buffer_put(u32x2 w, __amdgpu_buffer_rsrc_t r, ui
 
 template <>
 struct Store<float> {
+    using Raw = float;
+    __device__ static __forceinline__ Raw load_raw(const float* p) { return ld(p); }
+    __device__ static __forceinline__ float decode(Raw r, int, const PopShift&) { return r; }
+    __device__ static __forceinline__ float decode_shift(Raw r, float) { return r; }
     __device__ static __forceinline__ float load(const float* p, int, const PopShift&) { return ld(p); }
     __device__ static __forceinline__ void store(float* p, float v, int, const PopShift&) { st(p, v); }
     __device__ static __forceinline__ uint32_t raw(float v, int, const PopShift&) { return __builtin_bit_cast(uint32_t, v); }
@@ -109,6 +113,10 @@ struct Store<float> {
 
 template <>
 struct Store<double> {
+    using Raw = double;
+    __device__ static __forceinline__ Raw load_raw(const double* p) { return ld(p); }
+    __device__ static __forceinline__ double decode(Raw r, int, const PopShift&) { return r; }
+    __device__ static __forceinline__ double decode_shift(Raw r, float) { return r; }
     __device__ static __forceinline__ double load(const double* p, int, const PopShift&) { return ld(p); }
     __device__ static __forceinline__ void store(double* p, double v, int, const PopShift&) { st(p, v); }
     __device__ static __forceinline__ u32x2 raw(double v, int, const PopShift&) { return __builtin_bit_cast(u32x2, v); }
@@ -155,12 +163,24 @@ __host__ __device__ inline uint16_t float_to_bf16_bits(float v)
 // they sit in scalar registers.
 template <>
 struct Store<Half16> {
-    __device__ static __forceinline__ float load(const Half16* p, int I, const PopShift& sh)
+    using Raw = unsigned; // the stored bits, zero-extended
+    __device__ static __forceinline__ Raw load_raw(const Half16* p) { return ld(&p->bits); }
+    __device__ static __forceinline__ float decode(Raw h, int I, const PopShift& sh)
     {
-        const unsigned h = ld(&p->bits);
         float v;
         asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(v) : "v"(h), "s"(sh[I]));
         return v;
+    }
+    // the same with a per-lane offset
+    __device__ static __forceinline__ float decode_shift(Raw h, float shift)
+    {
+        float v;
+        asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(v) : "v"(h), "v"(shift));
+        return v;
+    }
+    __device__ static __forceinline__ float load(const Half16* p, int I, const PopShift& sh)
+    {
+        return decode(load_raw(p), I, sh);
     }
     __device__ static __forceinline__ uint16_t raw(float v, int I, const PopShift& sh)
     {
@@ -184,9 +204,19 @@ struct Store<Half16> {
 
 template <>
 struct Store<BHalf16> {
+    using Raw = unsigned; // the stored bits, zero-extended
+    __device__ static __forceinline__ Raw load_raw(const BHalf16* p) { return ld(&p->bits); }
+    __device__ static __forceinline__ float decode(Raw h, int I, const PopShift& sh)
+    {
+        return bf16_bits_to_float((uint16_t)h) + sh[I];
+    }
+    __device__ static __forceinline__ float decode_shift(Raw h, float shift)
+    {
+        return bf16_bits_to_float((uint16_t)h) + shift;
+    }
     __device__ static __forceinline__ float load(const BHalf16* p, int I, const PopShift& sh)
     {
-        return bf16_bits_to_float(ld(&p->bits)) + sh[I];
+        return decode(load_raw(p), I, sh);
     }
     __device__ static __forceinline__ uint16_t raw(float v, int I, const PopShift& sh)
     {
