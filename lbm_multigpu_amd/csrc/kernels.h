@@ -112,6 +112,14 @@ struct StepParams {
                         // nullptr: slot = workgroup
     int y_wrap;         // rows 0 .. y_wrap periodic in y among themselves (-1: no wrap)
     int ghost_y[3];     // source rows of the im 52 / 53 / 54 ghost copies
+    // coarse level under refinement: nodes of the footprint box fp_a ..
+    // fp_a + fp_n - 1 also write their post-collision physical moments and
+    // CF data (fp_n[0] = 0: none; footprint_extract)
+    int fp_a[3], fp_n[3];
+    const float* fp_mold; // moments at the step's start (temporal midpoint)
+    float* fp_m;          // moments at the step's end [9][footprint]
+    float* fp_pnew;       // CF data of the end state, 16 floats per node
+    float* fp_phalf;      // CF data of the midpoint, or nullptr
     int xcd_chunk; // consecutive workgroups per XCD (logical_block)
     int nt_store;  // nontemporal population stores (lattices larger than the MALL)
 };
@@ -536,6 +544,11 @@ __device__ __forceinline__ void node_finish(const StepParams& p, int y, int cell
     node_store_policy<S, AUX>(p, cell, f);
 }
 
+// Footprint moments of a coarse node under refinement (defined with the
+// interface kernels).
+template <class S, class R>
+__device__ void footprint_extract(const StepParams& p, int x, int y, int z, int cell, int act, R f[Q]);
+
 // Fused pull stream + boundary dispatch + collide + bounce-back for one node.
 template <int MODEL, class S, int AUX>
 __device__ __forceinline__ void node_update(const StepParams& p, int x, int y, int z,
@@ -547,6 +560,7 @@ __device__ __forceinline__ void node_update(const StepParams& p, int x, int y, i
     R f[Q];
     const int act = node_pull<S>(p, x, y, z, cell, f, fx, fy, fz);
     node_finish<MODEL, S, AUX>(p, y, cell, act, f);
+    if (p.fp_n[0] > 0) footprint_extract<S>(p, x, y, z, cell, act, f);
 }
 
 // Momentum-exchange partial of one workgroup: wavefront sums, then the

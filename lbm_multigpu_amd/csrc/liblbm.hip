@@ -27,6 +27,10 @@ namespace {
 
 thread_local std::string g_last_error;
 
+#ifndef LBM_FINE_OVERLAP
+#define LBM_FINE_OVERLAP 1 // first fine sub-step on the side stream beside the coarse step
+#endif
+
 struct Level {
     int nx = 0, ny = 0, nz = 0, nxp = 0, nsegw = 0;
     long long qs = 0;
@@ -52,6 +56,11 @@ struct lbm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // refinement: the first fine sub-step's stream beside the coarse step
+    // (default priority), and the events that fork it from and join it to
+    // the main stream
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     Level lv[2];
     int nlevels = 1;
     float* u_in = nullptr;     // west inlet profile [nz*ny]
@@ -117,9 +126,9 @@ struct lbm_mlp {
 static int refine_setup(lbm_ctx* ctx);
 static void classify_fine(lbm_ctx* ctx, bool user_mask);
 static int refine_extract(lbm_ctx* ctx, const float* Mold, float* M, float* Pnew, float* Phalf);
-static int refine_cf(lbm_ctx* ctx, const float* P);
+static int refine_cf(lbm_ctx* ctx, const float* P, hipStream_t stream);
 static int refine_fc(lbm_ctx* ctx);
-static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f, bool record);
+static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f, double* fpart, hipStream_t stream);
 static int set_geometry_at(lbm_ctx* ctx, const uint8_t* mask, long long t);
 static int flush_forces(lbm_ctx* ctx);
 
@@ -393,6 +402,11 @@ extern "C" int lbm_create(const lbm_config* cfg, lbm_ctx** out)
         return bail(fail(nullptr, LBM_EDEVICE, "lbm_create: stream creation failed"));
     if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess)
         return bail(fail(nullptr, LBM_EDEVICE, "lbm_create: event creation failed"));
+    if (cfg->refinement &&
+        (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+         hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+         hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess))
+        return bail(fail(nullptr, LBM_EDEVICE, "lbm_create: refinement stream or event creation failed"));
     if ((rc = alloc_level(ctx, ctx->lv[0], cfg->nx, cfg->ny, cfg->nz, cfg->storage)) != LBM_OK) {
         g_last_error = ctx->err;
         return bail(rc);
@@ -447,8 +461,12 @@ extern "C" void lbm_destroy(lbm_ctx* ctx)
                     (void*)ctx->dwall,
                     (void*)ctx->omega_field})
         if (p) (void)hipFree(p);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -800,6 +818,10 @@ static StepParams step_params(lbm_ctx* ctx, const Level& L, float omega)
     p.s1 = c.s_dhum[1]; p.s2 = c.s_dhum[2]; p.s4 = c.s_dhum[4]; p.s10 = c.s_dhum[10]; p.s16 = c.s_dhum[16];
     p.omega_node = nullptr;
     p.sh = L.shift;
+    p.fp_n[0] = p.fp_n[1] = p.fp_n[2] = 0; // no footprint extraction (coarse_step sets it)
+    p.fp_a[0] = p.fp_a[1] = p.fp_a[2] = 0;
+    p.fp_mold = nullptr;
+    p.fp_m = p.fp_pnew = p.fp_phalf = nullptr;
     p.xcd_chunk = xcd_chunk_of(L.storage, L.nsegw);
     // both lattices within the 256 MB last-level cache: normal stores, so the
     // next step's loads can hit it (128x96x96 FP32: 31 vs 25 GLUPS); beyond
@@ -963,6 +985,18 @@ static int coarse_step(lbm_ctx* ctx)
         if (int rc = ensure_fpart(ctx, (size_t)L.blocks * 3)) return rc;
         p.fpart = ctx->fpart;
     }
+    if (ctx->nlevels > 1) {
+        // the footprint's moments at the step's end and the CF data of this
+        // step's second fine sub-step and of the next step's first
+        for (int a = 0; a < 3; ++a) {
+            p.fp_a[a] = ctx->rg.a[a];
+            p.fp_n[a] = ctx->rg.nc[a];
+        }
+        p.fp_mold = ctx->Mold;
+        p.fp_m = ctx->Mnew;
+        p.fp_pnew = ctx->Pnew;
+        p.fp_phalf = c.temporal_interp ? ctx->Phalf : nullptr;
+    }
     long long nparts = 0;
     with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
@@ -988,22 +1022,23 @@ static int coarse_step(lbm_ctx* ctx)
     return LBM_OK;
 }
 
-// Fine-level sub-step (update_inn_LR, MRTtest/DynInlet3.cu:2051-2201).
-// With record set (the first sub-step of a recorded coarse step, flag_F at
-// :3183) the fine level's momentum exchange goes to the force history.
-static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f, bool record)
+// Fine-level sub-step (update_inn_LR, MRTtest/DynInlet3.cu:2051-2201) on
+// the given stream.  With fpart set (the batch position of the first
+// sub-step of a recorded coarse step, flag_F at :3183) the fine level's
+// momentum exchange goes to the force history.
+static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f, double* fpart, hipStream_t stream)
 {
     Level& F = ctx->lv[1];
     const lbm_config& c = ctx->cfg;
     StepParams p = step_params(ctx, F, omega_f);
     p.bcx = p.bcy = p.bcz = LBM_BC_CLAMP;
-    if (record) {
-        if (int rc = batch_slot(ctx, ctx->t, F.nfslot * nwave_of_storage(F.storage), &p.fpart)) return rc;
+    if (fpart) {
+        p.fpart = fpart;
         p.fslot = F.fslot;
     }
     with_storage(F.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        launch_sc<S>(c.model, p, F.blocks, ctx->stream);
+        launch_sc<S>(c.model, p, F.blocks, stream);
         return 0;
     });
     std::swap(F.fa, F.fb);
@@ -1081,13 +1116,13 @@ static int refine_extract(lbm_ctx* ctx, const float* Mold, float* M, float* Pnew
     return LBM_OK;
 }
 
-static int refine_cf(lbm_ctx* ctx, const float* P)
+static int refine_cf(lbm_ctx* ctx, const float* P, hipStream_t stream)
 {
     Level& F = ctx->lv[1];
     with_storage(F.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
         const long long n = frame_nodes(ctx->rg.nf);
-        hipLaunchKernelGGL(k_interp_cf<S>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (S*)F.fa,
+        hipLaunchKernelGGL(k_interp_cf<S>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, (S*)F.fa,
                            F.qs, F.nxp, F.mask, (const f32x4v*)P, ctx->rg, F.shift);
         return 0;
     });
@@ -1139,10 +1174,16 @@ static float plane_mean(const Level& L, double sum)
 // 3147-3170): AverageV on the plane y = DYNY1, then dpdy += diff*KP*|DPDY|
 // with diff = (Av_V - UMAX)/UMAX, in FP32.  DPDY is the configured initial
 // force.  The new dpdy acts from the next step on.
+static bool control_due(const lbm_ctx* ctx, long long t)
+{
+    const lbm_config& c = ctx->cfg;
+    return c.ctrl_every > 0 && t % c.ctrl_every == 0 && t > c.ctrl_start;
+}
+
 static int inlet_control(lbm_ctx* ctx, long long t)
 {
     const lbm_config& c = ctx->cfg;
-    if (c.ctrl_every <= 0 || t % c.ctrl_every != 0 || t <= c.ctrl_start) return LBM_OK;
+    if (!control_due(ctx, t)) return LBM_OK;
     const Level& L = ctx->lv[0];
     double sum = 0;
     int rc = plane_v(ctx, c.dyn_y1 >= 0 ? c.dyn_y1 : L.ny - 1, &sum);
@@ -1194,21 +1235,49 @@ extern "C" int lbm_step(lbm_ctx* ctx, int nsteps)
         // ImageFcn(x, y, z, t): the trip obstacle exists for t < trip_until
         if (!ctx->user_mask && c.geometry == LBM_GEOM_PRECURSOR && ctx->trip_on != (ctx->t < c.trip_until))
             if (int rc = set_geometry_at(ctx, nullptr, ctx->t)) return rc;
-        if (int rc = coarse_step(ctx)) return rc;
-        if (int rc = inlet_control(ctx, ctx->t)) return rc;
         if (ctx->nlevels > 1) {
-            // coarse t -> t+1 done; fine sub-steps t -> t+1/2 -> t+1 (main(),
-            // MRTtest/DynInlet3.cu:3180-3245), then the fine -> coarse restriction
+            // coarse t -> t+1, the dynamic-inlet control; fine sub-steps
+            // t -> t+1/2 -> t+1 (main(), MRTtest/DynInlet3.cu:3147-3245), then
+            // the fine -> coarse restriction.  The first sub-step and its CF
+            // pass read only the fine level and the coarse data packed one
+            // step earlier (Pold), so they run on the side stream beside the
+            // coarse stream-collide and the footprint extraction; the fork
+            // follows every earlier main-stream launch (the last restriction,
+            // a force-batch flush).  On a control step the body force they
+            // use changes after the coarse step, so they wait for it there.
             const bool temporal = ctx->cfg.temporal_interp;
-            int rc = refine_extract(ctx, ctx->Mold, ctx->Mnew, ctx->Pnew, temporal ? ctx->Phalf : nullptr);
-            if (!rc) rc = refine_cf(ctx, ctx->Pold);
-            if (!rc) rc = lbm_internal_fine_step(ctx, ctx->rg.omega_f, records_force(ctx, ctx->t));
-            if (!rc && temporal) rc = refine_cf(ctx, ctx->Phalf);
-            if (!rc) rc = lbm_internal_fine_step(ctx, ctx->rg.omega_f, false);
+            const bool overlap = LBM_FINE_OVERLAP && !control_due(ctx, ctx->t);
+            Level& F = ctx->lv[1];
+            double* fpart = nullptr;
+            if (records_force(ctx, ctx->t))
+                if (int rc = batch_slot(ctx, ctx->t, F.nfslot * nwave_of_storage(F.storage), &fpart)) return rc;
+            int rc = LBM_OK;
+            if (overlap) {
+                HIP_TRY(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
+                HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+                rc = refine_cf(ctx, ctx->Pold, ctx->side);
+                if (!rc) rc = lbm_internal_fine_step(ctx, ctx->rg.omega_f, fpart, ctx->side);
+                if (rc) return rc;
+                HIP_TRY(ctx, hipEventRecord(ctx->ev_join, ctx->side));
+            }
+            if ((rc = coarse_step(ctx))) return rc;
+            if ((rc = inlet_control(ctx, ctx->t))) return rc;
+            if (!overlap) {
+                rc = refine_cf(ctx, ctx->Pold, ctx->stream);
+                if (!rc) rc = lbm_internal_fine_step(ctx, ctx->rg.omega_f, fpart, ctx->stream);
+                if (rc) return rc;
+            }
+            // (the coarse stream-collide wrote Mnew, Pnew and Phalf)
+            if (overlap) HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+            if (temporal) rc = refine_cf(ctx, ctx->Phalf, ctx->stream);
+            if (!rc) rc = lbm_internal_fine_step(ctx, ctx->rg.omega_f, nullptr, ctx->stream);
             if (!rc) rc = refine_fc(ctx);
             if (rc) return rc;
             std::swap(ctx->Mold, ctx->Mnew);
             std::swap(ctx->Pold, ctx->Pnew);
+        } else {
+            if (int rc = coarse_step(ctx)) return rc;
+            if (int rc = inlet_control(ctx, ctx->t)) return rc;
         }
         ctx->t += 1;
     }

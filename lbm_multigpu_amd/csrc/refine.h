@@ -209,6 +209,41 @@ __global__ void __launch_bounds__(256) k_extract(const S* fc, long long qs, int 
     }
 }
 
+// The same from inside the coarse stream-collide (the reference extracts in
+// update_inn, MRTtest/DynInlet3.cu:1801-1815): a footprint node's populations
+// as the launch leaves them in memory -- the stored post-collision values, or
+// for a solid skipped under link bounce-back the values it keeps -- give the
+// moments k_extract would read back, without the second pass over them.
+template <class S, class R>
+__device__ __forceinline__ void footprint_extract(const StepParams& p, int x, int y, int z, int cell, int act, R f[Q])
+{
+    const int i = x - p.fp_a[0], j = y - p.fp_a[1], k = z - p.fp_a[2];
+    if ((unsigned)i >= (unsigned)p.fp_n[0] || (unsigned)j >= (unsigned)p.fp_n[1] || (unsigned)k >= (unsigned)p.fp_n[2])
+        return;
+    const long long NC = (long long)p.fp_n[0] * p.fp_n[1] * p.fp_n[2];
+    const long long o = i + (long long)p.fp_n[0] * (j + (long long)p.fp_n[1] * k);
+    float g[Q];
+    if (act == ACT_SKIP) {
+        const S* fo = static_cast<const S*>(p.fout);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) g[q] = float(Store<S>::to_real(fo[q * p.qs + cell], q, p.sh));
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) g[q] = float(Store<S>::stored(f[q], q, p.sh));
+    }
+    float mom[9];
+    phys_moments(g, mom);
+#pragma unroll
+    for (int m = 0; m < 9; ++m) p.fp_m[m * NC + o] = mom[m];
+    cf_pack(mom, reinterpret_cast<f32x4v*>(p.fp_pnew) + 4 * o);
+    if (p.fp_phalf) {
+        float mid[9];
+#pragma unroll
+        for (int m = 0; m < 9; ++m) mid[m] = (1.f - 0.5f) * p.fp_mold[m * NC + o] + 0.5f * mom[m];
+        cf_pack(mid, reinterpret_cast<f32x4v*>(p.fp_phalf) + 4 * o);
+    }
+}
+
 // Frame node number o of the CF frame (two fine nodes thick on all six faces)
 // -> (i, j, k).  Enumerated as the z slabs (k < 2 or k >= nf_z - 2), then the
 // y slabs of the remaining planes, then the x slabs of the remaining rows; x
@@ -307,11 +342,19 @@ __global__ void __launch_bounds__(256) k_interp_cf(S* ff, long long fqs, int fnx
     for (int q = 0; q < Q; ++q) ff[q * fqs + fcell] = Store<S>::from_real(typename Arith<S>::R(f[q]), q, sh);
 }
 
-// Fine -> coarse.  One workgroup per tile of 32x2x2 coarse nodes: its 256
-// threads load the 64x4x4 fine nodes below them as 16 contiguous 64-node
-// rows (one wavefront per row and plane), reduce them to the 9 physical
-// moments in LDS, and 128 threads then finish one coarse node each.
-constexpr int FC_TX = 32, FC_TY = 2, FC_TZ = 2;
+// Fine -> coarse.  One workgroup per tile of FC_TX x FC_TY x FC_TZ coarse
+// nodes: its 256 threads load the (2 FC_TX) x (2 FC_TY) x (2 FC_TZ) fine nodes
+// below them as contiguous 64-node rows (one wavefront per row and plane),
+// reduce them to the 9 physical moments in LDS, and one thread per coarse
+// node then finishes it.
+#ifndef LBM_FC_TY
+#define LBM_FC_TY 1
+#endif
+#ifndef LBM_FC_TZ
+#define LBM_FC_TZ 1
+#endif
+constexpr int FC_TX = 32, FC_TY = LBM_FC_TY, FC_TZ = LBM_FC_TZ;
+constexpr int FC_ROWS = 4 * FC_TY * FC_TZ; // fine rows of a tile (a multiple of the 4 wavefronts)
 
 template <class S>
 __global__ void __launch_bounds__(256) k_interp_fc(const S* ff, long long fqs, int fnxp, S* fc, long long cqs,
@@ -324,9 +367,9 @@ __global__ void __launch_bounds__(256) k_interp_fc(const S* ff, long long fqs, i
     const int fx0 = 2 * (2 + FC_TX * bx) - 1, fy0 = 2 * (2 + FC_TY * by) - 1, fz0 = 2 * (2 + FC_TZ * bz) - 1;
     const int lxf = threadIdx.x & 63, row0 = threadIdx.x >> 6; // 4 wavefronts
 #pragma unroll
-    for (int rr = 0; rr < 4; ++rr) {
-        const int row = row0 + 4 * rr;          // 0..15: (fy, fz) of the tile
-        const int ly = row & 3, lz = row >> 2;
+    for (int rr = 0; rr < FC_ROWS / 4; ++rr) {
+        const int row = row0 + 4 * rr;          // (fy, fz) of the tile
+        const int ly = row % (2 * FC_TY), lz = row / (2 * FC_TY);
         const int i = fx0 + lxf, j = fy0 + ly, k = fz0 + lz;
         float mom[9];
 #pragma unroll

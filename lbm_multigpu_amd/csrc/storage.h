@@ -104,6 +104,8 @@ struct Store<float> {
     __device__ static __forceinline__ Raw load_raw(const float* p) { return ld(p); }
     __device__ static __forceinline__ float decode(Raw r, int, const PopShift&) { return r; }
     __device__ static __forceinline__ float decode_shift(Raw r, float) { return r; }
+    // the value a store of v leaves in memory
+    __device__ static __forceinline__ float stored(float v, int, const PopShift&) { return v; }
     __device__ static __forceinline__ float load(const float* p, int, const PopShift&) { return ld(p); }
     __device__ static __forceinline__ void store(float* p, float v, int, const PopShift&) { st(p, v); }
     __device__ static __forceinline__ uint32_t raw(float v, int, const PopShift&) { return __builtin_bit_cast(uint32_t, v); }
@@ -117,6 +119,7 @@ struct Store<double> {
     __device__ static __forceinline__ Raw load_raw(const double* p) { return ld(p); }
     __device__ static __forceinline__ double decode(Raw r, int, const PopShift&) { return r; }
     __device__ static __forceinline__ double decode_shift(Raw r, float) { return r; }
+    __device__ static __forceinline__ double stored(double v, int, const PopShift&) { return v; }
     __device__ static __forceinline__ double load(const double* p, int, const PopShift&) { return ld(p); }
     __device__ static __forceinline__ void store(double* p, double v, int, const PopShift&) { st(p, v); }
     __device__ static __forceinline__ u32x2 raw(double v, int, const PopShift&) { return __builtin_bit_cast(u32x2, v); }
@@ -182,6 +185,11 @@ struct Store<Half16> {
     {
         return decode(load_raw(p), I, sh);
     }
+    // the value a store of v leaves in memory (rounded to the 16-bit grid)
+    __device__ static __forceinline__ float stored(float v, int I, const PopShift& sh)
+    {
+        return decode(raw(v, I, sh), I, sh);
+    }
     __device__ static __forceinline__ uint16_t raw(float v, int I, const PopShift& sh)
     {
         unsigned o;
@@ -217,6 +225,11 @@ struct Store<BHalf16> {
     __device__ static __forceinline__ float load(const BHalf16* p, int I, const PopShift& sh)
     {
         return decode(load_raw(p), I, sh);
+    }
+    // the value a store of v leaves in memory (rounded to the 16-bit grid)
+    __device__ static __forceinline__ float stored(float v, int I, const PopShift& sh)
+    {
+        return decode(raw(v, I, sh), I, sh);
     }
     __device__ static __forceinline__ uint16_t raw(float v, int I, const PopShift& sh)
     {
