@@ -336,6 +336,15 @@ int lbm_set_closure(lbm_ctx* ctx, lbm_mlp* m, float nu_cap);
 /* evaluate the network every `every` steps (default 1); the steps between
  * evaluations collide at the per-node rates of the last one */
 int lbm_set_closure_interval(lbm_ctx* ctx, int every);
+/* where the network's eddy viscosity goes (no counterpart in the reference):
+ * 0 (default) tau = tau0 + 3 nu_t; 1 tau = tau_Smagorinsky + 3 nu_t, the
+ * network as a learned correction on top of the configured CS */
+int lbm_set_closure_base(lbm_ctx* ctx, int smagorinsky);
+/* the per-node collision rate of the last closure step, dense [nz][ny][nx]
+ * (coarse level): > 0 the network's, < 0 minus the Smagorinsky rate a node
+ * fell back to (inputs outside the training range); a diagnostic of the
+ * closure, which has no counterpart in the reference */
+int lbm_get_closure_rates(lbm_ctx* ctx, float* rates);
 /* training samples from a context's current state, box-filtered by 2 (the
  * solver's own finer run): features X[n][7] and the SGS eddy viscosity
  * y[n] = max(0, -tau_dev:S/(2 S:S)) of a hashed fraction `keep` of the

@@ -77,6 +77,7 @@ EXPORTS = [
     "lbm_mlp_gradient", "lbm_mlp_train", "lbm_mlp_forward", "lbm_set_closure",
     "lbm_closure_samples", "lbm_stats_enable", "lbm_stats_get", "lbm_set_closure_interval",
     "lbm_get_inlet_control", "lbm_plane_mean_v", "lbm_get_dpdy", "lbm_mass_momentum_noalloc",
+    "lbm_get_closure_rates", "lbm_set_closure_base",
 ]
 
 # closure network: 7 -> 64 -> 64 -> 64 -> 1, flat FP32 parameter vector
@@ -147,6 +148,8 @@ def load_library(path=LIB_PATH):
     lib.lbm_set_closure.argtypes = [P, P, C.c_float]
     lib.lbm_set_closure_interval.argtypes = [P, C.c_int]
     lib.lbm_closure_samples.argtypes = [P, C.c_uint64, C.c_float, f32, f32, C.c_int, C.POINTER(C.c_int)]
+    lib.lbm_get_closure_rates.argtypes = [P, P]
+    lib.lbm_set_closure_base.argtypes = [P, C.c_int]
     _lib = lib
     return lib
 
@@ -316,14 +319,24 @@ class Solver:
         self._check(self.lib.lbm_stats_get(self.h, level, mean.ctypes.data, dev.ctypes.data))
         return mean, dev
 
-    def set_closure(self, mlp, nu_cap=0.05, every=1):
+    def set_closure(self, mlp, nu_cap=0.05, every=1, smagorinsky_base=False):
         """Attach a trained Mlp (None detaches): the collide then takes its
         eddy viscosity from the network, Smagorinsky outside its range; the
-        network is evaluated every `every` steps."""
+        network is evaluated every `every` steps.  With smagorinsky_base the
+        network's nu_t adds to the Smagorinsky relaxation time of the
+        configured CS instead of to tau0."""
         self._closure = mlp  # keep the network alive while attached
         self._check(self.lib.lbm_set_closure(self.h, None if mlp is None else mlp.h, nu_cap))
         if mlp is not None:
             self._check(self.lib.lbm_set_closure_interval(self.h, every))
+            self._check(self.lib.lbm_set_closure_base(self.h, int(bool(smagorinsky_base))))
+
+    def closure_rates(self):
+        """Per-node collision rate of the last closure step [nz, ny, nx]:
+        > 0 the network's, < 0 minus the Smagorinsky fallback's."""
+        out = np.empty(self.shape(0), np.float32)
+        self._check(self.lib.lbm_get_closure_rates(self.h, out.ctypes.data))
+        return out
 
     def closure_samples(self, seed, keep, max_samples):
         """Features X[n, 7] and eddy-viscosity targets y[n] from this run,

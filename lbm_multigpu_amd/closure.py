@@ -19,6 +19,12 @@ solver's own data:
    the fused running statistics; each is compared with the fine run's mean
    at the coarse cell centres, and the GPU time of each run is measured with
    HIP events.
+4. optionally (calibrate) the network's output scale and the Smagorinsky
+   constant are chosen a posteriori: short coarse runs over a grid of each,
+   scored against the fine mean over the training window (the time of the
+   training snapshots); every model is then compared over the held-out test
+   window (the time of the last 20 % of the snapshots), which neither the
+   training nor the calibration saw.
 
 Every resolution uses the same physical set-up: channel length 4 H, square
 cylinder of side H/8 at x = H, halfway walls, seeded inlet perturbation,
@@ -50,11 +56,12 @@ def channel_at(r, base=(512, 128, 128), tau_fine=0.52, cs_smag=0.1, seed=0):
 
 
 def generate_samples(cfg, spinup, snapshots, every, keep=0.02, seed=1, max_per_snapshot=200000,
-                     stats_from=None, log=print):
+                     stats_from=None, split=None, log=print):
     """Advance the fine run and collect box-filtered training samples.
 
     Returns (X, y, snap_id, solver-state dict with the fine mean velocity
-    when stats_from is given, GPU ms)."""
+    when stats_from is given -- also the mean up to snapshot `split` as
+    "mean_train" -- GPU ms)."""
     s = Solver(cfg)
     s.set_geometry()
     s.init_equilibrium()
@@ -68,11 +75,15 @@ def generate_samples(cfg, spinup, snapshots, every, keep=0.02, seed=1, max_per_s
         Xs.append(X)
         ys.append(y)
         ids.append(np.full(len(y), k, np.int32))
+        if stats_from is not None and split is not None and k + 1 == split:
+            out_train = s.stats()[0]
         if (k + 1) % 20 == 0:
             log(f"  fine snapshot {k + 1}/{snapshots}: {len(y)} samples, t={s.time()}")
     out = {"mask": s.get_mask()}
     if stats_from is not None:
         out["mean"] = s.stats()[0]
+        if split is not None:
+            out["mean_train"] = out_train
     s.close()
     return np.concatenate(Xs), np.concatenate(ys), np.concatenate(ids), out, ms
 
@@ -107,22 +118,33 @@ def fit(X, y, seed=0, steps=3000, batch=4096, lr=2e-3, lr_final=2e-4, log=print)
     return m, norm, np.concatenate(hist)
 
 
-def mean_velocity(cfg, spinup, avg_steps, mlp=None, refresh=1):
-    """Time-averaged velocity of a run after its start-up; returns (mean
-    u[3, nz, ny, nx], mask, GPU ms of the whole run, ms per step).  With a
-    network, it is evaluated every `refresh` steps."""
+def mean_velocity(cfg, spinup, avg_steps, mlp=None, refresh=1, split=None, smag_base=False):
+    """Time-averaged velocity of a run after its start-up; returns (means,
+    mask, GPU ms of the whole run, ms per step), where means holds the mean
+    u[3, nz, ny, nx] over the whole averaging window ("all") and, with
+    split, over its first `split` steps ("train") and the rest ("test").
+    With a network, it is evaluated every `refresh` steps (on top of the
+    Smagorinsky relaxation time with smag_base)."""
     s = Solver(cfg)
     s.set_geometry()
     s.init_equilibrium()
     if mlp is not None:
-        s.set_closure(mlp, every=refresh)
+        s.set_closure(mlp, every=refresh, smagorinsky_base=smag_base)
     s.stats_enable(spinup, 1 << 62)
     ms = s.step_timed(spinup)
-    ms2 = s.step_timed(avg_steps)
-    mean = s.stats()[0]
+    means = {}
+    if split:
+        ms2 = s.step_timed(split)
+        means["train"] = s.stats()[0].astype(np.float64)
+        ms2 += s.step_timed(avg_steps - split)
+    else:
+        ms2 = s.step_timed(avg_steps)
+    means["all"] = s.stats()[0].astype(np.float64)
+    if split:
+        means["test"] = (means["all"] * avg_steps - means["train"] * split) / (avg_steps - split)
     mask = s.get_mask()
     s.close()
-    return mean, mask, ms + ms2, ms2 / avg_steps
+    return means, mask, ms + ms2, ms2 / avg_steps
 
 
 def at_coarse_centres(field, r_src, r_dst):
@@ -151,8 +173,14 @@ def l2_errors(u, u_ref, region, u0):
             "wake_l2": float(np.linalg.norm(d) / np.linalg.norm((u_ref - u0)[region]))}
 
 
+ALPHAS = (0.25, 0.5, 1.0, 2.0, 4.0, 8.0)     # network output scales tried a posteriori
+CS_GRID = (0.03, 0.05, 0.07, 0.1, 0.14, 0.2)  # Smagorinsky constants tried a posteriori
+HYBRID_ALPHAS = (0.5, 1.0, 2.0, 4.0)          # network scales on top of Smagorinsky
+
+
 def run(scale=1.0, spinup_fine=16000, snapshots=400, every=100, keep=0.02, train_steps=3000,
-        resolutions=(0.5625, 0.625, 0.6875, 0.75), seed=0, refresh=(1, 10), tau_fine=0.52, log=print):
+        resolutions=(0.5625, 0.625, 0.6875, 0.75), seed=0, refresh=(1, 10), tau_fine=0.52, calibrate=False,
+        alphas=ALPHAS, cs_grid=CS_GRID, log=print):
     """The full experiment at `scale` times the 512x128x128 fine grid.
 
     Coarse runs (half the fine resolution): the learned closure at each
@@ -160,18 +188,29 @@ def run(scale=1.0, spinup_fine=16000, snapshots=400, every=100, keep=0.02, train
     constant), Smagorinsky with the a-priori best-fit constant, and no
     sub-grid model; then Smagorinsky Cs = 0.1 at each of `resolutions`
     (fractions of the fine grid), the curve against which the learned run's
-    error is priced in GPU time."""
+    error is priced in GPU time.  Every run's error is reported over the
+    whole averaging window ("rel_l2", "wake_l2"), the training window
+    ("train") and the held-out test window ("test").
+
+    With calibrate, the network's output scale alpha (nu_t -> alpha nu_t)
+    and the Smagorinsky constant are each chosen from a grid by the wake
+    error over the training window of coarse runs (the network evaluated
+    every max(refresh) steps there), and the runs "coarse_learned_calibrated"
+    (every step) and "coarse_smagorinsky_calibrated" are added; so is the
+    learned correction to Smagorinsky (the network's nu_t on top of the
+    Smagorinsky relaxation time, "coarse_hybrid_calibrated"), its constant
+    and scale chosen together on the same grid."""
     t0 = time.time()
     base = tuple(int(round(b * scale)) for b in (512, 128, 128))
     fine = channel_at(1.0, base, tau_fine=tau_fine, seed=seed)
     u0 = fine.umax
     fine_avg = snapshots * every
+    # hold out the last 20% of the snapshots (and of the averaging window)
+    split = int(snapshots * 0.8)
     log(f"fine run {base} tau {1 / fine.omega:.4f}: spin-up {spinup_fine}, {snapshots} snapshots every {every}")
     X, y, ids, fout, ms_fine = generate_samples(fine, spinup_fine, snapshots, every, keep, seed + 1,
-                                                stats_from=spinup_fine, log=log)
+                                                stats_from=spinup_fine, split=split, log=log)
     log(f"  {len(y)} samples, nu_t mean {y.mean():.3e}, zero fraction {np.mean(y == 0):.3f}")
-    # hold out the last 20% of the snapshots
-    split = int(snapshots * 0.8)
     tr, te = ids < split, ids >= split
     mlp, norm, hist = fit(X[tr], y[tr], seed, train_steps, log=log)
     pred = mlp.forward(X[te])
@@ -185,35 +224,86 @@ def run(scale=1.0, spinup_fine=16000, snapshots=400, every=100, keep=0.02, train
                "corr_mlp": float(np.corrcoef(pred, y[te])[0, 1]),
                "corr_smag": float(np.corrcoef(smagorinsky_nu(X[te], 1.0), y[te])[0, 1])}
     log(f"  a-priori test MSE: mlp {apriori['mse_mlp']:.3e}, best-fit Smagorinsky {apriori['mse_smag_bestfit']:.3e}")
-    u_ref = at_coarse_centres(fout["mean"][0], 1.0, 0.5)
-    coarse = channel_at(0.5, base, tau_fine=tau_fine, seed=seed)
+    f_all = fout["mean"][0].astype(np.float64)
+    f_train = fout["mean_train"][0].astype(np.float64)
+    f_test = (f_all * snapshots - f_train * split) / (snapshots - split)
+    u_ref = {w: at_coarse_centres(f, 1.0, 0.5) for w, f in (("all", f_all), ("train", f_train), ("test", f_test))}
     # the same physical time as the fine run: half the steps
     sp_c, av_c = spinup_fine // 2, fine_avg // 2
-    runs = {}
     cs_fit = math.sqrt(max(c_fit, 0.0))
+    runs = {}
+    region = None
+
+    def coarse_run(name, r, k, cs, alpha=1.0, smag_base=False):
+        nonlocal region
+        cfg = channel_at(r, base, tau_fine=tau_fine, cs_smag=cs, seed=seed)
+        sp, av = int(round(spinup_fine * r)), int(round(fine_avg * r))
+        if k:
+            mlp.set_norm(norm["mu"], norm["sd"], norm["lo"], norm["hi"], alpha * norm["nu_scale"])
+        means, mask, ms, ms_step = mean_velocity(cfg, sp, av, mlp if k else None, max(k, 1),
+                                                 split=int(round(av * split / snapshots)), smag_base=smag_base)
+        if k:
+            mlp.set_norm(norm["mu"], norm["sd"], norm["lo"], norm["hi"], norm["nu_scale"])
+        if r == 0.5 and region is None:
+            region = eval_region(mask)
+        u_c = {w: (at_coarse_centres(m[0], r, 0.5) if r != 0.5 else m[0]) for w, m in means.items()}
+        rr = {"grid": [cfg.nx, cfg.ny, cfg.nz], "tau": 1 / cfg.omega, "steps": sp + av, "cs": cs,
+              "gpu_ms": ms, "ms_per_step": ms_step}
+        if k:
+            rr["alpha"] = alpha
+            rr["base"] = "smagorinsky" if smag_base else "tau0"
+        rr.update(l2_errors(u_c["all"], u_ref["all"], region, u0))
+        for w in ("train", "test"):
+            rr[w] = l2_errors(u_c[w], u_ref[w], region, u0)
+        runs[name] = rr
+        log(f"  {name}: grid {cfg.nx}x{cfg.ny}x{cfg.nz}, {sp + av} steps, {ms / 1e3:.2f} s GPU, "
+            f"wake L2 train {rr['train']['wake_l2']:.4f} test {rr['test']['wake_l2']:.4f}")
+        return rr
+
     cases = [("coarse_learned" if k == 1 else f"coarse_learned_refresh{k}", 0.5, k, 0.1) for k in refresh]
     cases += [("coarse_smagorinsky", 0.5, 0, 0.1), ("coarse_smagorinsky_bestfit", 0.5, 0, cs_fit),
               ("coarse_no_model", 0.5, 0, 0.0)]
     cases += [(f"smagorinsky_r{r}", r, 0, 0.1) for r in resolutions]
     for name, r, k, cs in cases:
-        cfg = channel_at(r, base, tau_fine=tau_fine, cs_smag=cs, seed=seed)
-        sp, av = int(round(spinup_fine * r)), int(round(fine_avg * r))
-        mean, mask, ms, ms_step = mean_velocity(cfg, sp, av, mlp if k else None, max(k, 1))
-        u_c = at_coarse_centres(mean[0], r, 0.5) if r != 0.5 else mean[0].astype(np.float64)
-        if r == 0.5:
-            region = eval_region(mask)
-        runs[name] = {"grid": [cfg.nx, cfg.ny, cfg.nz], "tau": 1 / cfg.omega, "steps": sp + av, "cs": cs,
-                      "gpu_ms": ms, "ms_per_step": ms_step, "_u": u_c}
-        log(f"  {name}: grid {cfg.nx}x{cfg.ny}x{cfg.nz}, {sp + av} steps, {ms / 1e3:.2f} s GPU")
+        coarse_run(name, r, k, cs)
     assert sp_c + av_c == runs["coarse_smagorinsky"]["steps"]
+    calib = None
+    if calibrate:
+        kc = max(refresh)
+        scores_a = {}
+        for a in alphas:
+            rr = coarse_run(f"calib_alpha{a}", 0.5, kc, 0.1, alpha=a)
+            scores_a[a] = rr["train"]["wake_l2"]
+        scores_c = {}
+        for c in cs_grid:
+            rr = runs.get("coarse_smagorinsky") if c == 0.1 else coarse_run(f"calib_cs{c}", 0.5, 0, c)
+            scores_c[c] = rr["train"]["wake_l2"]
+        scores_h = {}
+        for c in cs_grid:
+            for a in HYBRID_ALPHAS:
+                rr = coarse_run(f"calib_hybrid_cs{c}_alpha{a}", 0.5, kc, c, alpha=a, smag_base=True)
+                scores_h[(c, a)] = rr["train"]["wake_l2"]
+        finite = lambda d: {k: v for k, v in d.items() if math.isfinite(v)}
+        a_best = min(finite(scores_a), key=scores_a.get)
+        c_best = min(finite(scores_c), key=scores_c.get)
+        h_best = min(finite(scores_h), key=scores_h.get)
+        calib = {"alpha": a_best, "cs": c_best, "hybrid_cs": h_best[0], "hybrid_alpha": h_best[1],
+                 "train_wake_l2_by_alpha": scores_a, "train_wake_l2_by_cs": scores_c,
+                 "train_wake_l2_hybrid": {f"{c},{a}": v for (c, a), v in scores_h.items()}}
+        log(f"  calibrated: alpha {a_best}, Cs {c_best}, hybrid Cs {h_best[0]} alpha {h_best[1]}")
+        coarse_run("coarse_learned_calibrated", 0.5, 1, 0.1, alpha=a_best)
+        coarse_run("coarse_smagorinsky_calibrated", 0.5, 0, c_best)
+        coarse_run("coarse_hybrid_calibrated", 0.5, 1, h_best[0], alpha=h_best[1], smag_base=True)
     for name, rr in runs.items():
-        rr.update(l2_errors(rr.pop("_u"), u_ref, region, u0))
-        log(f"  {name}: rel L2 {rr['rel_l2']:.4f}, wake L2 {rr['wake_l2']:.4f}")
+        log(f"  {name}: rel L2 {rr['rel_l2']:.4f}, wake L2 {rr['wake_l2']:.4f}, test wake L2 {rr['test']['wake_l2']:.4f}")
     res = {"fine": {"grid": list(base), "tau": 1 / fine.omega, "steps": spinup_fine + fine_avg, "gpu_ms": ms_fine,
                     "samples": int(len(y))},
            "apriori": apriori, "training": {"steps": train_steps, "loss_first": float(hist[:50].mean()),
                                             "loss_last": float(hist[-50:].mean())},
-           "norm": norm, "runs": runs, "wall_s": time.time() - t0, "eval_cells": int(region.sum())}
+           "norm": norm, "runs": runs, "calibration": calib, "wall_s": time.time() - t0,
+           "eval_cells": int(region.sum()),
+           "windows": {"train_coarse_steps": [sp_c, sp_c + av_c * split // snapshots],
+                       "test_coarse_steps": [sp_c + av_c * split // snapshots, sp_c + av_c]}}
     pts = sorted((v["gpu_ms"], v["wake_l2"]) for k, v in runs.items()
                  if k.startswith("smagorinsky_r") or k == "coarse_smagorinsky")
     target = runs[cases[0][0]]["wake_l2"]
@@ -252,10 +342,12 @@ def main(argv=None):
     ap.add_argument("--refresh", type=int, nargs="+", default=[1, 10],
                     help="network evaluation intervals of the learned coarse runs")
     ap.add_argument("--tau", type=float, default=0.52, help="relaxation time of the fine run")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="choose the network output scale and the Smagorinsky constant a posteriori")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     res = run(a.scale, a.spinup, a.snapshots, a.every, a.keep, a.train_steps, refresh=tuple(a.refresh),
-              tau_fine=a.tau, log=lambda *m: print(*m, flush=True))
+              tau_fine=a.tau, calibrate=a.calibrate, log=lambda *m: print(*m, flush=True))
     txt = json.dumps(res, indent=1)
     if a.out:
         with open(a.out, "w") as fh:

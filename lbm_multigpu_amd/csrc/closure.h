@@ -298,6 +298,7 @@ struct ClosureArgs {
     const float* dwall;    // per node (padded layout)
     MlpNorm nrm;
     float nu_cap;       // upper bound on nu_t (stability)
+    int smag_base;      // 1: nu_t adds to the Smagorinsky relaxation time, 0: to tau0
     // per-node rate field: read first, as the rate the node was last relaxed
     // at (|value|; 0: none yet, the base rate), then overwritten with this
     // step's rate -- the network's (> 0, held by the steps until the next
@@ -369,7 +370,7 @@ __global__ void __launch_bounds__(64 * CLOSURE_NW, 2) k_stream_collide_closure(S
             CollideParams<R> cp = collide_params<R>(p, y);
             if (in_range) {
                 const float nut = fminf(ca.nrm.nu_scale * softplus(zo), ca.nu_cap);
-                const float wr = 1.f / (1.f / p.omega + 3.f * nut);
+                const float wr = 1.f / ((ca.smag_base ? 1.f / float(omega_les) : 1.f / p.omega) + 3.f * nut);
                 cp.omega = R(wr);
                 if (ca.omega_out) *at(ca.omega_out, cell) = wr;
             } else {

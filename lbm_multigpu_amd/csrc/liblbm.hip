@@ -96,6 +96,7 @@ struct lbm_ctx {
     float* dwall = nullptr;    // wall distance per node (padded layout)
     float nu_cap = 0.05f;
     int closure_every = 1;     // network refresh interval (steps)
+    int closure_smag_base = 0; // the network's nu_t adds to the Smagorinsky tau (1) or to tau0 (0)
     float dpdy = 0.f;          // body force as the inlet control leaves it
     std::vector<double> ctrl;  // control history, rows (t, Av_V, dpdy)
     long long closure_t0 = 0;  // step of the first refresh
@@ -872,6 +873,7 @@ static long long launch_closure(lbm_ctx* ctx, const StepParams& p, const Level& 
     ca.dwall = ctx->dwall;
     ca.nrm = ctx->mlp->nrm;
     ca.nu_cap = ctx->nu_cap;
+    ca.smag_base = ctx->closure_smag_base;
     const dim3 blk(64 * CLOSURE_NW), grd(blocks);
     if (ctx->cfg.model == LBM_MODEL_MRT_DHUMIERES)
         hipLaunchKernelGGL((k_stream_collide_closure<MODEL_MRT_DHUM, S>), grd, blk, 0, ctx->stream, p, ca);
@@ -1634,6 +1636,21 @@ extern "C" int lbm_stats_get(lbm_ctx* ctx, int level, float* mean, float* dev)
 
 // ---------------------------------------------------------------- closure
 
+extern "C" int lbm_get_closure_rates(lbm_ctx* ctx, float* rates)
+{
+    if (!ctx || !rates) return fail(ctx, LBM_EINVAL, "lbm_get_closure_rates: null argument");
+    if (!ctx->omega_field) return fail(ctx, LBM_ESTATE, "lbm_get_closure_rates: no closure attached");
+    const Level& L = ctx->lv[0];
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<float> plane((size_t)L.qs);
+    HIP_TRY(ctx, hipMemcpy(plane.data(), ctx->omega_field, plane.size() * 4, hipMemcpyDeviceToHost));
+    for (int z = 0; z < L.nz; ++z)
+        for (int y = 0; y < L.ny; ++y)
+            std::memcpy(rates + (size_t)L.nx * (y + (size_t)L.ny * z), plane.data() + (size_t)L.nxp * (y + (size_t)L.ny * z),
+                        (size_t)L.nx * 4);
+    return LBM_OK;
+}
+
 // Distance (cells) from each node to the nearest solid node or wall face:
 // solid nodes 0, faces of LBM_BC_WALL axes half a cell outside the domain,
 // then a two-pass 3-D chamfer transform (steps 1, sqrt2, sqrt3).
@@ -1931,6 +1948,17 @@ extern "C" int lbm_set_closure_interval(lbm_ctx* ctx, int every)
     }
     ctx->closure_every = every;
     ctx->closure_t0 = ctx->t;
+    return LBM_OK;
+}
+
+// Where the network's eddy viscosity goes: on top of tau0 (the plain
+// learned closure) or on top of the Smagorinsky relaxation time of the
+// configured CS (a learned correction to Smagorinsky).
+extern "C" int lbm_set_closure_base(lbm_ctx* ctx, int smagorinsky)
+{
+    if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_set_closure_base: null context");
+    if (smagorinsky != 0 && smagorinsky != 1) return fail(ctx, LBM_EINVAL, "lbm_set_closure_base: 0 or 1");
+    ctx->closure_smag_base = smagorinsky;
     return LBM_OK;
 }
 

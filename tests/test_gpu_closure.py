@@ -290,6 +290,42 @@ def test_fused_closure_collide_matches_reference():
     assert np.abs(g[:, out] - g_smag[:, out]).max() < 1e-6 * scale
 
 
+def test_closure_on_smagorinsky_base_matches_reference():
+    """lbm_set_closure_base(1): in range, tau = tau_Smagorinsky + 3 nu_t,
+    with tau_Smagorinsky the oracle's tau(Q) law at the configured CS
+    (MRTtest/DynInlet3.cu:722-729) for the node's pre-collision populations;
+    out of range the plain Smagorinsky rate.  One step against the NumPy
+    network and the oracle's collide, and measurably apart from the
+    tau0-based closure."""
+    import ctypes
+    c, f, m, net = _closure_setup(11)
+    c.cs = 0.04  # Cs = 0.2: the two bases well apart on this near-equilibrium state
+    say("closure on the Smagorinsky base")
+    out = {}
+    for base in (True, False):
+        with L.Solver(c) as s:
+            s.set_populations(f)
+            s.set_closure(m, smagorinsky_base=base)
+            s.step(1)
+            out[base] = s.populations()
+    fs = _pre_collision(c, f)
+    inr, _, nut = _reference_rates(c, fs, net)
+    pl = orc_params(c)
+    fr = fs.reshape(19, -1)
+    w = ctypes.c_float()
+    omega = np.zeros(fr.shape[1], np.float32)
+    for n in np.flatnonzero(inr):
+        O.lib().orc_les_omega_f32(ctypes.byref(pl), np.ascontiguousarray(fr[:, n]), ctypes.byref(w), None)
+        omega[n] = np.float32(1) / (np.float32(1) / np.float32(w.value) + np.float32(3) * nut[n])
+    ref = _reference_collide(c, fs, inr, omega)
+    scale = np.abs(ref).max()
+    err = np.abs(out[True] - ref).max() / scale
+    apart = np.abs(out[False] - ref).max() / scale
+    say("  err", err, "vs tau0 base", apart)
+    assert err < 1e-5
+    assert apart > 20 * 1e-5
+
+
 def test_closure_strain_uses_the_last_applied_rate():
     """Pi_neq before a collision scales with the rate the node was last
     relaxed at (tau0 + 3 nu_t), so from the second step on the closure's
