@@ -116,10 +116,11 @@ struct StepParams {
     // fp_a + fp_n - 1 also write their post-collision physical moments and
     // CF data (fp_n[0] = 0: none; footprint_extract)
     int fp_a[3], fp_n[3];
-    const float* fp_mold; // moments at the step's start (temporal midpoint)
-    float* fp_m;          // moments at the step's end [9][footprint]
-    float* fp_pnew;       // CF data of the end state, 16 floats per node
-    float* fp_phalf;      // CF data of the midpoint, or nullptr
+    // (all in the interface's arithmetic type: double for FP64 storage, float otherwise)
+    const void* fp_mold;  // moments at the step's start (temporal midpoint)
+    void* fp_m;           // moments at the step's end [9][footprint]
+    void* fp_pnew;        // CF data of the end state, 16 values per node
+    void* fp_phalf;       // CF data of the midpoint, or nullptr
     int xcd_chunk; // consecutive workgroups per XCD (logical_block)
     int nt_store;  // nontemporal population stores (lattices larger than the MALL)
 };

@@ -85,12 +85,14 @@ struct lbm_ctx {
     bool initialised = false;
     // local refinement
     RefineGeom rg{};
-    float* Mold = nullptr;     // footprint moments at the step's start
-    float* Mnew = nullptr;     // and at its end
+    // interface data, in the interface's arithmetic type (double for FP64
+    // storage, float otherwise)
+    void* Mold = nullptr;      // footprint moments at the step's start
+    void* Mnew = nullptr;      // and at its end
     int fc_nb[3] = {0, 0, 0};  // FC workgroups per axis (FC_TX x FC_TY x FC_TZ coarse nodes each)
-    float* Pold = nullptr;     // CF data (moments + strain rates, 16 floats per node) at the step's start
-    float* Pnew = nullptr;     // at its end (the next step's Pold)
-    float* Phalf = nullptr;    // at its midpoint (temporal interpolation)
+    void* Pold = nullptr;      // CF data (moments + strain rates, 16 values per node) at the step's start
+    void* Pnew = nullptr;      // at its end (the next step's Pold)
+    void* Phalf = nullptr;     // at its midpoint (temporal interpolation)
     // learned closure
     lbm_mlp* mlp = nullptr;    // not owned
     float* dwall = nullptr;    // wall distance per node (padded layout)
@@ -126,8 +128,8 @@ struct lbm_mlp {
 
 static int refine_setup(lbm_ctx* ctx);
 static void classify_fine(lbm_ctx* ctx, bool user_mask);
-static int refine_extract(lbm_ctx* ctx, const float* Mold, float* M, float* Pnew, float* Phalf);
-static int refine_cf(lbm_ctx* ctx, const float* P, hipStream_t stream);
+static int refine_extract(lbm_ctx* ctx, const void* Mold, void* M, void* Pnew, void* Phalf);
+static int refine_cf(lbm_ctx* ctx, const void* P, hipStream_t stream);
 static int refine_fc(lbm_ctx* ctx);
 static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f, double* fpart, hipStream_t stream);
 static int set_geometry_at(lbm_ctx* ctx, const uint8_t* mask, long long t);
@@ -1069,11 +1071,14 @@ static int refine_setup(lbm_ctx* ctx)
     g.cs = c.cs;
     g.eq_kind = c.interface_eq ? EQ_MRT_MEQ : (c.model == LBM_MODEL_BGK ? EQ_BGK_PROJ : EQ_BGK_MEQ);
     const size_t NC = (size_t)g.nc[0] * g.nc[1] * g.nc[2];
-    HIP_TRY(ctx, hipMalloc(&ctx->Mold, 9 * NC * sizeof(float)));
-    HIP_TRY(ctx, hipMalloc(&ctx->Mnew, 9 * NC * sizeof(float)));
-    HIP_TRY(ctx, hipMalloc(&ctx->Pold, 16 * NC * sizeof(float)));
-    HIP_TRY(ctx, hipMalloc(&ctx->Pnew, 16 * NC * sizeof(float)));
-    HIP_TRY(ctx, hipMalloc(&ctx->Phalf, 16 * NC * sizeof(float)));
+    // the interface computes in double for FP64 storage (the FP64 yardstick
+    // end to end), in float otherwise
+    const size_t rsz = c.storage == LBM_STORE_F64 ? sizeof(double) : sizeof(float);
+    HIP_TRY(ctx, hipMalloc(&ctx->Mold, 9 * NC * rsz));
+    HIP_TRY(ctx, hipMalloc(&ctx->Mnew, 9 * NC * rsz));
+    HIP_TRY(ctx, hipMalloc(&ctx->Pold, 16 * NC * rsz));
+    HIP_TRY(ctx, hipMalloc(&ctx->Pnew, 16 * NC * rsz));
+    HIP_TRY(ctx, hipMalloc(&ctx->Phalf, 16 * NC * rsz));
     return LBM_OK;
 }
 
@@ -1103,14 +1108,15 @@ static void classify_fine(lbm_ctx* ctx, bool user_mask)
             }
 }
 
-static int refine_extract(lbm_ctx* ctx, const float* Mold, float* M, float* Pnew, float* Phalf)
+static int refine_extract(lbm_ctx* ctx, const void* Mold, void* M, void* Pnew, void* Phalf)
 {
     Level& L = ctx->lv[0];
     const size_t NC = (size_t)ctx->rg.nc[0] * ctx->rg.nc[1] * ctx->rg.nc[2];
     with_storage(L.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
+        using R = typename Arith<S>::R;
         hipLaunchKernelGGL(k_extract<S>, dim3((unsigned)((NC + 255) / 256)), dim3(256), 0, ctx->stream,
-                           (const S*)L.fa, L.qs, L.nxp, L.ny, ctx->rg, Mold, M, (f32x4v*)Pnew, (f32x4v*)Phalf,
+                           (const S*)L.fa, L.qs, L.nxp, L.ny, ctx->rg, (const R*)Mold, (R*)M, (R*)Pnew, (R*)Phalf,
                            L.shift);
         return 0;
     });
@@ -1118,14 +1124,15 @@ static int refine_extract(lbm_ctx* ctx, const float* Mold, float* M, float* Pnew
     return LBM_OK;
 }
 
-static int refine_cf(lbm_ctx* ctx, const float* P, hipStream_t stream)
+static int refine_cf(lbm_ctx* ctx, const void* P, hipStream_t stream)
 {
     Level& F = ctx->lv[1];
     with_storage(F.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
+        using R = typename Arith<S>::R;
         const long long n = frame_nodes(ctx->rg.nf);
         hipLaunchKernelGGL(k_interp_cf<S>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, (S*)F.fa,
-                           F.qs, F.nxp, F.mask, (const f32x4v*)P, ctx->rg, F.shift);
+                           F.qs, F.nxp, F.mask, (const R*)P, ctx->rg, F.shift);
         return 0;
     });
     HIP_TRY(ctx, hipGetLastError());

@@ -57,66 +57,70 @@ __device__ __forceinline__ void phys_moments(const R f[Q], R mom[9])
 }
 
 // StrainRate (:367-393) as coded: omega = 1, m1 = 0.
-__device__ __forceinline__ void strain_rate(const float m[9], float S[6])
+template <class R>
+__device__ __forceinline__ void strain_rate(const R m[9], R S[6])
 {
-    const float u = m[1], v = m[2], w = m[3];
-    const float m9 = m[4] - (2.f * u * u - (v * v + w * w));
-    const float m11 = m[5] - (v * v - w * w);
-    S[0] = -0.026315789f * (19.f * m9);
-    S[1] = -0.013157895f * (-19.f * (m9 - 3.f * m11));
-    S[2] = -0.013157895f * (-19.f * (m9 + 3.f * m11));
-    S[3] = -1.5f * (m[6] - u * v);
-    S[4] = -1.5f * (m[7] - v * w);
-    S[5] = -1.5f * (m[8] - u * w);
+    const R u = m[1], v = m[2], w = m[3];
+    const R m9 = m[4] - (R(2.) * u * u - (v * v + w * w));
+    const R m11 = m[5] - (v * v - w * w);
+    S[0] = -R(0.026315789) * (R(19.) * m9);
+    S[1] = -R(0.013157895) * (-R(19.) * (m9 - R(3.) * m11));
+    S[2] = -R(0.013157895) * (-R(19.) * (m9 + R(3.) * m11));
+    S[3] = -R(1.5) * (m[6] - u * v);
+    S[4] = -R(1.5) * (m[7] - v * w);
+    S[5] = -R(1.5) * (m[8] - u * w);
 }
 
-__device__ __forceinline__ float smag_of(const float S[6])
+template <class R>
+__device__ __forceinline__ R smag_of(const R S[6])
 {
-    return sqrtf(2.f * (S[0] * S[0] + S[1] * S[1] + S[2] * S[2] + 2.f * S[3] * S[3] + 2.f * S[4] * S[4] +
-                        2.f * S[5] * S[5]));
+    return sqrt(R(2.) * (S[0] * S[0] + S[1] * S[1] + S[2] * S[2] + R(2.) * S[3] * S[3] + R(2.) * S[4] * S[4] +
+                        R(2.) * S[5] * S[5]));
 }
 
 // Populations from moments: conserved and non-equilibrium-scaled second
 // moments; every other moment at the equilibrium selected by eq_kind.
-__device__ __forceinline__ void invert_phys(float f[Q], const float mom[9], float SF, int eq_kind)
+template <class R>
+__device__ __forceinline__ void invert_phys(R f[Q], const R mom[9], R SF, int eq_kind)
 {
-    float m[Q];
+    R m[Q];
 #pragma unroll
-    for (int k = 0; k < Q; ++k) m[k] = 0.f;
-    const float u = mom[1], v = mom[2], w = mom[3];
-    const float usq = u * u + v * v + w * w;
+    for (int k = 0; k < Q; ++k) m[k] = R(0.);
+    const R u = mom[1], v = mom[2], w = mom[3];
+    const R usq = u * u + v * v + w * w;
     m[0] = mom[0];
-    m[1] = -11.f * mom[0] + 19.f * usq;
+    m[1] = -R(11.) * mom[0] + R(19.) * usq;
     if (eq_kind == EQ_MRT_MEQ) {
-        m[2] = 7.53968254f * usq;
+        m[2] = R(7.53968254) * usq;
     } else {
-        const float m9eq = 2.f * u * u - (v * v + w * w), m11eq = v * v - w * w;
-        m[2] = 3.f * mom[0] - 5.5f * usq;
-        m[10] = eq_kind == EQ_BGK_MEQ ? -0.5f * m9eq * (1.0f / 3.0f) : -0.5f * m9eq;
-        m[12] = -0.5f * m11eq;
+        const R m9eq = R(2.) * u * u - (v * v + w * w), m11eq = v * v - w * w;
+        m[2] = R(3.) * mom[0] - R(5.5) * usq;
+        m[10] = eq_kind == EQ_BGK_MEQ ? -R(0.5) * m9eq * (R(1.0) / R(3.0)) : -R(0.5) * m9eq;
+        m[12] = -R(0.5) * m11eq;
     }
-    m[3] = u; m[4] = -0.666666667f * u;
-    m[5] = v; m[6] = -0.666666667f * v;
-    m[7] = w; m[8] = -0.666666667f * w;
-    m[9] = mom[4] * SF + (1.f - SF) * (2.f * u * u - (v * v + w * w));
-    m[11] = mom[5] * SF + (1.f - SF) * (v * v - w * w);
-    m[13] = mom[6] * SF + (1.f - SF) * u * v;
-    m[14] = mom[7] * SF + (1.f - SF) * v * w;
-    m[15] = mom[8] * SF + (1.f - SF) * u * w;
+    m[3] = u; m[4] = -R(0.666666667) * u;
+    m[5] = v; m[6] = -R(0.666666667) * v;
+    m[7] = w; m[8] = -R(0.666666667) * w;
+    m[9] = mom[4] * SF + (R(1.) - SF) * (R(2.) * u * u - (v * v + w * w));
+    m[11] = mom[5] * SF + (R(1.) - SF) * (v * v - w * w);
+    m[13] = mom[6] * SF + (R(1.) - SF) * u * v;
+    m[14] = mom[7] * SF + (R(1.) - SF) * v * w;
+    m[15] = mom[8] * SF + (R(1.) - SF) * u * w;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        float acc = 0.f;
+        R acc = R(0.);
 #pragma unroll
         for (int k = 0; k < Q; ++k)
-            if (mrow(k, i) != 0) acc += (float(mrow(k, i)) / float(NORM[k])) * m[k];
+            if (mrow(k, i) != 0) acc += (R(mrow(k, i)) / R(NORM[k])) * m[k];
         f[i] = acc;
     }
 }
 
 // ORDER-2 correction (:2312-2357).  Corners c = 0..7 are (x,y,z) bits of c;
 // fc_form = 1 gives the InterpFC weights (xpr = ypr = zpr = 0).
-__device__ __forceinline__ void order2(const float (*Sc)[6], const float (*uc)[3], float mom[9], float xf, float yf,
-                                       float zf, bool fc_form)
+template <class R>
+__device__ __forceinline__ void order2(const R (*Sc)[6], const R (*uc)[3], R mom[9], R xf, R yf, R zf,
+                                       bool fc_form)
 {
 #define SXX(k) Sc[(k)-1][0]
 #define SYY(k) Sc[(k)-1][1]
@@ -127,21 +131,21 @@ __device__ __forceinline__ void order2(const float (*Sc)[6], const float (*uc)[3
 #define U(k) uc[(k)-1][0]
 #define V(k) uc[(k)-1][1]
 #define W(k) uc[(k)-1][2]
-    const float cx = -((SXX(8) - SXX(7) + SXX(6) - SXX(5) + SXX(4) - SXX(3) + SXX(2) - SXX(1))) * 0.03125f;
-    const float cy = -((SXY(8) + SXY(7) - SXY(6) - SXY(5) + SXY(4) + SXY(3) - SXY(2) - SXY(1)) - V(8) + V(7) +
-                       V(6) - V(5) - V(4) + V(3) + V(2) - V(1)) * 0.0625f;
-    const float cz = -((SXZ(8) + SXZ(7) + SXZ(6) + SXZ(5) - SXZ(4) - SXZ(3) - SXZ(2) - SXZ(1)) - W(8) + W(7) -
-                       W(6) + W(5) + W(4) - W(3) + W(2) - W(1)) * 0.0625f;
-    const float dx = -((SXY(8) - SXY(7) + SXY(6) - SXY(5) + SXY(4) - SXY(3) + SXY(2) - SXY(1)) - U(8) + U(7) +
-                       U(6) - U(5) - U(4) + U(3) + U(2) - U(1)) * 0.0625f;
-    const float dy = -((SYY(8) + SYY(7) - SYY(6) - SYY(5) + SYY(4) + SYY(3) - SYY(2) - SYY(1))) * 0.03125f;
-    const float dz = -((SYZ(8) + SYZ(7) + SYZ(6) + SYZ(5) - SYZ(4) - SYZ(3) - SYZ(2) - SYZ(1)) - W(8) - W(7) +
-                       W(6) + W(5) + W(4) + W(3) - W(2) - W(1)) * 0.0625f;
-    const float ex = -((SXZ(8) - SXZ(7) + SXZ(6) - SXZ(5) + SXZ(4) - SXZ(3) + SXZ(2) - SXZ(1)) - U(8) + U(7) -
-                       U(6) + U(5) + U(4) - U(3) + U(2) - U(1)) * 0.0625f;
-    const float ey = -((SYZ(8) + SYZ(7) - SYZ(6) - SYZ(5) + SYZ(4) + SYZ(3) - SYZ(2) - SYZ(1)) - V(8) - V(7) +
-                       V(6) + V(5) + V(4) + V(3) - V(2) - V(1)) * 0.0625f;
-    const float ez = -((SZZ(8) + SZZ(7) + SZZ(6) + SZZ(5) - SZZ(4) - SZZ(3) - SZZ(2) - SZZ(1))) * 0.03125f;
+    const R cx = -((SXX(8) - SXX(7) + SXX(6) - SXX(5) + SXX(4) - SXX(3) + SXX(2) - SXX(1))) * R(0.03125);
+    const R cy = -((SXY(8) + SXY(7) - SXY(6) - SXY(5) + SXY(4) + SXY(3) - SXY(2) - SXY(1)) - V(8) + V(7) +
+                       V(6) - V(5) - V(4) + V(3) + V(2) - V(1)) * R(0.0625);
+    const R cz = -((SXZ(8) + SXZ(7) + SXZ(6) + SXZ(5) - SXZ(4) - SXZ(3) - SXZ(2) - SXZ(1)) - W(8) + W(7) -
+                       W(6) + W(5) + W(4) - W(3) + W(2) - W(1)) * R(0.0625);
+    const R dx = -((SXY(8) - SXY(7) + SXY(6) - SXY(5) + SXY(4) - SXY(3) + SXY(2) - SXY(1)) - U(8) + U(7) +
+                       U(6) - U(5) - U(4) + U(3) + U(2) - U(1)) * R(0.0625);
+    const R dy = -((SYY(8) + SYY(7) - SYY(6) - SYY(5) + SYY(4) + SYY(3) - SYY(2) - SYY(1))) * R(0.03125);
+    const R dz = -((SYZ(8) + SYZ(7) + SYZ(6) + SYZ(5) - SYZ(4) - SYZ(3) - SYZ(2) - SYZ(1)) - W(8) - W(7) +
+                       W(6) + W(5) + W(4) + W(3) - W(2) - W(1)) * R(0.0625);
+    const R ex = -((SXZ(8) - SXZ(7) + SXZ(6) - SXZ(5) + SXZ(4) - SXZ(3) + SXZ(2) - SXZ(1)) - U(8) + U(7) -
+                       U(6) + U(5) + U(4) - U(3) + U(2) - U(1)) * R(0.0625);
+    const R ey = -((SYZ(8) + SYZ(7) - SYZ(6) - SYZ(5) + SYZ(4) + SYZ(3) - SYZ(2) - SYZ(1)) - V(8) - V(7) +
+                       V(6) + V(5) + V(4) + V(3) - V(2) - V(1)) * R(0.0625);
+    const R ez = -((SZZ(8) + SZZ(7) + SZZ(6) + SZZ(5) - SZZ(4) - SZZ(3) - SZZ(2) - SZZ(1))) * R(0.03125);
 #undef SXX
 #undef SYY
 #undef SZZ
@@ -151,30 +155,56 @@ __device__ __forceinline__ void order2(const float (*Sc)[6], const float (*uc)[3
 #undef U
 #undef V
 #undef W
-    float wx = 1.f, wy = 1.f, wz = 1.f;
+    R wx = R(1.), wy = R(1.), wz = R(1.);
     if (!fc_form) {
-        wx = 1.f - (4.f * xf * xf - 4.f * xf + 1.f);
-        wy = 1.f - (4.f * yf * yf - 4.f * yf + 1.f);
-        wz = 1.f - (4.f * zf * zf - 4.f * zf + 1.f);
+        wx = R(1.) - (R(4.) * xf * xf - R(4.) * xf + R(1.));
+        wy = R(1.) - (R(4.) * yf * yf - R(4.) * yf + R(1.));
+        wz = R(1.) - (R(4.) * zf * zf - R(4.) * zf + R(1.));
     }
     mom[1] += cx * wx + cy * wy + cz * wz;
     mom[2] += dx * wx + dy * wy + dz * wz;
     mom[3] += ex * wx + ey * wy + ez * wz;
 }
 
-typedef float f32x4v __attribute__((ext_vector_type(4)));
-
 // Footprint node data for one CF pass: 9 moments and their 6 strain rates,
-// 16 floats per node (AoS, so a CF thread fetches a corner with four 16-byte
-// loads).
-__device__ __forceinline__ void cf_pack(const float m[9], f32x4v* P)
+// 16 values per node (AoS, so a CF thread fetches a corner with 16-byte
+// loads: four for FP32 interface data, eight for FP64), in the interface's
+// arithmetic type R (double for FP64 storage, float otherwise).
+template <class R> struct CfVec;
+template <> struct CfVec<float> { typedef float V __attribute__((ext_vector_type(4))); static constexpr int N = 4; };
+template <> struct CfVec<double> { typedef double V __attribute__((ext_vector_type(2))); static constexpr int N = 2; };
+
+template <class R>
+__device__ __forceinline__ void cf_pack(const R m[9], R* P)
 {
-    float Sv[6];
+    R Sv[6];
     strain_rate(m, Sv);
-    P[0] = f32x4v{m[0], m[1], m[2], m[3]};
-    P[1] = f32x4v{m[4], m[5], m[6], m[7]};
-    P[2] = f32x4v{m[8], Sv[0], Sv[1], Sv[2]};
-    P[3] = f32x4v{Sv[3], Sv[4], Sv[5], 0.f};
+    const R v[16] = {m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8], Sv[0], Sv[1], Sv[2], Sv[3], Sv[4], Sv[5], R(0)};
+    typedef typename CfVec<R>::V V;
+    constexpr int N = CfVec<R>::N;
+    V* pv = reinterpret_cast<V*>(P);
+#pragma unroll
+    for (int k = 0; k < 16 / N; ++k) {
+        V x;
+#pragma unroll
+        for (int e = 0; e < N; ++e) x[e] = v[k * N + e];
+        pv[k] = x;
+    }
+}
+
+// The 16 values of one footprint node.
+template <class R>
+__device__ __forceinline__ void cf_load(const R* P, R out[16])
+{
+    typedef typename CfVec<R>::V V;
+    constexpr int N = CfVec<R>::N;
+    const V* pv = reinterpret_cast<const V*>(P);
+#pragma unroll
+    for (int k = 0; k < 16 / N; ++k) {
+        const V x = pv[k];
+#pragma unroll
+        for (int e = 0; e < N; ++e) out[k * N + e] = x[e];
+    }
 }
 
 // Footprint moments of the coarse lattice at the step's end (one thread per
@@ -184,28 +214,30 @@ __device__ __forceinline__ void cf_pack(const float m[9], f32x4v* P)
 // this step, temporal interpolation; skipped when Phalf is null).
 template <class S>
 __global__ void __launch_bounds__(256) k_extract(const S* fc, long long qs, int nxp, int ny, RefineGeom g,
-                                                 const float* Mold, float* M, f32x4v* Pnew, f32x4v* Phalf,
+                                                 const typename Arith<S>::R* Mold, typename Arith<S>::R* M,
+                                                 typename Arith<S>::R* Pnew, typename Arith<S>::R* Phalf,
                                                  PopShift sh)
 {
+    using R = typename Arith<S>::R;
     const int ncx = g.nc[0], ncy = g.nc[1], ncz = g.nc[2];
     const long long NC = (long long)ncx * ncy * ncz;
     const long long o = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (o >= NC) return;
     const int i = int(o % ncx), j = int((o / ncx) % ncy), k = int(o / ((long long)ncx * ncy));
     const int cell = (g.a[0] + i) + nxp * ((g.a[1] + j) + ny * (g.a[2] + k));
-    float f[Q];
+    R f[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f[q] = float(Store<S>::to_real(fc[q * qs + cell], q, sh));
-    float mom[9];
+    for (int q = 0; q < Q; ++q) f[q] = R(Store<S>::to_real(fc[q * qs + cell], q, sh));
+    R mom[9];
     phys_moments(f, mom);
 #pragma unroll
     for (int m = 0; m < 9; ++m) M[m * NC + o] = mom[m];
-    cf_pack(mom, Pnew + 4 * o);
+    cf_pack(mom, Pnew + 16 * o);
     if (Phalf) {
-        float mid[9];
+        R mid[9];
 #pragma unroll
-        for (int m = 0; m < 9; ++m) mid[m] = (1.f - 0.5f) * Mold[m * NC + o] + 0.5f * mom[m];
-        cf_pack(mid, Phalf + 4 * o);
+        for (int m = 0; m < 9; ++m) mid[m] = (R(1) - R(0.5)) * Mold[m * NC + o] + R(0.5) * mom[m];
+        cf_pack(mid, Phalf + 16 * o);
     }
 }
 
@@ -222,25 +254,27 @@ __device__ __forceinline__ void footprint_extract(const StepParams& p, int x, in
         return;
     const long long NC = (long long)p.fp_n[0] * p.fp_n[1] * p.fp_n[2];
     const long long o = i + (long long)p.fp_n[0] * (j + (long long)p.fp_n[1] * k);
-    float g[Q];
+    R g[Q];
     if (act == ACT_SKIP) {
         const S* fo = static_cast<const S*>(p.fout);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) g[q] = float(Store<S>::to_real(fo[q * p.qs + cell], q, p.sh));
+        for (int q = 0; q < Q; ++q) g[q] = R(Store<S>::to_real(fo[q * p.qs + cell], q, p.sh));
     } else {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) g[q] = float(Store<S>::stored(f[q], q, p.sh));
+        for (int q = 0; q < Q; ++q) g[q] = R(Store<S>::stored(f[q], q, p.sh));
     }
-    float mom[9];
+    R mom[9];
     phys_moments(g, mom);
+    R* M = static_cast<R*>(p.fp_m);
 #pragma unroll
-    for (int m = 0; m < 9; ++m) p.fp_m[m * NC + o] = mom[m];
-    cf_pack(mom, reinterpret_cast<f32x4v*>(p.fp_pnew) + 4 * o);
+    for (int m = 0; m < 9; ++m) M[m * NC + o] = mom[m];
+    cf_pack(mom, static_cast<R*>(p.fp_pnew) + 16 * o);
     if (p.fp_phalf) {
-        float mid[9];
+        const R* Mold = static_cast<const R*>(p.fp_mold);
+        R mid[9];
 #pragma unroll
-        for (int m = 0; m < 9; ++m) mid[m] = (1.f - 0.5f) * p.fp_mold[m * NC + o] + 0.5f * mom[m];
-        cf_pack(mid, reinterpret_cast<f32x4v*>(p.fp_phalf) + 4 * o);
+        for (int m = 0; m < 9; ++m) mid[m] = (R(1) - R(0.5)) * Mold[m * NC + o] + R(0.5) * mom[m];
+        cf_pack(mid, static_cast<R*>(p.fp_phalf) + 16 * o);
     }
 }
 
@@ -292,8 +326,9 @@ __host__ __device__ inline long long frame_nodes(const int nf[3])
 // correction and the LES-aware rescale.
 template <class S>
 __global__ void __launch_bounds__(256) k_interp_cf(S* ff, long long fqs, int fnxp, const unsigned char* fmask,
-                                                   const f32x4v* P, RefineGeom g, PopShift sh)
+                                                   const typename Arith<S>::R* P, RefineGeom g, PopShift sh)
 {
+    using R = typename Arith<S>::R;
     int i, j, k;
     if (!frame_node(blockIdx.x * 256LL + threadIdx.x, g, i, j, k)) return;
     const int nfy = g.nf[1];
@@ -301,45 +336,38 @@ __global__ void __launch_bounds__(256) k_interp_cf(S* ff, long long fqs, int fnx
     if (fmask && is_solid(fmask[fcell])) return;
     const int ncx = g.nc[0], ncy = g.nc[1];
     const int lx = i >> 1, ly = j >> 1, lz = k >> 1;
-    const float xf = (i & 1) ? 0.75f : 0.25f, yf = (j & 1) ? 0.75f : 0.25f, zf = (k & 1) ? 0.75f : 0.25f;
-    f32x4v pc[8][4];
+    const R xf = (i & 1) ? R(0.75) : R(0.25), yf = (j & 1) ? R(0.75) : R(0.25), zf = (k & 1) ? R(0.75) : R(0.25);
+    R Sc[8][6], uc[8][3], mom[9];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) mom[m] = R(0);
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const int dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
         const long long o = (lx + dx) + (long long)ncx * ((ly + dy) + (long long)ncy * (lz + dz));
+        R v[16]; // 9 moments, then the 6 strain rates
+        cf_load(P + 16 * o, v);
+        const R w = (dx ? xf : R(1) - xf) * (dy ? yf : R(1) - yf) * (dz ? zf : R(1) - zf);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) pc[c][v] = P[4 * o + v];
-    }
-    float Sc[8][6], uc[8][3], mom[9];
+        for (int q = 0; q < 9; ++q) mom[q] += v[q] * w;
 #pragma unroll
-    for (int m = 0; m < 9; ++m) mom[m] = 0.f;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const int dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
-        const float m[9] = {pc[c][0][0], pc[c][0][1], pc[c][0][2], pc[c][0][3], pc[c][1][0],
-                            pc[c][1][1], pc[c][1][2], pc[c][1][3], pc[c][2][0]};
-        const float w = (dx ? xf : 1.f - xf) * (dy ? yf : 1.f - yf) * (dz ? zf : 1.f - zf);
-#pragma unroll
-        for (int q = 0; q < 9; ++q) mom[q] += m[q] * w;
-        Sc[c][0] = pc[c][2][1]; Sc[c][1] = pc[c][2][2]; Sc[c][2] = pc[c][2][3];
-        Sc[c][3] = pc[c][3][0]; Sc[c][4] = pc[c][3][1]; Sc[c][5] = pc[c][3][2];
-        uc[c][0] = m[1]; uc[c][1] = m[2]; uc[c][2] = m[3];
+        for (int q = 0; q < 6; ++q) Sc[c][q] = v[9 + q];
+        uc[c][0] = v[1]; uc[c][1] = v[2]; uc[c][2] = v[3];
     }
     order2(Sc, uc, mom, xf, yf, zf, false);
     // InvertPhysicalMoments_LES_cf (:540-595)
-    float Sv[6];
+    R Sv[6];
     strain_rate(mom, Sv);
-    const float Smag_c = smag_of(Sv);
-    float tau_c = 1.f / g.omega_c;
-    const float tau_f = 2.f * tau_c - 0.5f + 1.5f * Smag_c * g.cs;
-    const float omega_f = 1.f / tau_f;
-    tau_c = tau_c + Smag_c * g.cs;
-    const float omega_c = 1.f / tau_c;
-    const float SF = (0.5f * omega_c * (1.f - omega_f)) / ((1.f - omega_c) * omega_f);
-    float f[Q];
+    const R Smag_c = smag_of(Sv);
+    R tau_c = R(1) / R(g.omega_c);
+    const R tau_f = R(2) * tau_c - R(0.5) + R(1.5) * Smag_c * R(g.cs);
+    const R omega_f = R(1) / tau_f;
+    tau_c = tau_c + Smag_c * R(g.cs);
+    const R omega_c = R(1) / tau_c;
+    const R SF = (R(0.5) * omega_c * (R(1) - omega_f)) / ((R(1) - omega_c) * omega_f);
+    R f[Q];
     invert_phys(f, mom, SF, g.eq_kind);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) ff[q * fqs + fcell] = Store<S>::from_real(typename Arith<S>::R(f[q]), q, sh);
+    for (int q = 0; q < Q; ++q) ff[q * fqs + fcell] = Store<S>::from_real(f[q], q, sh);
 }
 
 // Fine -> coarse.  One workgroup per tile of FC_TX x FC_TY x FC_TZ coarse
@@ -361,7 +389,8 @@ __global__ void __launch_bounds__(256) k_interp_fc(const S* ff, long long fqs, i
                                                    int cnxp, int cny, const unsigned char* cmask, RefineGeom g,
                                                    int nbx, int nby, PopShift sh)
 {
-    __shared__ float fm[2 * FC_TZ][2 * FC_TY][2 * FC_TX][9]; // fine moments [fz][fy][fx][m]
+    using R = typename Arith<S>::R;
+    __shared__ R fm[2 * FC_TZ][2 * FC_TY][2 * FC_TX][9]; // fine moments [fz][fy][fx][m]
     const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
     // coarse nodes a + 2 + (FC_T* b) + l; fine nodes 2(c - a) - 1 .. 2(c - a)
     const int fx0 = 2 * (2 + FC_TX * bx) - 1, fy0 = 2 * (2 + FC_TY * by) - 1, fz0 = 2 * (2 + FC_TZ * bz) - 1;
@@ -371,14 +400,14 @@ __global__ void __launch_bounds__(256) k_interp_fc(const S* ff, long long fqs, i
         const int row = row0 + 4 * rr;          // (fy, fz) of the tile
         const int ly = row % (2 * FC_TY), lz = row / (2 * FC_TY);
         const int i = fx0 + lxf, j = fy0 + ly, k = fz0 + lz;
-        float mom[9];
+        R mom[9];
 #pragma unroll
-        for (int m = 0; m < 9; ++m) mom[m] = 0.f;
+        for (int m = 0; m < 9; ++m) mom[m] = R(0);
         if (i < g.nf[0] && j < g.nf[1] && k < g.nf[2]) {
             const int fcell = i + fnxp * (j + g.nf[1] * k);
-            float f[Q];
+            R f[Q];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) f[q] = float(Store<S>::to_real(ff[q * fqs + fcell], q, sh));
+            for (int q = 0; q < Q; ++q) f[q] = R(Store<S>::to_real(ff[q * fqs + fcell], q, sh));
             phys_moments(f, mom);
         }
 #pragma unroll
@@ -392,13 +421,13 @@ __global__ void __launch_bounds__(256) k_interp_fc(const S* ff, long long fqs, i
     if (cx > g.a[0] + g.nf[0] / 2 - 2 || cy > g.a[1] + g.nf[1] / 2 - 2 || cz > g.a[2] + g.nf[2] / 2 - 2) return;
     const int ccell = cx + cnxp * (cy + cny * cz);
     if (cmask && is_solid(cmask[ccell])) return;
-    float Sc[8][6], uc[8][3], avg[9];
+    R Sc[8][6], uc[8][3], avg[9];
 #pragma unroll
-    for (int m = 0; m < 9; ++m) avg[m] = 0.f;
+    for (int m = 0; m < 9; ++m) avg[m] = R(0);
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const int dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
-        float m[9];
+        R m[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) m[q] = fm[2 * lz + dz][2 * ly + dy][2 * lx + dx][q];
 #pragma unroll
@@ -407,23 +436,23 @@ __global__ void __launch_bounds__(256) k_interp_fc(const S* ff, long long fqs, i
         uc[c][0] = m[1]; uc[c][1] = m[2]; uc[c][2] = m[3];
     }
 #pragma unroll
-    for (int q = 0; q < 9; ++q) avg[q] *= 0.125f;
-    order2(Sc, uc, avg, 0.f, 0.f, 0.f, true);
+    for (int q = 0; q < 9; ++q) avg[q] *= R(0.125);
+    order2(Sc, uc, avg, R(0), R(0), R(0), true);
     // InvertPhysicalMoments_LES_fc (:483-538)
-    float Sv[6];
+    R Sv[6];
     strain_rate(avg, Sv);
-    const float Smag_f = smag_of(Sv);
-    float tau_f = 1.f / g.omega_f;
-    float tau_c = tau_f + 0.5f + 12.f * Smag_f * g.cs;
-    tau_c *= 0.5f;
-    const float omega_c = 1.f / tau_c;
-    tau_f = tau_f + Smag_f * g.cs;
-    const float omega_f = 1.f / tau_f;
-    const float SF = (1.f - omega_c) * omega_f / (0.5f * omega_c * (1.f - omega_f));
-    float f[Q];
+    const R Smag_f = smag_of(Sv);
+    R tau_f = R(1) / R(g.omega_f);
+    R tau_c = tau_f + R(0.5) + R(12) * Smag_f * R(g.cs);
+    tau_c *= R(0.5);
+    const R omega_c = R(1) / tau_c;
+    tau_f = tau_f + Smag_f * R(g.cs);
+    const R omega_f = R(1) / tau_f;
+    const R SF = (R(1) - omega_c) * omega_f / (R(0.5) * omega_c * (R(1) - omega_f));
+    R f[Q];
     invert_phys(f, avg, SF, g.eq_kind);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) fc[q * cqs + ccell] = Store<S>::from_real(typename Arith<S>::R(f[q]), q, sh);
+    for (int q = 0; q < Q; ++q) fc[q * cqs + ccell] = Store<S>::from_real(f[q], q, sh);
 }
 
 } // namespace lbm

@@ -171,6 +171,22 @@ void orc_refine_step_f32(const orc_params* pc, const orc_params* pf, const orc_r
                          const uint8_t* cmask, const uint8_t* fmask, const float* u_in,
                          float* ca, float* cb, float* fa, float* fb, float* Mold, float* Mnew,
                          double* fine_force);
+/* the same interface in double (the FP64 yardstick of the refined path) */
+void orc_phys_moments_f64(const double f[19], double mom[9]);
+void orc_strain_rate_f64(const double mom[9], double S[6]);
+void orc_invert_phys_les_cf_f64(double f[19], const double mom[9], double omega_c, double cs, int eq_kind);
+void orc_invert_phys_les_fc_f64(double f[19], const double mom[9], double omega_f, double cs, int eq_kind);
+void orc_order2_correction_f64(const double corner_mom[8][9], double mom[9], double xf, double yf, double zf,
+                               int fc_form);
+void orc_extract_f64(const orc_params* pc, const orc_refine* r, const double* fc, double* M);
+void orc_interp_cf_f64(const orc_refine* r, const double* Mold, const double* Mnew, double alpha,
+                       const uint8_t* fmask, double* ff);
+void orc_interp_fc_f64(const orc_params* pc, const orc_refine* r, const double* ff, const uint8_t* cmask,
+                       double* fc);
+void orc_refine_step_f64(const orc_params* pc, const orc_params* pf, const orc_refine* r,
+                         const uint8_t* cmask, const uint8_t* fmask, const double* u_in,
+                         double* ca, double* cb, double* fa, double* fb, double* Mold, double* Mnew,
+                         double* fine_force);
 
 #ifdef __cplusplus
 }

@@ -210,20 +210,26 @@ def _bind_refine():
     L = lib()
     if getattr(L, "_refine_bound", False):
         return L
-    f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
-    L.orc_phys_moments_f32.argtypes = [f32p, f32p]
-    L.orc_strain_rate_f32.argtypes = [f32p, f32p]
-    L.orc_invert_phys_les_cf_f32.argtypes = [f32p, f32p, C.c_float, C.c_float, C.c_int]
-    L.orc_invert_phys_les_fc_f32.argtypes = [f32p, f32p, C.c_float, C.c_float, C.c_int]
-    L.orc_order2_correction_f32.argtypes = [f32p, f32p, C.c_float, C.c_float, C.c_float, C.c_int]
-    L.orc_extract_f32.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcRefine), f32p, f32p]
-    L.orc_interp_cf_f32.argtypes = [C.POINTER(OrcRefine), f32p, f32p, C.c_float, C.c_void_p, f32p]
-    L.orc_interp_fc_f32.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcRefine), f32p, C.c_void_p, f32p]
-    L.orc_refine_step_f32.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcParams), C.POINTER(OrcRefine),
-                                      C.c_void_p, C.c_void_p, C.c_void_p, f32p, f32p, f32p, f32p, f32p, f32p,
-                                      C.c_void_p]
+    for suf, dt, cr in (("f32", np.float32, C.c_float), ("f64", np.float64, C.c_double)):
+        fp = np.ctypeslib.ndpointer(dt, flags="C_CONTIGUOUS")
+        getattr(L, "orc_phys_moments_" + suf).argtypes = [fp, fp]
+        getattr(L, "orc_strain_rate_" + suf).argtypes = [fp, fp]
+        getattr(L, "orc_invert_phys_les_cf_" + suf).argtypes = [fp, fp, cr, cr, C.c_int]
+        getattr(L, "orc_invert_phys_les_fc_" + suf).argtypes = [fp, fp, cr, cr, C.c_int]
+        getattr(L, "orc_order2_correction_" + suf).argtypes = [fp, fp, cr, cr, cr, C.c_int]
+        getattr(L, "orc_extract_" + suf).argtypes = [C.POINTER(OrcParams), C.POINTER(OrcRefine), fp, fp]
+        getattr(L, "orc_interp_cf_" + suf).argtypes = [C.POINTER(OrcRefine), fp, fp, cr, C.c_void_p, fp]
+        getattr(L, "orc_interp_fc_" + suf).argtypes = [C.POINTER(OrcParams), C.POINTER(OrcRefine), fp, C.c_void_p,
+                                                       fp]
+        getattr(L, "orc_refine_step_" + suf).argtypes = [C.POINTER(OrcParams), C.POINTER(OrcParams),
+                                                         C.POINTER(OrcRefine), C.c_void_p, C.c_void_p, C.c_void_p,
+                                                         fp, fp, fp, fp, fp, fp, C.c_void_p]
     L._refine_bound = True
     return L
+
+
+def _suf(dtype):
+    return "f64" if np.dtype(dtype) == np.float64 else "f32"
 
 
 def refine(a, nf, omega_c, omega_f=None, cs=0.0, temporal=1, eq_kind=0):
@@ -240,38 +246,49 @@ def footprint_size(r):
     return (r.nf[2] // 2 + 1) * (r.nf[1] // 2 + 1) * (r.nf[0] // 2 + 1)
 
 
-def extract(pc, r, fc):
-    M = np.zeros(9 * footprint_size(r), np.float32)
-    _bind_refine().orc_extract_f32(C.byref(pc), C.byref(r), np.ascontiguousarray(fc, np.float32), M)
+def extract(pc, r, fc, dtype=None):
+    """Footprint moments of a coarse lattice, in the lattice's precision
+    (float32 unless fc or dtype is float64)."""
+    dt = np.dtype(dtype or (np.float64 if np.asarray(fc).dtype == np.float64 else np.float32))
+    M = np.zeros(9 * footprint_size(r), dt)
+    getattr(_bind_refine(), "orc_extract_" + _suf(dt))(C.byref(pc), C.byref(r), np.ascontiguousarray(fc, dt), M)
     return M
 
 
 def interp_cf(r, Mold, Mnew, alpha, ff, fmask=None):
-    ff = np.ascontiguousarray(ff, np.float32).copy()
-    _bind_refine().orc_interp_cf_f32(C.byref(r), Mold, Mnew, alpha, _ptr(fmask), ff)
+    dt = np.asarray(Mold).dtype
+    ff = np.ascontiguousarray(ff, dt).copy()
+    getattr(_bind_refine(), "orc_interp_cf_" + _suf(dt))(C.byref(r), Mold, Mnew, alpha, _ptr(fmask), ff)
     return ff
 
 
 def interp_fc(pc, r, ff, fc, cmask=None):
-    fc = np.ascontiguousarray(fc, np.float32).copy()
-    _bind_refine().orc_interp_fc_f32(C.byref(pc), C.byref(r), np.ascontiguousarray(ff, np.float32), _ptr(cmask), fc)
+    dt = np.float64 if np.asarray(fc).dtype == np.float64 else np.float32
+    fc = np.ascontiguousarray(fc, dt).copy()
+    getattr(_bind_refine(), "orc_interp_fc_" + _suf(dt))(C.byref(pc), C.byref(r), np.ascontiguousarray(ff, dt),
+                                                         _ptr(cmask), fc)
     return fc
 
 
 def refine_run(pc, pf, r, ca, fa, nsteps, cmask=None, fmask=None, u_in=None, forces=False):
-    """Advance coarse (ca) and fine (fa) lattices nsteps coarse steps; with
+    """Advance coarse (ca) and fine (fa) lattices nsteps coarse steps, in
+    float32, or in float64 when ca is float64 (the FP64 interface); with
     forces, also the fine level's first-sub-step momentum exchange of each
     coarse step, [nsteps, 3]."""
     L = _bind_refine()
-    ca = np.ascontiguousarray(ca, np.float32).copy()
-    fa = np.ascontiguousarray(fa, np.float32).copy()
+    dt = np.float64 if np.asarray(ca).dtype == np.float64 else np.float32
+    ca = np.ascontiguousarray(ca, dt).copy()
+    fa = np.ascontiguousarray(fa, dt).copy()
+    if u_in is not None:
+        u_in = np.ascontiguousarray(u_in, dt)
     cb, fb = np.empty_like(ca), np.empty_like(fa)
     Mold = extract(pc, r, ca)
     Mnew = np.empty_like(Mold)
     F = np.zeros((max(nsteps, 1), 3), np.float64)
+    step = getattr(L, "orc_refine_step_" + _suf(dt))
     for t in range(nsteps):
-        L.orc_refine_step_f32(C.byref(pc), C.byref(pf), C.byref(r), _ptr(cmask), _ptr(fmask), _ptr(u_in),
-                              ca, cb, fa, fb, Mold, Mnew, F[t].ctypes.data)
+        step(C.byref(pc), C.byref(pf), C.byref(r), _ptr(cmask), _ptr(fmask), _ptr(u_in), ca, cb, fa, fb, Mold, Mnew,
+             F[t].ctypes.data)
     return (ca, fa, F) if forces else (ca, fa)
 
 

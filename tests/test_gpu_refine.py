@@ -1,5 +1,5 @@
 """2:1 local refinement on the GPU against the oracle's restatement
-(oracle/refine_impl.c) on the same inputs: both levels after several coarse
+(oracle/refine_impl.h, float and double instances) on the same inputs: both levels after several coarse
 steps, with and without temporal interpolation and with both interface
 equilibria; the fine mask bit-exact against the geometry evaluated at the
 fine nodes; the full-size patch stays finite and tracks the coarse flow."""
@@ -108,6 +108,43 @@ def test_refined_channel_other_storage_formats(storage, tol):
         err = rel_vel([g[sel] for g in got[1:]], [x[sel] for x in ref[1:]])
         say(f"  {name}: velocity error vs the FP32 oracle {err:.2e} (bar {tol})")
         assert err < tol
+
+
+@pytest.mark.parametrize("temporal", [True, False])
+def test_refined_channel_fp64_end_to_end(temporal):
+    """FP64 populations with the interface in FP64 too (moments, CF data,
+    trilinear/ORDER-2 interpolation, LES rescale): 10 coarse steps against
+    the oracle's double instance of the interface from the same FP64 state,
+    to 1e-11 -- FP64 rounding, where an FP32 step anywhere on the path would
+    show at 1e-7."""
+    say("refine FP64 end to end, temporal", temporal)
+    cc = small_case(temporal)
+    cc.storage = "f64"
+    c = cc.to_c()
+    T = 10
+    with L.Solver(c) as s:
+        s.set_geometry(None)
+        cm, fm = to_orc(s.get_mask(0)), to_orc(s.get_mask(1))
+        s.init_equilibrium(L.INIT_MRT)
+        c0, f0 = s.populations(0, f64=True), s.populations(1, f64=True)
+        s.step(T)
+        cg, fg = s.moments(0), s.moments(1)
+    pc = orc_params(c)
+    r = O.refine([int(round(x - 0.25)) for x in cc.lr_loc], cc.lr_size, float(np.float32(c.omega)),
+                 cs=float(np.float32(c.cs)), temporal=int(temporal), eq_kind=O.EQ_BGK_MEQ)
+    pf = O.params(*cc.lr_size, bc=(O.CLAMP,) * 3, collision=pc.collision, halfway=1, omega=r.omega_f,
+                  cs=pc.cs, dpdy=0.5 * pc.dpdy)
+    u_in = O.inlet_profile(c.ny, c.nz, c.umax, c.inlet_amp, c.seed)
+    ca, fa = O.refine_run(pc, pf, r, c0, f0, T, cmask=cm, fmask=fm, u_in=u_in)
+    assert ca.dtype == np.float64
+    co = O.macro(pc, ca)
+    fo = O.macro(O.params(*cc.lr_size), fa)
+    inner = np.zeros_like(fm, dtype=bool)
+    inner[2:-2, 2:-2, 2:-2] = True
+    for name, got, ref, sel in (("coarse", cg, co, cm == O.FLUID), ("fine", fg, fo, (fm == O.FLUID) & inner)):
+        err = max(rel(got[0][sel], ref[0][sel]), rel_vel([g[sel] for g in got[1:]], [x[sel] for x in ref[1:]]))
+        say(f"  {name}: error vs the FP64 oracle {err:.2e}")
+        assert err < 1e-11
 
 
 def test_fine_mask_is_geometry_at_fine_nodes():

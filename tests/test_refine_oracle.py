@@ -9,8 +9,15 @@ following MRTtest/DynInlet3.cu:367-595 and :2202-2560) with closed forms:
     SF_cf = LRFACTOR omega_c (1 - omega_f) / ((1 - omega_c) omega_f) of main()
     (MRTtest/DynInlet3.cu:2878), and FC's SF to its inverse;
   * StrainRate(m) = -3/2 (Pi_neq) (omega = 1 as coded) against the definition.
+The float instance and the double instance (the FP64 yardstick of the refined
+path, the same source compiled in double) are both held to these, and the two
+agree on a refined run to float rounding.  The double instance keeps the
+reference's decimal constants (-0.026315789 for -1/38 in StrainRate,
+0.666666667, 7.53968254), so its closed-form residuals are set by their
+truncation (1e-12 to 1e-10 here), not by double rounding.
 """
 import numpy as np
+import pytest
 
 import oracle_lib as O
 
@@ -19,22 +26,26 @@ C = np.array([(0, 0, 0), (1, 0, 0), (0, 1, 0), (-1, 0, 0), (0, -1, 0), (1, 1, 0)
               (0, 1, -1), (-1, 0, -1), (0, -1, -1)])
 
 
-def mrt_eq(rho, u, v, w):
+DTYPES = [np.float32, np.float64]
+SUF = {np.float32: "f32", np.float64: "f64"}
+
+
+def mrt_eq(rho, u, v, w, dtype=np.float32):
     """The fixed point of the LES-family collide: M^-1 bgk_meq (BGK feq with
     3pixx = -meq9/6), which CF/FC rebuild with eq_kind = EQ_BGK_MEQ."""
     f = O.feq_bgk(rho, u, v, w, dtype=np.float64)
     m9 = 2 * u * u - (v * v + w * w)
     col10 = O.moment_matrix()[10] / 72.0
-    return (f + (-m9 / 6 + m9 / 2) * col10).astype(np.float32)
+    return (f + (-m9 / 6 + m9 / 2) * col10).astype(dtype)
 
 
-def coarse_field(n, fn):
+def coarse_field(n, fn, dtype=np.float32):
     nx, ny, nz = n
-    f = np.empty((19, nz, ny, nx), np.float32)
+    f = np.empty((19, nz, ny, nx), dtype)
     for z in range(nz):
         for y in range(ny):
             for x in range(nx):
-                f[:, z, y, x] = mrt_eq(*fn(x, y, z))
+                f[:, z, y, x] = mrt_eq(*fn(x, y, z), dtype=dtype)
     return f
 
 
@@ -61,14 +72,18 @@ def test_phys_moments_and_strain_rate():
         assert np.allclose(S[[3, 4, 5]], -1.5 * np.array([Pneq[0, 1], Pneq[1, 2], Pneq[0, 2]]), atol=3e-7)
 
 
-def setup(n=(20, 16, 16), a=(3, 2, 2), nf=(16, 12, 12), omega=1.6, fn=None):
+def setup(n=(20, 16, 16), a=(3, 2, 2), nf=(16, 12, 12), omega=1.6, fn=None, dtype=np.float32):
     pc = O.params(*n, bc=(O.CLAMP,) * 3, collision=O.MRT_LES, omega=omega)
     r = O.refine(a, nf, omega)
-    fc = coarse_field(n, fn)
+    fc = coarse_field(n, fn, dtype)
     return pc, r, fc
 
 
-def test_cf_fc_reproduce_uniform_and_linear_fields():
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_cf_fc_reproduce_uniform_and_linear_fields(dtype):
+    # the linear field's u u terms are quadratic in position, which the
+    # trilinear moments miss by up to 9e-8 even in double; the uniform field is
+    # exact up to the constants' truncation
     def lin(x, y, z):
         return (1.0 + 1e-4 * x, 0.04 + 1e-3 * y, -0.01 + 5e-4 * z, 0.002 * x - 0.001 * z)
 
@@ -76,11 +91,13 @@ def test_cf_fc_reproduce_uniform_and_linear_fields():
         rho, u, v, w = lin(x, y, z)
         return (1.0, u, v, w)
 
-    for fn in (lambda x, y, z: (1.01, 0.03, -0.01, 0.02), lin_rho1):
-        pc, r, fc = setup(fn=fn)
+    for fn, tol64 in ((lambda x, y, z: (1.01, 0.03, -0.01, 0.02), 3e-12), (lin_rho1, 1.5e-7)):
+        tol = 3e-7 if dtype == np.float32 else tol64
+        pc, r, fc = setup(fn=fn, dtype=dtype)
         M = O.extract(pc, r, fc)
+        assert M.dtype == dtype
         nf = tuple(r.nf)
-        ff = np.zeros((19, nf[2], nf[1], nf[0]), np.float32)
+        ff = np.zeros((19, nf[2], nf[1], nf[0]), dtype)
         out = O.interp_cf(r, M, M, 0.0, ff)
         for k in range(nf[2]):
             for j in range(nf[1]):
@@ -90,20 +107,20 @@ def test_cf_fc_reproduce_uniform_and_linear_fields():
                         assert np.all(out[:, k, j, i] == 0)
                         continue
                     X, Y, Z = r.a[0] + 0.25 + 0.5 * i, r.a[1] + 0.25 + 0.5 * j, r.a[2] + 0.25 + 0.5 * k
-                    exp = mrt_eq(*fn(X, Y, Z))
-                    assert np.abs(out[:, k, j, i] - exp).max() < 3e-7
+                    exp = mrt_eq(*fn(X, Y, Z), dtype=dtype)
+                    assert np.abs(out[:, k, j, i] - exp).max() < tol
         # FC of the exact fine field gives the coarse equilibrium back
         fine = np.empty_like(ff)
         for k in range(nf[2]):
             for j in range(nf[1]):
                 for i in range(nf[0]):
                     fine[:, k, j, i] = mrt_eq(*fn(r.a[0] + 0.25 + 0.5 * i, r.a[1] + 0.25 + 0.5 * j,
-                                                  r.a[2] + 0.25 + 0.5 * k))
+                                                  r.a[2] + 0.25 + 0.5 * k), dtype=dtype)
         back = O.interp_fc(pc, r, fine, np.zeros_like(fc))
         for z in range(r.a[2] + 2, r.a[2] + nf[2] // 2 - 1):
             for y in range(r.a[1] + 2, r.a[1] + nf[1] // 2 - 1):
                 for x in range(r.a[0] + 2, r.a[0] + nf[0] // 2 - 1):
-                    assert np.abs(back[:, z, y, x] - fc[:, z, y, x]).max() < 3e-7
+                    assert np.abs(back[:, z, y, x] - fc[:, z, y, x]).max() < tol
         assert np.all(back[:, :, :, : r.a[0] + 2] == 0)
 
 
@@ -119,23 +136,24 @@ def _moments_with_strain(u, S):
                      P[0, 1] + ux * uy, P[1, 2] + uy * uz, P[0, 2] + ux * uz])
 
 
-def _order2_at(vel, sgrad, pts):
+def _order2_at(vel, sgrad, pts, dtype=np.float32):
     """Trilinear interpolation of the 8 corner moments of the unit cube plus
     the ORDER-2 correction, at fine positions pts; returns (got, exact) j."""
     L = O._bind_refine()
     corners = [np.array([k & 1, (k >> 1) & 1, (k >> 2) & 1], float) for k in range(8)]
-    c = np.ascontiguousarray([_moments_with_strain(vel(X), sgrad(X)) for X in corners], np.float32)
+    c = np.ascontiguousarray([_moments_with_strain(vel(X), sgrad(X)) for X in corners], dtype)
     got, exact = [], []
     for p in pts:
         w = np.array([np.prod([p[a] if X[a] else 1 - p[a] for a in range(3)]) for X in corners])
-        mom = (w @ c.astype(np.float64)).astype(np.float32)
-        L.orc_order2_correction_f32(c, mom, *(float(v) for v in p), 0)
+        mom = (w @ c.astype(np.float64)).astype(dtype)
+        getattr(L, "orc_order2_correction_" + SUF[dtype])(c, mom, *(float(v) for v in p), 0)
         got.append(mom[1:4].copy())
         exact.append(vel(np.asarray(p, float)))
     return np.array(got), np.array(exact)
 
 
-def test_order2_reproduces_divergence_free_quadratic_fields():
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_order2_reproduces_divergence_free_quadratic_fields(dtype):
     """ORDER 2 (MRTtest/DynInlet3.cu:2312-2357): the trilinear momentum plus
     the correction built from the corners' strain rates and velocity
     differences reproduces every divergence-free quadratic velocity field
@@ -173,13 +191,14 @@ def test_order2_reproduces_divergence_free_quadratic_fields():
     ]
     pts = [(x, y, z) for x in (0.25, 0.75, 0.4) for y in (0.25, 0.75, 0.6) for z in (0.25, 0.75)]
     for vel, Gfn in fields:
-        got, exact = _order2_at(vel, grad_field(Gfn), pts)
-        assert np.abs(got - exact).max() < 2e-9
+        got, exact = _order2_at(vel, grad_field(Gfn), pts, dtype)
+        assert np.abs(got - exact).max() < (2e-9 if dtype == np.float32 else 2e-10)
         # trilinear alone misses the curvature by up to a/4 x (1 - x) terms
         assert np.abs(exact).max() > 0
 
 
-def test_order2_with_compressible_curvature_recovers_two_thirds():
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_order2_with_compressible_curvature_recovers_two_thirds(dtype):
     """u = a x^2 alone: the strain rate the corners can carry is deviatoric
     (StrainRate drops the trace), so S_xx = 2/3 du/dx and the correction
     supplies 2/3 of the curvature: the residual is exactly a x (1 - x) / 3."""
@@ -193,10 +212,11 @@ def test_order2_with_compressible_curvature_recovers_two_thirds():
         return g - np.trace(g) / 3 * np.eye(3)
 
     pts = [(x, 0.5, 0.5) for x in (0.25, 0.75, 0.4)]
-    got, exact = _order2_at(vel, sgrad, pts)
+    got, exact = _order2_at(vel, sgrad, pts, dtype)
+    tol = 2e-9 if dtype == np.float32 else 1e-10
     for p, g, ex in zip(pts, got, exact):
-        assert abs((g[0] - ex[0]) - a * p[0] * (1 - p[0]) / 3) < 2e-9
-        assert abs(g[1]) < 1e-9 and abs(g[2]) < 1e-9
+        assert abs((g[0] - ex[0]) - a * p[0] * (1 - p[0]) / 3) < tol
+        assert abs(g[1]) < tol and abs(g[2]) < tol
 
 
 def test_temporal_midpoint_is_exact():
@@ -240,7 +260,8 @@ def test_les_rescale_reduces_to_reference_sf():
         assert abs(m[2] - 475 / 63 * (u * u + v * v + w * w)) < 1e-6 and abs(m[10]) < 1e-7 and abs(m[12]) < 1e-7
 
 
-def test_refined_run_conserves_uniform_flow():
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_refined_run_conserves_uniform_flow(dtype):
     """A uniform stream at the collide's fixed point stays exactly uniform
     through the refined region when the interface rebuilds populations with
     the collide's own equilibrium (eq_kind = EQ_BGK_MEQ); with mrt_meq as
@@ -250,12 +271,37 @@ def test_refined_run_conserves_uniform_flow():
     pc = O.params(*n, bc=(O.PERIODIC,) * 3, collision=O.MRT_LES, omega=1.5, cs=0.01)
     r = O.refine((4, 3, 3), (20, 16, 16), 1.5, cs=0.01)
     pf = O.params(*r.nf, bc=(O.CLAMP,) * 3, collision=O.MRT_LES, omega=r.omega_f, cs=0.01)
-    fc = coarse_field(n, lambda x, y, z: (1.0, 0.03, 0.01, -0.005))
-    ff = coarse_field(tuple(r.nf), lambda x, y, z: (1.0, 0.03, 0.01, -0.005))
+    fc = coarse_field(n, lambda x, y, z: (1.0, 0.03, 0.01, -0.005), dtype)
+    ff = coarse_field(tuple(r.nf), lambda x, y, z: (1.0, 0.03, 0.01, -0.005), dtype)
     ca, fa = O.refine_run(pc, pf, r, fc, ff, 5)
-    assert np.abs(ca - fc).max() < 2e-6 and np.abs(fa - ff).max() < 2e-6
+    assert ca.dtype == dtype
+    tol = 2e-6 if dtype == np.float32 else 3e-12
+    assert np.abs(ca - fc).max() < tol and np.abs(fa - ff).max() < tol
     r.eq_kind = O.EQ_MRT_MEQ
     ca, fa = O.refine_run(pc, pf, r, fc, ff, 5)
     pfine = O.params(*r.nf)
     rho = O.macro(pfine, fa)[0]
     assert np.abs(rho - 1.0).max() > 1e-3
+
+
+def test_double_interface_agrees_with_float_on_a_refined_run():
+    """The double instance against the float one on 4 coarse steps of a
+    random near-equilibrium state with walls, forcing and LES: FP32 rounding
+    apart (the two are one source compiled twice, so a macro or constant
+    slip in either shows here)."""
+    rng = np.random.default_rng(5)
+    n, a, nf = (24, 16, 16), (6, 4, 4), (16, 12, 12)
+    pc = O.params(*n, bc=(O.CLAMP, O.WALL, O.WALL), collision=O.MRT_LES, omega=1 / 0.53, cs=0.01, halfway=1,
+                  dpdy=1e-5)
+    r = O.refine(a, nf, float(np.float32(1 / 0.53)), cs=0.01)
+    pf = O.params(*nf, bc=(O.CLAMP,) * 3, collision=O.MRT_LES, omega=r.omega_f, cs=0.01, halfway=1, dpdy=5e-6)
+    fc = coarse_field(n, lambda x, y, z: (1.0, 0.04, 0.0, 0.0), np.float64)
+    ff = coarse_field(nf, lambda x, y, z: (1.0, 0.04, 0.0, 0.0), np.float64)
+    fc *= 1 + rng.uniform(-0.02, 0.02, fc.shape)
+    ff *= 1 + rng.uniform(-0.02, 0.02, ff.shape)
+    c64, f64 = O.refine_run(pc, pf, r, fc, ff, 4)
+    c32, f32 = O.refine_run(pc, pf, r, fc.astype(np.float32), ff.astype(np.float32), 4)
+    assert c64.dtype == np.float64 and c32.dtype == np.float32
+    scale = np.abs(c64).max()
+    d = max(np.abs(c64 - c32).max(), np.abs(f64 - f32).max()) / scale
+    assert 1e-9 < d < 2e-6
