@@ -133,9 +133,8 @@ struct StepParams {
 //    sweep the lattice together but each streams C workgroups' stretches
 //    back to back (the x-shifted loads' straddling lines then stay in one
 //    XCD's L2).  Ids past the last whole window map to themselves.
-__device__ __forceinline__ int logical_block(int chunk = 1)
+__device__ __forceinline__ int logical_block_of(int hw, int G, int chunk)
 {
-    const int G = gridDim.x, hw = blockIdx.x;
     if (LBM_XCD_REMAP) {
         const int per = G >> 3, rem = G & 7, xcd = hw & 7;
         return xcd * per + (xcd < rem ? xcd : rem) + (hw >> 3);
@@ -148,6 +147,10 @@ __device__ __forceinline__ int logical_block(int chunk = 1)
         }
     }
     return hw;
+}
+__device__ __forceinline__ int logical_block(int chunk = 1)
+{
+    return logical_block_of(blockIdx.x, gridDim.x, chunk);
 }
 
 __device__ __forceinline__ void axis_nb(int c, int n, int bc, int& m, int& p, bool& om, bool& op)
@@ -542,7 +545,8 @@ __device__ __forceinline__ void node_finish(const StepParams& p, int y, int cell
     // a ghost copy's statistics see the copied populations, as update_inn's
     // vel_av follows the copies (MRTtest/DynInlet3.cu:1747-1779)
     if (act == ACT_COPY && p.stat_mode) stats_update(p, cell, f);
-    node_store_policy<S, AUX>(p, cell, f);
+    if constexpr (AUX < 0) node_store<S>(p, cell, f); // the level's policy at run time
+    else node_store_policy<S, AUX>(p, cell, f);
 }
 
 // Footprint moments of a coarse node under refinement (defined with the
@@ -630,20 +634,41 @@ __global__ void __launch_bounds__(256) k_reduce_steps(const double* part, long l
 
 // The stream-collide step; NT selects nontemporal population stores (the
 // level's p.nt_store, as a template argument so each variant is straight-line).
-template <int MODEL, class S, bool NT>
-__global__ void __launch_bounds__(Shape<S>::THREADS, Shape<S>::MIN_WAVES) k_stream_collide(StepParams p)
+template <int MODEL, class S, int AUX>
+__device__ __forceinline__ void sc_block(const StepParams& p, int lb)
 {
     using R = typename Arith<S>::R;
     constexpr int NW = Shape<S>::NW;
-    const int lb = logical_block(p.xcd_chunk);
     // only workgroups holding a force node write a partial; the slot is read
     // (a scalar load) before the pull, so its latency hides under the loads
     const int slot = p.fpart ? (p.fslot ? p.fslot[lb] : lb) : -1;
     R fx = 0, fy = 0, fz = 0;
     int x, y, z;
     const bool row_ok = wave_node<NW>(lb, p.nsegw, p.ny, p.nz, x, y, z);
-    if (row_ok && x < p.nx) node_update<MODEL, S, NT ? kStoreNT : 0>(p, x, y, z, fx, fy, fz);
+    if (row_ok && x < p.nx) node_update<MODEL, S, AUX>(p, x, y, z, fx, fy, fz);
     if (slot >= 0) wave_force_partial<NW>(p, slot, fx, fy, fz);
+}
+
+template <int MODEL, class S, bool NT>
+__global__ void __launch_bounds__(Shape<S>::THREADS, Shape<S>::MIN_WAVES) k_stream_collide(StepParams p)
+{
+    sc_block<MODEL, S, NT ? kStoreNT : 0>(p, logical_block(p.xcd_chunk));
+}
+
+// Two levels in one launch: workgroups below blocks_a update level a (the
+// coarse level), the rest level b (a fine sub-step that reads nothing level
+// a writes).  The hardware hands out workgroup ids in order, so b's
+// workgroups fill the compute units a's last workgroups leave idle, with no
+// second stream and no cross-stream wait.  The store policy is each level's
+// own, chosen at run time (p.nt_store; as template arguments it measured
+// 0.5 % faster for 2.5 times the code).
+template <int MODEL, class S>
+__global__ void __launch_bounds__(Shape<S>::THREADS, Shape<S>::MIN_WAVES)
+    k_stream_collide_two(StepParams a, StepParams b, int blocks_a)
+{
+    const int hw = blockIdx.x;
+    if (hw < blocks_a) sc_block<MODEL, S, -1>(a, logical_block_of(hw, blocks_a, a.xcd_chunk));
+    else sc_block<MODEL, S, -1>(b, hw - blocks_a);
 }
 
 // Deterministic FP64 reduction of ncomp-component partials (fixed order:

@@ -28,7 +28,7 @@ namespace {
 thread_local std::string g_last_error;
 
 #ifndef LBM_FINE_OVERLAP
-#define LBM_FINE_OVERLAP 1 // first fine sub-step on the side stream beside the coarse step
+#define LBM_FINE_OVERLAP 1 // first fine sub-step in the coarse stream-collide's launch
 #endif
 
 struct Level {
@@ -56,11 +56,6 @@ struct lbm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // refinement: the first fine sub-step's stream beside the coarse step
-    // (default priority), and the events that fork it from and join it to
-    // the main stream
-    hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     Level lv[2];
     int nlevels = 1;
     float* u_in = nullptr;     // west inlet profile [nz*ny]
@@ -131,7 +126,7 @@ static void classify_fine(lbm_ctx* ctx, bool user_mask);
 static int refine_extract(lbm_ctx* ctx, const void* Mold, void* M, void* Pnew, void* Phalf);
 static int refine_cf(lbm_ctx* ctx, const void* P, hipStream_t stream);
 static int refine_fc(lbm_ctx* ctx);
-static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f, double* fpart, hipStream_t stream);
+static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f, double* fpart);
 static int set_geometry_at(lbm_ctx* ctx, const uint8_t* mask, long long t);
 static int flush_forces(lbm_ctx* ctx);
 
@@ -405,11 +400,6 @@ extern "C" int lbm_create(const lbm_config* cfg, lbm_ctx** out)
         return bail(fail(nullptr, LBM_EDEVICE, "lbm_create: stream creation failed"));
     if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess)
         return bail(fail(nullptr, LBM_EDEVICE, "lbm_create: event creation failed"));
-    if (cfg->refinement &&
-        (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
-         hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
-         hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess))
-        return bail(fail(nullptr, LBM_EDEVICE, "lbm_create: refinement stream or event creation failed"));
     if ((rc = alloc_level(ctx, ctx->lv[0], cfg->nx, cfg->ny, cfg->nz, cfg->storage)) != LBM_OK) {
         g_last_error = ctx->err;
         return bail(rc);
@@ -464,12 +454,8 @@ extern "C" void lbm_destroy(lbm_ctx* ctx)
                     (void*)ctx->dwall,
                     (void*)ctx->omega_field})
         if (p) (void)hipFree(p);
-    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
-    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
-    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -862,6 +848,24 @@ static void launch_sc(int model, const StepParams& p, int blocks, hipStream_t s)
     else launch_sc_policy<S, false>(model, p, blocks, s);
 }
 
+// Level a (blocks_a workgroups) and level b (blocks_b) in one launch.
+template <class S>
+static void launch_sc_two(int model, const StepParams& a, int blocks_a, const StepParams& b, int blocks_b,
+                          hipStream_t s)
+{
+    const dim3 blk(Shape<S>::THREADS), grd(blocks_a + blocks_b);
+    switch (model) {
+    case LBM_MODEL_BGK: hipLaunchKernelGGL((k_stream_collide_two<MODEL_BGK, S>), grd, blk, 0, s, a, b, blocks_a); break;
+    case LBM_MODEL_MRT_CPU3D:
+        hipLaunchKernelGGL((k_stream_collide_two<MODEL_MRT_CPU3D, S>), grd, blk, 0, s, a, b, blocks_a);
+        break;
+    case LBM_MODEL_MRT_LES:
+        hipLaunchKernelGGL((k_stream_collide_two<MODEL_MRT_LES, S>), grd, blk, 0, s, a, b, blocks_a);
+        break;
+    default: hipLaunchKernelGGL((k_stream_collide_two<MODEL_MRT_DHUM, S>), grd, blk, 0, s, a, b, blocks_a); break;
+    }
+}
+
 // Launches the fused closure collide over level L; returns its workgroup
 // count (the number of force partials it wrote).
 template <class S>
@@ -968,13 +972,48 @@ static bool records_force(const lbm_ctx* ctx, long long t)
     return c.force_steps > 0 && t > c.startf && (t - c.startf) < c.force_steps;
 }
 
+// The coarse level's step parameters; with refinement its footprint nodes
+// also write the moments at the step's end and the CF data of this step's
+// second fine sub-step and of the next step's first.
+static StepParams coarse_params(lbm_ctx* ctx)
+{
+    StepParams p = step_params(ctx, ctx->lv[0], ctx->cfg.omega);
+    if (ctx->nlevels > 1) {
+        for (int a = 0; a < 3; ++a) {
+            p.fp_a[a] = ctx->rg.a[a];
+            p.fp_n[a] = ctx->rg.nc[a];
+        }
+        p.fp_mold = ctx->Mold;
+        p.fp_m = ctx->Mnew;
+        p.fp_pnew = ctx->Pnew;
+        p.fp_phalf = ctx->cfg.temporal_interp ? ctx->Phalf : nullptr;
+    }
+    return p;
+}
+
+// A fine sub-step's parameters (update_inn_LR, MRTtest/DynInlet3.cu:2051-2201):
+// clamped axes at the fine rate; with fpart (the batch position of the first
+// sub-step of a recorded coarse step, flag_F at :3183) the fine level's
+// momentum exchange goes to the force history.
+static StepParams fine_params(lbm_ctx* ctx, float omega_f, double* fpart)
+{
+    Level& F = ctx->lv[1];
+    StepParams p = step_params(ctx, F, omega_f);
+    p.bcx = p.bcy = p.bcz = LBM_BC_CLAMP;
+    if (fpart) {
+        p.fpart = fpart;
+        p.fslot = F.fslot;
+    }
+    return p;
+}
+
 // One coarse-level stream-collide launch (update_inn/top/bot on one GPU),
 // plus the deterministic force reduction when this step is recorded.
 static int coarse_step(lbm_ctx* ctx)
 {
     Level& L = ctx->lv[0];
     const lbm_config& c = ctx->cfg;
-    StepParams p = step_params(ctx, L, c.omega);
+    StepParams p = coarse_params(ctx);
     // with refinement the fine level records the force, as main() turns the
     // coarse flag_F off under REFINEMENT (MRTtest/DynInlet3.cu:3130)
     const bool rec = ctx->nlevels == 1 && records_force(ctx, ctx->t);
@@ -988,18 +1027,6 @@ static int coarse_step(lbm_ctx* ctx)
         if (int rc = flush_forces(ctx)) return rc;
         if (int rc = ensure_fpart(ctx, (size_t)L.blocks * 3)) return rc;
         p.fpart = ctx->fpart;
-    }
-    if (ctx->nlevels > 1) {
-        // the footprint's moments at the step's end and the CF data of this
-        // step's second fine sub-step and of the next step's first
-        for (int a = 0; a < 3; ++a) {
-            p.fp_a[a] = ctx->rg.a[a];
-            p.fp_n[a] = ctx->rg.nc[a];
-        }
-        p.fp_mold = ctx->Mold;
-        p.fp_m = ctx->Mnew;
-        p.fp_pnew = ctx->Pnew;
-        p.fp_phalf = c.temporal_interp ? ctx->Phalf : nullptr;
     }
     long long nparts = 0;
     with_storage(L.storage, [&](auto* tag) {
@@ -1026,23 +1053,14 @@ static int coarse_step(lbm_ctx* ctx)
     return LBM_OK;
 }
 
-// Fine-level sub-step (update_inn_LR, MRTtest/DynInlet3.cu:2051-2201) on
-// the given stream.  With fpart set (the batch position of the first
-// sub-step of a recorded coarse step, flag_F at :3183) the fine level's
-// momentum exchange goes to the force history.
-static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f, double* fpart, hipStream_t stream)
+// One fine sub-step launch on its own (fine_params).
+static int lbm_internal_fine_step(lbm_ctx* ctx, float omega_f, double* fpart)
 {
     Level& F = ctx->lv[1];
-    const lbm_config& c = ctx->cfg;
-    StepParams p = step_params(ctx, F, omega_f);
-    p.bcx = p.bcy = p.bcz = LBM_BC_CLAMP;
-    if (fpart) {
-        p.fpart = fpart;
-        p.fslot = F.fslot;
-    }
+    const StepParams p = fine_params(ctx, omega_f, fpart);
     with_storage(F.storage, [&](auto* tag) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        launch_sc<S>(c.model, p, F.blocks, stream);
+        launch_sc<S>(ctx->cfg.model, p, F.blocks, ctx->stream);
         return 0;
     });
     std::swap(F.fa, F.fb);
@@ -1240,50 +1258,65 @@ extern "C" int lbm_step(lbm_ctx* ctx, int nsteps)
     if (!ctx->initialised) return fail(ctx, LBM_ESTATE, "lbm_step: call lbm_init_equilibrium first");
     if (nsteps < 0) return fail(ctx, LBM_EINVAL, "lbm_step: nsteps < 0");
     const lbm_config& c = ctx->cfg;
+    // the trip obstacle exists for t < trip_until (ImageFcn(x, y, z, t)): a
+    // step at which the generated mask changes
+    auto geometry_switch = [&](long long t) {
+        return !ctx->user_mask && c.geometry == LBM_GEOM_PRECURSOR && ctx->trip_on != (t < c.trip_until);
+    };
+    // the CF pass of the current step's first fine sub-step already launched
+    // (at the end of the previous step of this call)
+    bool cf_done = false;
     for (int s = 0; s < nsteps; ++s) {
-        // ImageFcn(x, y, z, t): the trip obstacle exists for t < trip_until
-        if (!ctx->user_mask && c.geometry == LBM_GEOM_PRECURSOR && ctx->trip_on != (ctx->t < c.trip_until))
+        if (geometry_switch(ctx->t))
             if (int rc = set_geometry_at(ctx, nullptr, ctx->t)) return rc;
         if (ctx->nlevels > 1) {
             // coarse t -> t+1, the dynamic-inlet control; fine sub-steps
             // t -> t+1/2 -> t+1 (main(), MRTtest/DynInlet3.cu:3147-3245), then
-            // the fine -> coarse restriction.  The first sub-step and its CF
-            // pass read only the fine level and the coarse data packed one
-            // step earlier (Pold), so they run on the side stream beside the
-            // coarse stream-collide and the footprint extraction; the fork
-            // follows every earlier main-stream launch (the last restriction,
-            // a force-batch flush).  On a control step the body force they
-            // use changes after the coarse step, so they wait for it there.
+            // the fine -> coarse restriction.  The first sub-step reads only
+            // the fine level and the frame CF fills from the coarse data
+            // packed one step earlier (Pold), so it runs in the same launch
+            // as the coarse stream-collide (which packs this step's CF data),
+            // its workgroups after the coarse ones.  On a control step the
+            // body force it uses changes after the coarse step, so it runs
+            // on its own after the control.  The CF pass of the next step's
+            // first sub-step is launched before this step's restriction,
+            // which it does not touch (the restriction reads fine nodes off
+            // the frame and writes coarse ones; CF writes only the frame).
             const bool temporal = ctx->cfg.temporal_interp;
-            const bool overlap = LBM_FINE_OVERLAP && !control_due(ctx, ctx->t);
+            // (16-bit storage: the two-level kernel would not fit the 64
+            // registers those kernels are held to, so its levels run apart)
+            const bool together = LBM_FINE_OVERLAP && !control_due(ctx, ctx->t) && !is16_storage(ctx->lv[0].storage);
+            Level& L = ctx->lv[0];
             Level& F = ctx->lv[1];
             double* fpart = nullptr;
             if (records_force(ctx, ctx->t))
                 if (int rc = batch_slot(ctx, ctx->t, F.nfslot * nwave_of_storage(F.storage), &fpart)) return rc;
             int rc = LBM_OK;
-            if (overlap) {
-                HIP_TRY(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
-                HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-                rc = refine_cf(ctx, ctx->Pold, ctx->side);
-                if (!rc) rc = lbm_internal_fine_step(ctx, ctx->rg.omega_f, fpart, ctx->side);
-                if (rc) return rc;
-                HIP_TRY(ctx, hipEventRecord(ctx->ev_join, ctx->side));
-            }
-            if ((rc = coarse_step(ctx))) return rc;
-            if ((rc = inlet_control(ctx, ctx->t))) return rc;
-            if (!overlap) {
-                rc = refine_cf(ctx, ctx->Pold, ctx->stream);
-                if (!rc) rc = lbm_internal_fine_step(ctx, ctx->rg.omega_f, fpart, ctx->stream);
-                if (rc) return rc;
+            if (!cf_done && (rc = refine_cf(ctx, ctx->Pold, ctx->stream))) return rc;
+            if (together) {
+                StepParams pa = coarse_params(ctx);
+                StepParams pb = fine_params(ctx, ctx->rg.omega_f, fpart);
+                with_storage(L.storage, [&](auto* tag) {
+                    using S = std::remove_pointer_t<decltype(tag)>;
+                    if constexpr (sizeof(S) != 2) launch_sc_two<S>(c.model, pa, L.blocks, pb, F.blocks, ctx->stream);
+                    return 0;
+                });
+                std::swap(L.fa, L.fb);
+                std::swap(F.fa, F.fb);
+            } else {
+                if ((rc = coarse_step(ctx))) return rc;
+                if ((rc = inlet_control(ctx, ctx->t))) return rc;
+                if ((rc = lbm_internal_fine_step(ctx, ctx->rg.omega_f, fpart))) return rc;
             }
             // (the coarse stream-collide wrote Mnew, Pnew and Phalf)
-            if (overlap) HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
             if (temporal) rc = refine_cf(ctx, ctx->Phalf, ctx->stream);
-            if (!rc) rc = lbm_internal_fine_step(ctx, ctx->rg.omega_f, nullptr, ctx->stream);
-            if (!rc) rc = refine_fc(ctx);
+            if (!rc) rc = lbm_internal_fine_step(ctx, ctx->rg.omega_f, nullptr);
             if (rc) return rc;
             std::swap(ctx->Mold, ctx->Mnew);
             std::swap(ctx->Pold, ctx->Pnew);
+            cf_done = LBM_FINE_OVERLAP && s + 1 < nsteps && !geometry_switch(ctx->t + 1);
+            if (cf_done && (rc = refine_cf(ctx, ctx->Pold, ctx->stream))) return rc;
+            if ((rc = refine_fc(ctx))) return rc;
         } else {
             if (int rc = coarse_step(ctx)) return rc;
             if (int rc = inlet_control(ctx, ctx->t)) return rc;
