@@ -327,7 +327,7 @@ __global__ void __launch_bounds__(64 * CLOSURE_NW, 2) k_stream_collide_closure(S
     chain_stage(s, ca.chain);
     const int lb = logical_block();
     int x, y, z;
-    const bool row_ok = wave_node<NW>(lb, p.nsegw, p.ny, p.nz, x, y, z);
+    const bool row_ok = wave_node_map<NW>(lb, p.map, p.nz, x, y, z);
     const bool active = row_ok && x < p.nx;
     const int cell = active ? x + p.nxp * (y + p.ny * z) : 0;
     R fx = 0, fy = 0, fz = 0;

@@ -84,6 +84,56 @@ __device__ __forceinline__ bool wave_node(int lb, int nsegw, int ny, int nz, int
     return z < nz;
 }
 
+// Division by a wave-uniform divisor fixed at launch, as a multiply-high and a
+// shift on the scalar unit (Granlund and Montgomery, "Division by invariant
+// integers using multiplication", PLDI 1994, section 4: l = ceil(log2 d),
+// m = floor(2^32 (2^l - d) / d) + 1, n / d = (mulhi(m, n) + n) >> l with the
+// sum in 33 bits).  A plain division by a run-time value compiles to a
+// float-reciprocal sequence on the vector unit, several instructions per
+// wavefront in the 16-bit kernels, which are bound by the vector unit.
+struct FastDiv {
+    uint32_t d, m, l;
+    __host__ static FastDiv of(uint32_t d)
+    {
+        FastDiv f;
+        f.d = d;
+        f.l = 0;
+        while ((1ull << f.l) < d) ++f.l;
+        f.m = (uint32_t)((((1ull << f.l) - d) << 32) / d + 1);
+        return f;
+    }
+    __host__ __device__ uint32_t div(uint32_t n) const
+    {
+#ifdef __HIP_DEVICE_COMPILE__
+        const uint32_t t = __umulhi(m, n);
+#else
+        const uint32_t t = (uint32_t)(((uint64_t)m * n) >> 32);
+#endif
+        return (uint32_t)(((uint64_t)t + n) >> l);
+    }
+};
+
+// The stream-collide's wave-to-node map with its divisors precomputed:
+// segment index g -> row = g / nsegw, z = row / ny.
+struct NodeMap {
+    FastDiv seg, ny;
+    // XCD-chunked renumbering (logical_block_of): window of 8 * chunk ids,
+    // and the ids below win_end, which are whole windows
+    FastDiv win;
+    int chunk, win_end;
+};
+
+__host__ inline NodeMap node_map_of(int nsegw, int ny, int chunk, int blocks)
+{
+    NodeMap m;
+    m.seg = FastDiv::of((uint32_t)nsegw);
+    m.ny = FastDiv::of((uint32_t)ny);
+    m.chunk = chunk;
+    m.win = FastDiv::of((uint32_t)(8 * chunk));
+    m.win_end = chunk > 1 ? (blocks / (8 * chunk)) * (8 * chunk) : 0;
+    return m;
+}
+
 struct StepParams {
     const void* fin;
     void* fout;
@@ -123,6 +173,7 @@ struct StepParams {
     void* fp_phalf;       // CF data of the midpoint, or nullptr
     int xcd_chunk; // consecutive workgroups per XCD (logical_block)
     int nt_store;  // nontemporal population stores (lattices larger than the MALL)
+    NodeMap map;   // the wave-to-node map's divisors (nsegw, ny, XCD window)
 };
 
 // XCD-aware renumbering.  The hardware id hw goes to XCD hw % 8.
@@ -151,6 +202,30 @@ __device__ __forceinline__ int logical_block_of(int hw, int G, int chunk)
 __device__ __forceinline__ int logical_block(int chunk = 1)
 {
     return logical_block_of(blockIdx.x, gridDim.x, chunk);
+}
+// The same renumbering with the window's divisor precomputed (NodeMap).
+__device__ __forceinline__ int logical_block_map(int hw, const NodeMap& m)
+{
+    if (LBM_XCD_REMAP) return logical_block_of(hw, (int)gridDim.x, m.chunk);
+    if (m.chunk > 1 && hw < m.win_end) {
+        const int W = 8 * m.chunk;
+        const int r = hw - (int)m.win.div((uint32_t)hw) * W;
+        return hw - r + (r & 7) * m.chunk + (r >> 3);
+    }
+    return hw;
+}
+
+// wave_node with the divisions by nsegw and ny done as FastDiv products.
+template <int NW>
+__device__ __forceinline__ bool wave_node_map(int lb, const NodeMap& m, int nz, int& x, int& y, int& z)
+{
+    const uint32_t g = (uint32_t)__builtin_amdgcn_readfirstlane(lb * NW + wave_id());
+    const uint32_t row = m.seg.div(g);
+    x = (int)(g - row * m.seg.d) * 64 + lane_id();
+    const uint32_t zz = m.ny.div(row);
+    y = (int)(row - zz * m.ny.d);
+    z = (int)zz;
+    return z < nz;
 }
 
 __device__ __forceinline__ void axis_nb(int c, int n, int bc, int& m, int& p, bool& om, bool& op)
@@ -644,7 +719,7 @@ __device__ __forceinline__ void sc_block(const StepParams& p, int lb)
     const int slot = p.fpart ? (p.fslot ? p.fslot[lb] : lb) : -1;
     R fx = 0, fy = 0, fz = 0;
     int x, y, z;
-    const bool row_ok = wave_node<NW>(lb, p.nsegw, p.ny, p.nz, x, y, z);
+    const bool row_ok = wave_node_map<NW>(lb, p.map, p.nz, x, y, z);
     if (row_ok && x < p.nx) node_update<MODEL, S, AUX>(p, x, y, z, fx, fy, fz);
     if (slot >= 0) wave_force_partial<NW>(p, slot, fx, fy, fz);
 }
@@ -652,7 +727,7 @@ __device__ __forceinline__ void sc_block(const StepParams& p, int lb)
 template <int MODEL, class S, bool NT>
 __global__ void __launch_bounds__(Shape<S>::THREADS, Shape<S>::MIN_WAVES) k_stream_collide(StepParams p)
 {
-    sc_block<MODEL, S, NT ? kStoreNT : 0>(p, logical_block(p.xcd_chunk));
+    sc_block<MODEL, S, NT ? kStoreNT : 0>(p, logical_block_map(blockIdx.x, p.map));
 }
 
 // Two levels in one launch: workgroups below blocks_a update level a (the
@@ -667,7 +742,7 @@ __global__ void __launch_bounds__(Shape<S>::THREADS, Shape<S>::MIN_WAVES)
     k_stream_collide_two(StepParams a, StepParams b, int blocks_a)
 {
     const int hw = blockIdx.x;
-    if (hw < blocks_a) sc_block<MODEL, S, -1>(a, logical_block_of(hw, blocks_a, a.xcd_chunk));
+    if (hw < blocks_a) sc_block<MODEL, S, -1>(a, logical_block_map(hw, a.map));
     else sc_block<MODEL, S, -1>(b, hw - blocks_a);
 }
 

@@ -812,6 +812,8 @@ static StepParams step_params(lbm_ctx* ctx, const Level& L, float omega)
     p.fp_mold = nullptr;
     p.fp_m = p.fp_pnew = p.fp_phalf = nullptr;
     p.xcd_chunk = xcd_chunk_of(L.storage, L.nsegw);
+    // the level's stream-collide launches take L.blocks workgroups
+    p.map = node_map_of(L.nsegw, L.ny, p.xcd_chunk, L.blocks);
     // both lattices within the 256 MB last-level cache: normal stores, so the
     // next step's loads can hit it (128x96x96 FP32: 31 vs 25 GLUPS); beyond
     // it, nontemporal stores (512x128x128 FP32: 35.3 vs 32.6 GLUPS)
@@ -1157,6 +1159,26 @@ static int refine_cf(lbm_ctx* ctx, const void* P, hipStream_t stream)
     return LBM_OK;
 }
 
+// The restriction together with the CF pass of the next step's first fine
+// sub-step (from P) in one launch (k_interp_fc_cf).
+static int refine_fc_cf(lbm_ctx* ctx, const void* P)
+{
+    Level& L = ctx->lv[0];
+    Level& F = ctx->lv[1];
+    const int nb = ctx->fc_nb[0] * ctx->fc_nb[1] * ctx->fc_nb[2];
+    const int nb_cf = (int)((frame_nodes(ctx->rg.nf) + 255) / 256);
+    with_storage(F.storage, [&](auto* tag) {
+        using S = std::remove_pointer_t<decltype(tag)>;
+        using R = typename Arith<S>::R;
+        hipLaunchKernelGGL(k_interp_fc_cf<S>, dim3(nb_cf + nb), dim3(256), 0, ctx->stream, (const S*)F.fa, (S*)F.fa,
+                           F.qs, F.nxp, F.mask, (const R*)P, (S*)L.fa, L.qs, L.nxp, L.ny, L.mask, ctx->rg,
+                           ctx->fc_nb[0], ctx->fc_nb[1], nb_cf, F.shift);
+        return 0;
+    });
+    HIP_TRY(ctx, hipGetLastError());
+    return LBM_OK;
+}
+
 static int refine_fc(lbm_ctx* ctx)
 {
     Level& L = ctx->lv[0];
@@ -1315,8 +1337,7 @@ extern "C" int lbm_step(lbm_ctx* ctx, int nsteps)
             std::swap(ctx->Mold, ctx->Mnew);
             std::swap(ctx->Pold, ctx->Pnew);
             cf_done = LBM_FINE_OVERLAP && s + 1 < nsteps && !geometry_switch(ctx->t + 1);
-            if (cf_done && (rc = refine_cf(ctx, ctx->Pold, ctx->stream))) return rc;
-            if ((rc = refine_fc(ctx))) return rc;
+            if ((rc = cf_done ? refine_fc_cf(ctx, ctx->Pold) : refine_fc(ctx))) return rc;
         } else {
             if (int rc = coarse_step(ctx)) return rc;
             if (int rc = inlet_control(ctx, ctx->t)) return rc;

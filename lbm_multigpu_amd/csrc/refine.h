@@ -252,6 +252,11 @@ __device__ __forceinline__ void footprint_extract(const StepParams& p, int x, in
     const int i = x - p.fp_a[0], j = y - p.fp_a[1], k = z - p.fp_a[2];
     if ((unsigned)i >= (unsigned)p.fp_n[0] || (unsigned)j >= (unsigned)p.fp_n[1] || (unsigned)k >= (unsigned)p.fp_n[2])
         return;
+    // k_interp_cf reads only the box's two outer layers on each face (the
+    // corners of fine frame nodes 0, 1, nf - 2, nf - 1 along an axis are
+    // coarse 0, 1, nc - 2, nc - 1); the inner nodes, which the restriction
+    // overwrites, write nothing (on the 128x96x96 patch 79 % of the box)
+    if (i >= 2 && i < p.fp_n[0] - 2 && j >= 2 && j < p.fp_n[1] - 2 && k >= 2 && k < p.fp_n[2] - 2) return;
     const long long NC = (long long)p.fp_n[0] * p.fp_n[1] * p.fp_n[2];
     const long long o = i + (long long)p.fp_n[0] * (j + (long long)p.fp_n[1] * k);
     R g[Q];
@@ -325,12 +330,12 @@ __host__ __device__ inline long long frame_nodes(const int nf[3])
 // and shared by neighbouring threads), then trilinear interpolation, ORDER-2
 // correction and the LES-aware rescale.
 template <class S>
-__global__ void __launch_bounds__(256) k_interp_cf(S* ff, long long fqs, int fnxp, const unsigned char* fmask,
-                                                   const typename Arith<S>::R* P, RefineGeom g, PopShift sh)
+__device__ __forceinline__ void interp_cf_node(long long o, S* ff, long long fqs, int fnxp, const unsigned char* fmask,
+                                               const typename Arith<S>::R* P, const RefineGeom& g, const PopShift& sh)
 {
     using R = typename Arith<S>::R;
     int i, j, k;
-    if (!frame_node(blockIdx.x * 256LL + threadIdx.x, g, i, j, k)) return;
+    if (!frame_node(o, g, i, j, k)) return;
     const int nfy = g.nf[1];
     const int fcell = i + fnxp * (j + nfy * k);
     if (fmask && is_solid(fmask[fcell])) return;
@@ -370,6 +375,13 @@ __global__ void __launch_bounds__(256) k_interp_cf(S* ff, long long fqs, int fnx
     for (int q = 0; q < Q; ++q) ff[q * fqs + fcell] = Store<S>::from_real(f[q], q, sh);
 }
 
+template <class S>
+__global__ void __launch_bounds__(256) k_interp_cf(S* ff, long long fqs, int fnxp, const unsigned char* fmask,
+                                                   const typename Arith<S>::R* P, RefineGeom g, PopShift sh)
+{
+    interp_cf_node<S>(blockIdx.x * 256LL + threadIdx.x, ff, fqs, fnxp, fmask, P, g, sh);
+}
+
 // Fine -> coarse.  One workgroup per tile of FC_TX x FC_TY x FC_TZ coarse
 // nodes: its 256 threads load the (2 FC_TX) x (2 FC_TY) x (2 FC_TZ) fine nodes
 // below them as contiguous 64-node rows (one wavefront per row and plane),
@@ -385,13 +397,13 @@ constexpr int FC_TX = 32, FC_TY = LBM_FC_TY, FC_TZ = LBM_FC_TZ;
 constexpr int FC_ROWS = 4 * FC_TY * FC_TZ; // fine rows of a tile (a multiple of the 4 wavefronts)
 
 template <class S>
-__global__ void __launch_bounds__(256) k_interp_fc(const S* ff, long long fqs, int fnxp, S* fc, long long cqs,
-                                                   int cnxp, int cny, const unsigned char* cmask, RefineGeom g,
-                                                   int nbx, int nby, PopShift sh)
+__device__ __forceinline__ void interp_fc_block(int blk, const S* ff, long long fqs, int fnxp, S* fc, long long cqs,
+                                                int cnxp, int cny, const unsigned char* cmask, const RefineGeom& g,
+                                                int nbx, int nby, const PopShift& sh)
 {
     using R = typename Arith<S>::R;
     __shared__ R fm[2 * FC_TZ][2 * FC_TY][2 * FC_TX][9]; // fine moments [fz][fy][fx][m]
-    const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
+    const int bx = blk % nbx, by = (blk / nbx) % nby, bz = blk / (nbx * nby);
     // coarse nodes a + 2 + (FC_T* b) + l; fine nodes 2(c - a) - 1 .. 2(c - a)
     const int fx0 = 2 * (2 + FC_TX * bx) - 1, fy0 = 2 * (2 + FC_TY * by) - 1, fz0 = 2 * (2 + FC_TZ * bz) - 1;
     const int lxf = threadIdx.x & 63, row0 = threadIdx.x >> 6; // 4 wavefronts
@@ -453,6 +465,35 @@ __global__ void __launch_bounds__(256) k_interp_fc(const S* ff, long long fqs, i
     invert_phys(f, avg, SF, g.eq_kind);
 #pragma unroll
     for (int q = 0; q < Q; ++q) fc[q * cqs + ccell] = Store<S>::from_real(f[q], q, sh);
+}
+
+template <class S>
+__global__ void __launch_bounds__(256) k_interp_fc(const S* ff, long long fqs, int fnxp, S* fc, long long cqs,
+                                                   int cnxp, int cny, const unsigned char* cmask, RefineGeom g,
+                                                   int nbx, int nby, PopShift sh)
+{
+    interp_fc_block<S>(blockIdx.x, ff, fqs, fnxp, fc, cqs, cnxp, cny, cmask, g, nbx, nby, sh);
+}
+
+// The restriction and the next step's first CF pass in one launch: the
+// first nb_cf workgroups fill the frame of the fine lattice ff (from the
+// coarse data P), the rest restrict ff's inner nodes onto the coarse
+// lattice.  CF writes only frame nodes (fine 0, 1, nf - 2, nf - 1 along an
+// axis) and reads only P; the restriction reads fine nodes 3 .. nf - 4 and
+// writes coarse ones, so the two sets of workgroups touch nothing in common
+// and the latency-bound CF pass runs beside the restriction instead of as a
+// launch of its own.
+template <class S>
+__global__ void __launch_bounds__(256) k_interp_fc_cf(const S* ff_in, S* ff, long long fqs, int fnxp,
+                                                      const unsigned char* fmask, const typename Arith<S>::R* P,
+                                                      S* fc, long long cqs, int cnxp, int cny,
+                                                      const unsigned char* cmask, RefineGeom g, int nbx, int nby,
+                                                      int nb_cf, PopShift sh)
+{
+    if ((int)blockIdx.x < nb_cf)
+        interp_cf_node<S>(blockIdx.x * 256LL + threadIdx.x, ff, fqs, fnxp, fmask, P, g, sh);
+    else
+        interp_fc_block<S>((int)blockIdx.x - nb_cf, ff_in, fqs, fnxp, fc, cqs, cnxp, cny, cmask, g, nbx, nby, sh);
 }
 
 } // namespace lbm
