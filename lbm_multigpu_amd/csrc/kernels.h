@@ -11,6 +11,7 @@
 #pragma once
 
 #include "collide.h"
+#include "fastdiv.h"
 #include "storage.h"
 
 namespace lbm {
@@ -83,35 +84,6 @@ __device__ __forceinline__ bool wave_node(int lb, int nsegw, int ny, int nz, int
     z = row / ny;
     return z < nz;
 }
-
-// Division by a wave-uniform divisor fixed at launch, as a multiply-high and a
-// shift on the scalar unit (Granlund and Montgomery, "Division by invariant
-// integers using multiplication", PLDI 1994, section 4: l = ceil(log2 d),
-// m = floor(2^32 (2^l - d) / d) + 1, n / d = (mulhi(m, n) + n) >> l with the
-// sum in 33 bits).  A plain division by a run-time value compiles to a
-// float-reciprocal sequence on the vector unit, several instructions per
-// wavefront in the 16-bit kernels, which are bound by the vector unit.
-struct FastDiv {
-    uint32_t d, m, l;
-    __host__ static FastDiv of(uint32_t d)
-    {
-        FastDiv f;
-        f.d = d;
-        f.l = 0;
-        while ((1ull << f.l) < d) ++f.l;
-        f.m = (uint32_t)((((1ull << f.l) - d) << 32) / d + 1);
-        return f;
-    }
-    __host__ __device__ uint32_t div(uint32_t n) const
-    {
-#ifdef __HIP_DEVICE_COMPILE__
-        const uint32_t t = __umulhi(m, n);
-#else
-        const uint32_t t = (uint32_t)(((uint64_t)m * n) >> 32);
-#endif
-        return (uint32_t)(((uint64_t)t + n) >> l);
-    }
-};
 
 // The stream-collide's wave-to-node map with its divisors precomputed:
 // segment index g -> row = g / nsegw, z = row / ny.

@@ -161,3 +161,35 @@ def test_full_length_storage_formats_against_fp64(workload):
             say(f"  {st} level {lev}: u L2 {l2:.2e}, Linf {linf:.2e}, mass {dm:.1e}")
             assert l2 <= l2b and linf <= linfb
         assert dm <= mb
+
+
+def _stated_channel_mask(nx, ny, nz, side, x_center):
+    """The channel's node codes built from its statement alone, [z][y][x]:
+    a square rod of side `side` cells spanning z, its x cells
+    x_center - side/2 .. x_center + side/2 - 1, centred in y; the velocity
+    inlet plane x = 0 and the outlet plane x = nx-1; halfway walls on the y
+    and z faces (no wall nodes), fluid everywhere else."""
+    m = np.full((nz, ny, nx), L.NODE_FLUID, np.uint8)
+    h = side // 2
+    y0 = ny // 2 - h  # centred: ny/2 - h .. ny/2 + h - 1 is symmetric about (ny-1)/2
+    m[:, y0:y0 + side, x_center - h:x_center + h] = L.NODE_SOLID_FORCE
+    m[:, :, 0] = L.NODE_INLET_WEST
+    m[:, :, nx - 1] = L.NODE_OUTLET_EAST
+    return m
+
+
+@pytest.mark.parametrize("dims", [(512, 128, 128, 16, 128), (256, 64, 64, 8, 64)])
+def test_channel_mask_is_the_stated_geometry(dims):
+    """The geometry classifier (lbm_set_geometry with no mask) on the bench's
+    channel and on the learned-closure workload's coarse channel, node for
+    node against the stated geometry."""
+    nx, ny, nz, side, xc = dims
+    say("channel mask", dims)
+    c = cfgmod.channel(nx=nx, ny=ny, nz=nz, side=side, x_center=xc).to_c()
+    with L.Solver(c) as s:
+        s.set_geometry(None)
+        got = s.get_mask()
+    exp = _stated_channel_mask(nx, ny, nz, side, xc)
+    bad = np.argwhere(got != exp)
+    assert bad.size == 0, f"{len(bad)} nodes differ, first [z,y,x] {bad[0]}: {got[tuple(bad[0])]} vs {exp[tuple(bad[0])]}"
+    assert (got == L.NODE_SOLID_FORCE).sum() == side * side * nz
