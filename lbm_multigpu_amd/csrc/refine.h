@@ -415,7 +415,10 @@ __device__ __forceinline__ void interp_fc_block(int blk, const S* ff, long long 
         R mom[9];
 #pragma unroll
         for (int m = 0; m < 9; ++m) mom[m] = R(0);
-        if (i < g.nf[0] && j < g.nf[1] && k < g.nf[2]) {
+        // only fine nodes 3 .. nf - 4 feed a restricted coarse node; the
+        // frame beyond them (which k_interp_fc_cf's CF workgroups may be
+        // writing in the same launch) is not read
+        if (i <= g.nf[0] - 4 && j <= g.nf[1] - 4 && k <= g.nf[2] - 4) {
             const int fcell = i + fnxp * (j + g.nf[1] * k);
             R f[Q];
 #pragma unroll
