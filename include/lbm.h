@@ -345,9 +345,16 @@ int lbm_set_closure_base(lbm_ctx* ctx, int smagorinsky);
  * fell back to (inputs outside the training range); a diagnostic of the
  * closure, which has no counterpart in the reference */
 int lbm_get_closure_rates(lbm_ctx* ctx, float* rates);
+/* what lbm_closure_samples takes as the eddy-viscosity target: 0 (the
+ * default) the filtered field's subfilter stress alone; 1 also the run's own
+ * Smagorinsky stress, filtered (the filtered equations of an LES carry both,
+ * so a coarser run's closure has to stand in for both).  No counterpart in
+ * the reference. */
+int lbm_set_closure_target(lbm_ctx* ctx, int with_model_stress);
 /* training samples from a context's current state, box-filtered by 2 (the
  * solver's own finer run): features X[n][7] and the SGS eddy viscosity
- * y[n] = max(0, -tau_dev:S/(2 S:S)) of a hashed fraction `keep` of the
+ * y[n] = max(0, -tau_dev:S/(2 S:S)) (with lbm_set_closure_target(ctx, 1):
+ * max(0, (-tau_dev:S + 2 <nu_f S_f>:S)/(2 S:S))) of a hashed fraction `keep` of the
  * filtered cells, in cell order.  When more than max_samples cells qualify
  * the call returns LBM_EINVAL with *count set to the number needed (a
  * truncated selection would depend on the order of the GPU's atomics). */

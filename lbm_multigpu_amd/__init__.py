@@ -75,7 +75,7 @@ EXPORTS = [
     "lbm_mlp_create", "lbm_mlp_destroy", "lbm_mlp_last_error", "lbm_mlp_num_params",
     "lbm_mlp_set_params", "lbm_mlp_get_params", "lbm_mlp_set_norm", "lbm_mlp_set_data",
     "lbm_mlp_gradient", "lbm_mlp_train", "lbm_mlp_forward", "lbm_set_closure",
-    "lbm_closure_samples", "lbm_stats_enable", "lbm_stats_get", "lbm_set_closure_interval",
+    "lbm_closure_samples", "lbm_set_closure_target", "lbm_stats_enable", "lbm_stats_get", "lbm_set_closure_interval",
     "lbm_get_inlet_control", "lbm_plane_mean_v", "lbm_get_dpdy", "lbm_mass_momentum_noalloc",
     "lbm_get_closure_rates", "lbm_set_closure_base",
 ]
@@ -148,6 +148,7 @@ def load_library(path=LIB_PATH):
     lib.lbm_set_closure.argtypes = [P, P, C.c_float]
     lib.lbm_set_closure_interval.argtypes = [P, C.c_int]
     lib.lbm_closure_samples.argtypes = [P, C.c_uint64, C.c_float, f32, f32, C.c_int, C.POINTER(C.c_int)]
+    lib.lbm_set_closure_target.argtypes = [P, C.c_int]
     lib.lbm_get_closure_rates.argtypes = [P, P]
     lib.lbm_set_closure_base.argtypes = [P, C.c_int]
     _lib = lib
@@ -337,6 +338,11 @@ class Solver:
         out = np.empty(self.shape(0), np.float32)
         self._check(self.lib.lbm_get_closure_rates(self.h, out.ctypes.data))
         return out
+
+    def set_closure_target(self, with_model_stress):
+        """closure_samples' target: the subfilter stress alone (False) or with
+        this run's own Smagorinsky stress, filtered (True)."""
+        self._check(self.lib.lbm_set_closure_target(self.h, int(bool(with_model_stress))))
 
     def closure_samples(self, seed, keep, max_samples):
         """Features X[n, 7] and eddy-viscosity targets y[n] from this run,

@@ -209,10 +209,19 @@ __global__ void __launch_bounds__(256, 2) k_mlp_forward(const float* X, long lon
 
 // Box filter of the fine run by 2: velocity, products and solid fraction of
 // the 8 fine nodes under coarse cell (X,Y,Z).  Output planes (coarse dense):
-// 0..2 u, 3..8 uu uv uw vv vw ww, 9 solid fraction.
+// 0..2 u, 3..8 uu uv uw vv vw ww, 9 solid fraction; with model = 1 also
+// 10..15 the fine run's own Smagorinsky stress over two, nu_f S_f (xx yy zz
+// xy xz yz), at each fine node from its stored post-collision populations:
+// Pi' = P - rho/3 I - j j is (1 - omega) times the pre-collision
+// non-equilibrium stress Pi that set the node's rate (les_omega: tau =
+// tau0/2 + sqrt(tau0^2 + 18 sqrt2 CS |Pi|)/2), so tau solves
+// tau = tau0/2 + sqrt(tau0^2 + 18 sqrt2 CS |Pi'| tau / |tau - 1|)/2 (a few
+// fixed-point steps from tau0; the trace of Pi', which relaxes at another
+// rate, is left out), nu_f = (tau - tau0)/3 and S_f = -3 Pi'_dev /
+// (2 rho (tau - 1)) (S = -3 Pi / (2 rho tau)).
 template <class S>
 __global__ void k_box_filter(const S* f, long long qs, int nxp, int nx, int ny, int nz, const unsigned char* mask,
-                             float* out, PopShift sh)
+                             float* out, PopShift sh, float omega0, float cs, int model)
 {
     using R = typename Arith<S>::R;
     const int cx = nx / 2, cy = ny / 2, cz = nz / 2;
@@ -220,35 +229,54 @@ __global__ void k_box_filter(const S* f, long long qs, int nxp, int nx, int ny, 
     const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (c >= NC) return;
     const int X = int(c % cx), Y = int((c / cx) % cy), Z = int(c / ((long long)cx * cy));
-    float acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    float acc[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const float tau0 = 1.f / omega0;
     for (int d = 0; d < 8; ++d) {
         const int x = 2 * X + (d & 1), y = 2 * Y + ((d >> 1) & 1), z = 2 * Z + ((d >> 2) & 1);
         const long long cell = x + (long long)nxp * (y + (long long)ny * z);
         if (is_solid(mask[cell])) { acc[9] += 1.f; continue; }
-        R rho = 0, u = 0, v = 0, w = 0;
+        R g[Q];
 #pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            const R val = Store<S>::to_real(f[i * qs + cell], i, sh);
-            rho += val;
-            if (CX[i]) u += R(CX[i]) * val;
-            if (CY[i]) v += R(CY[i]) * val;
-            if (CZ[i]) w += R(CZ[i]) * val;
-        }
-        const float U = float(u), V = float(v), W = float(w);
+        for (int i = 0; i < Q; ++i) g[i] = Store<S>::to_real(f[i * qs + cell], i, sh);
+        R sp[NP], dp[NP];
+        pair_split(g, sp, dp);
+        const Macro<R> m = macro_from_pairs(g[0], sp, dp);
+        const float U = float(m.jx), V = float(m.jy), W = float(m.jz);
         acc[0] += U; acc[1] += V; acc[2] += W;
         acc[3] += U * U; acc[4] += U * V; acc[5] += U * W; acc[6] += V * V; acc[7] += V * W; acc[8] += W * W;
+        if (model && cs > 0.f) {
+            const float third = float(m.rho) * (1.f / 3.f);
+            float pxx = float(m.pxx) - third - U * U, pyy = float(m.pyy) - third - V * V;
+            float pzz = float(m.pzz) - third - W * W;
+            const float pxy = float(m.pxy) - U * V, pxz = float(m.pxz) - U * W, pyz = float(m.pyz) - V * W;
+            const float tr = (pxx + pyy + pzz) * (1.f / 3.f);
+            pxx -= tr; pyy -= tr; pzz -= tr;
+            const float qpost = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.f * (pxy * pxy + pxz * pxz + pyz * pyz));
+            const float k = float(18.0 * 1.4142135623730951) * cs * qpost;
+            float tau = tau0;
+            for (int it = 0; it < 8; ++it)
+                tau = 0.5f * tau0 + 0.5f * sqrtf(tau0 * tau0 + k * tau / fmaxf(fabsf(tau - 1.f), 0.05f));
+            const float nuf = (tau - tau0) * (1.f / 3.f);
+            const float sc = nuf * -1.5f / (float(m.rho) * (tau - 1.f)); // nu_f S_f = sc Pi'_dev
+            acc[10] += sc * pxx; acc[11] += sc * pyy; acc[12] += sc * pzz;
+            acc[13] += sc * pxy; acc[14] += sc * pxz; acc[15] += sc * pyz;
+        }
     }
-#pragma unroll
-    for (int k = 0; k < 10; ++k) out[k * NC + c] = acc[k] * 0.125f;
+    const int nout = model ? 16 : 10;
+    for (int k = 0; k < nout; ++k) out[k * NC + c] = acc[k] * 0.125f;
 }
 
 // Features and target at coarse cell c from the box-filtered fields:
 // S from central differences of u (coarse cells), tau_ij = <u_i u_j> - <u_i><u_j>,
-// nu_t = max(0, -tau_dev:S / (2 S:S)).  Cells touching a solid are skipped;
-// selected cells are those whose hash(seed, c) falls under `keep`.
+// nu_t = max(0, -tau_dev:S / (2 S:S)).  With model = 1 the fine run's own
+// model stress, filtered, joins the subfilter stress the coarse closure has
+// to carry: nu_t = max(0, (-tau_dev:S + 2 <nu_f S_f>:S) / (2 S:S)) (the
+// filtered fine equations hold div(2 <nu_f S_f>) beside div(tau)).  Cells
+// touching a solid are skipped; selected cells are those whose hash(seed, c)
+// falls under `keep`.
 __global__ void k_closure_features(const float* bx, int cx, int cy, int cz, const float* dwall, uint64_t seed,
                                    uint32_t keep, float* X, float* y, unsigned int* cidx, unsigned long long* count,
-                                   unsigned long long cap)
+                                   unsigned long long cap, int model)
 {
     const long long NC = (long long)cx * cy * cz;
     const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -280,7 +308,11 @@ __global__ void k_closure_features(const float* bx, int cx, int cy, int cz, cons
     t[0] -= tr; t[3] -= tr; t[5] -= tr;
     const float ss = sxx * sxx + syy * syy + szz * szz + 2.f * (sxy * sxy + sxz * sxz + syz * syz);
     const float ts = t[0] * sxx + t[3] * syy + t[5] * szz + 2.f * (t[1] * sxy + t[2] * sxz + t[4] * syz);
-    const float nut = ss > 1e-20f ? fmaxf(0.f, -ts / (2.f * ss)) : 0.f;
+    float ms = 0.f;
+    if (model)
+        ms = at3(10, i, j, k) * sxx + at3(11, i, j, k) * syy + at3(12, i, j, k) * szz +
+             2.f * (at3(13, i, j, k) * sxy + at3(14, i, j, k) * sxz + at3(15, i, j, k) * syz);
+    const float nut = ss > 1e-20f ? fmaxf(0.f, (2.f * ms - ts) / (2.f * ss)) : 0.f;
     const unsigned long long slot = atomicAdd(count, 1ULL);
     if (slot >= cap) return;
     float x[MLP_IN];

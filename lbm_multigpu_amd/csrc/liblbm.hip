@@ -94,6 +94,7 @@ struct lbm_ctx {
     float nu_cap = 0.05f;
     int closure_every = 1;     // network refresh interval (steps)
     int closure_smag_base = 0; // the network's nu_t adds to the Smagorinsky tau (1) or to tau0 (0)
+    int closure_target = 0;    // lbm_closure_samples: 1 adds the run's own model stress to the target
     float dpdy = 0.f;          // body force as the inlet control leaves it
     std::vector<double> ctrl;  // control history, rows (t, Av_V, dpdy)
     long long closure_t0 = 0;  // step of the first refresh
@@ -2015,6 +2016,14 @@ extern "C" int lbm_set_closure_interval(lbm_ctx* ctx, int every)
 // Where the network's eddy viscosity goes: on top of tau0 (the plain
 // learned closure) or on top of the Smagorinsky relaxation time of the
 // configured CS (a learned correction to Smagorinsky).
+extern "C" int lbm_set_closure_target(lbm_ctx* ctx, int with_model_stress)
+{
+    if (!ctx || (with_model_stress != 0 && with_model_stress != 1))
+        return fail(ctx, LBM_EINVAL, "lbm_set_closure_target: bad argument");
+    ctx->closure_target = with_model_stress;
+    return LBM_OK;
+}
+
 extern "C" int lbm_set_closure_base(lbm_ctx* ctx, int smagorinsky)
 {
     if (!ctx) return fail(nullptr, LBM_EINVAL, "lbm_set_closure_base: null context");
@@ -2057,7 +2066,8 @@ extern "C" int lbm_closure_samples(lbm_ctx* ctx, uint64_t seed, float keep, floa
         for (void* q : {(void*)bx, (void*)dd, (void*)dX, (void*)dy, (void*)di, (void*)dcount})
             if (q) (void)hipFree(q);
     };
-    if (hipMalloc(&bx, NC * 10 * 4) || hipMalloc(&dd, NC * 4) || hipMalloc(&dX, (size_t)max_samples * MLP_IN * 4) ||
+    const int model = ctx->closure_target && ctx->cfg.cs > 0.f;
+    if (hipMalloc(&bx, NC * (model ? 16 : 10) * 4) || hipMalloc(&dd, NC * 4) || hipMalloc(&dX, (size_t)max_samples * MLP_IN * 4) ||
         hipMalloc(&dy, (size_t)max_samples * 4) || hipMalloc(&di, (size_t)max_samples * 4) || hipMalloc(&dcount, 8)) {
         cleanup();
         return fail(ctx, LBM_ENOMEM, "lbm_closure_samples: device allocation failed");
@@ -2068,12 +2078,13 @@ extern "C" int lbm_closure_samples(lbm_ctx* ctx, uint64_t seed, float keep, floa
         with_storage(L.storage, [&](auto* tag) {
             using S = std::remove_pointer_t<decltype(tag)>;
             hipLaunchKernelGGL(k_box_filter<S>, dim3((unsigned)((NC + 255) / 256)), dim3(256), 0, ctx->stream,
-                               (const S*)L.fa, L.qs, L.nxp, L.nx, L.ny, L.nz, L.mask, bx, L.shift);
+                               (const S*)L.fa, L.qs, L.nxp, L.nx, L.ny, L.nz, L.mask, bx, L.shift, ctx->cfg.omega,
+                               ctx->cfg.cs, model);
             return 0;
         });
         const uint32_t kp = (uint32_t)std::min(4294967295.0, std::max(0.0, (double)keep * 4294967296.0));
         hipLaunchKernelGGL(k_closure_features, dim3((unsigned)((NC + 255) / 256)), dim3(256), 0, ctx->stream, bx, cx, cy,
-                           cz, dd, seed, kp, dX, dy, di, dcount, (unsigned long long)max_samples);
+                           cz, dd, seed, kp, dX, dy, di, dcount, (unsigned long long)max_samples, model);
         e = hipGetLastError();
     }
     unsigned long long n = 0;

@@ -13,7 +13,11 @@ the CPU yardstick the HIP kernels are checked against:
   matrix cores' FP32 accumulation);
 * ``features``: the seven closure inputs from a strain-rate tensor, wall
   distance and speed, checked against closed forms in the same test file;
-* ``wall_distance``: the two-pass 3-D chamfer distance the library uses.
+* ``wall_distance``: the two-pass 3-D chamfer distance the library uses;
+* ``model_stress``: an LES node's Smagorinsky viscosity and strain rate
+  recovered from its stored post-collision populations (the sampler's
+  consistent target), pinned in tests/test_closure_target.py against the
+  oracle's own collide.
 
 Parameter layout (flat FP32, 8897): W1[7][64] b1[64] W2[64][64] b2[64]
 W3[64][64] b3[64] w4[64] b4.
@@ -213,6 +217,38 @@ def wall_distance(solid, wall_axes):
                             v = min(v, np.float32(d[Z, Y, X] + w[abs(dx) + abs(dy) + abs(dz)]))
                     d[z, y, x] = v
     return d
+
+
+D3Q19_C = np.array([(0, 0, 0), (1, 0, 0), (0, 1, 0), (-1, 0, 0), (0, -1, 0), (1, 1, 0), (-1, 1, 0), (-1, -1, 0),
+                    (1, -1, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1), (-1, 0, 1), (0, -1, 1), (0, 0, -1), (1, 0, -1),
+                    (0, 1, -1), (-1, 0, -1), (0, -1, -1)], np.float64)
+
+
+def model_stress(f, omega0, cs, iters=8):
+    """(nu_f, S_f) of LES nodes from their post-collision populations
+    f[19, ...] (FP64): Pi' = P - rho/3 I - j j is (1 - omega) times the
+    pre-collision non-equilibrium stress Pi that set the node's rate through
+    tau = tau0/2 + sqrt(tau0^2 + 18 sqrt2 cs |Pi|)/2, so tau solves
+    tau = tau0/2 + sqrt(tau0^2 + 18 sqrt2 cs |Pi'_dev| tau/|tau - 1|)/2
+    (fixed point from tau0; |tau - 1| floored at 0.05; the trace of Pi',
+    relaxed at another rate, left out); nu_f = (tau - tau0)/3 and
+    S_f = -3 Pi'_dev / (2 rho (tau - 1))."""
+    f = np.asarray(f, np.float64)
+    rho = f.sum(0)
+    j = np.einsum("ia,i...->a...", D3Q19_C, f)
+    P = np.einsum("ia,ib,i...->ab...", D3Q19_C, D3Q19_C, f)
+    I = np.eye(3).reshape((3, 3) + (1,) * (f.ndim - 1))
+    Pi = P - rho / 3.0 * I - j[:, None] * j[None, :]
+    Pi = Pi - np.trace(Pi) / 3.0 * I
+    q = np.sqrt((Pi * Pi).sum(axis=(0, 1)))
+    tau0 = 1.0 / omega0
+    k = 18.0 * math.sqrt(2.0) * cs * q
+    tau = np.full_like(q, tau0)
+    for _ in range(iters):
+        tau = 0.5 * tau0 + 0.5 * np.sqrt(tau0 * tau0 + k * tau / np.maximum(np.abs(tau - 1.0), 0.05))
+    nu = (tau - tau0) / 3.0
+    S = -1.5 * Pi / (rho * (tau - 1.0))
+    return nu, S
 
 
 def hash_u32(seed, counter):

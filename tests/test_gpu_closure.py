@@ -462,9 +462,11 @@ def test_closure_rejects_unsupported_contexts():
         m.gradient(50, 60)               # range beyond the data
 
 
-def _sampler_reference(f, mask, seed, keep):
+def _sampler_reference(f, mask, seed, keep, model=None):
     """Box filter by 2, central differences and the eddy-viscosity target,
-    restated in FP64 from the populations and the node mask."""
+    restated in FP64 from the populations and the node mask; model =
+    (omega0, cs) adds the run's own filtered model stress to the target
+    (mlp_ref.model_stress per fine node)."""
     Cv = CV
     solid = np.isin(mask, [L.NODE_SOLID, L.NODE_SOLID_FORCE])
     j = np.einsum("ia,i...->a...", Cv, f.astype(np.float64))
@@ -474,6 +476,13 @@ def _sampler_reference(f, mask, seed, keep):
 
     def box(a):
         return a.reshape(cz, 2, cy, 2, cx, 2).mean(axis=(1, 3, 5))
+
+    mb = None
+    if model is not None:
+        nu, Sf = M.model_stress(f.astype(np.float64), *model)
+        m = nu[None, None] * Sf
+        m[:, :, solid] = 0
+        mb = np.array([[box(m[a, b]) for b in range(3)] for a in range(3)])
 
     ub = np.stack([box(j[a]) for a in range(3)])
     uu = {(a, b): box(j[a] * j[b]) for a in range(3) for b in range(a, 3)}
@@ -499,25 +508,64 @@ def _sampler_reference(f, mask, seed, keep):
                 t = np.array([[uu[(min(a, b), max(a, b))][k, jj, i] - u[a] * u[b] for b in range(3)] for a in range(3)])
                 t -= np.trace(t) / 3 * np.eye(3)
                 ss = (S * S).sum()
-                nut = max(0.0, -(t * S).sum() / (2 * ss)) if ss > 1e-20 else 0.0
+                ms = (mb[:, :, k, jj, i] * S).sum() if mb is not None else 0.0
+                nut = max(0.0, (2 * ms - (t * S).sum()) / (2 * ss)) if ss > 1e-20 else 0.0
                 X.append(M.features(S, dc[k, jj, i], np.linalg.norm(u)))
                 y.append(nut)
                 cells.append(c)
     return np.array(X), np.array(y), np.array(cells)
 
 
-def test_sampler_matches_reference():
+def _les_state(cfg, shape, amp=0.02):
+    """Post-collision populations of a smooth LES flow: u = U + a sin/cos
+    modes, the non-equilibrium stress Pi = -(2 tau0/3) S of its strain rate
+    (second-order Hermite part), collided node by node by the oracle's LES
+    collide, so every node's stored stress carries the rate its collide
+    applied (the sampler's model stress then matters: with random
+    populations it does not)."""
+    nz, ny, nx = shape
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    k = 2 * np.pi
+    U = np.stack([cfg.umax + amp * np.sin(k * y / ny) * np.cos(k * z / nz),
+                  amp * np.sin(k * x / nx) * np.cos(k * z / nz),
+                  amp * np.cos(k * x / nx) * np.sin(k * y / ny)])
+    g = np.stack([np.stack(np.gradient(U[a], axis=(2, 1, 0))) for a in range(3)])  # d u_a / d x_b
+    S = 0.5 * (g + g.transpose(1, 0, 2, 3, 4))
+    eye = np.eye(3)[:, :, None, None, None]
+    Pi = -(2.0 / (3.0 * cfg.omega)) * S
+    Pi = Pi - np.trace(Pi) / 3.0 * eye
+    C = M.D3Q19_C
+    w = np.array([12, 2, 2, 2, 2, 1, 1, 1, 1, 2, 1, 1, 1, 1, 2, 1, 1, 1, 1], np.float64) / 36.0
+    cu = np.einsum("ia,a...->i...", C, U)
+    feq = w[:, None, None, None] * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * (U * U).sum(0))
+    H = np.einsum("ia,ib->iab", C, C) - np.eye(3) / 3.0
+    f = (feq + w[:, None, None, None] * 4.5 * np.einsum("iab,ab...->i...", H, Pi)).astype(np.float32)
+    p = orc_params(cfg.to_c())
+    for idx in np.ndindex(nz, ny, nx):
+        f[(slice(None),) + idx] = O.collide(p, np.ascontiguousarray(f[(slice(None),) + idx]))
+    return f
+
+
+@pytest.mark.parametrize("with_model", [False, True])
+def test_sampler_matches_reference(with_model):
     cfg = cfgmod.channel(nx=32, ny=16, nz=16, side=4, x_center=10, tau=0.52, u0=0.05, seed=3)
-    say("closure sampler")
+    say("closure sampler, model stress" if with_model else "closure sampler")
     rng = np.random.default_rng(8)
     with L.Solver(cfg) as s:
         s.set_geometry()
         mask = s.get_mask()
-        f = random_state(rng, mask.shape, amp=2e-3, umax=0.04)
+        f = _les_state(cfg, mask.shape) if with_model else random_state(rng, mask.shape, amp=2e-3, umax=0.04)
         s.set_populations(f)
+        s.set_closure_target(with_model)
         keep = 0.6
         X, y = s.closure_samples(seed=17, keep=keep, max_samples=4096)
-    Xr, yr, cells = _sampler_reference(f, mask, 17, int(keep * 2 ** 32))
+    model = (float(np.float32(cfg.omega)), float(np.float32(cfg.cs))) if with_model else None
+    Xr, yr, cells = _sampler_reference(f, mask, 17, int(keep * 2 ** 32), model)
+    if with_model:
+        _, y0, _ = _sampler_reference(f, mask, 17, int(keep * 2 ** 32))
+        say("  model stress moves the target by", float(np.abs(yr - y0).max() / np.abs(y0).max()),
+            "of its maximum; mean", float(y0.mean()), "->", float(yr.mean()))
+        assert np.abs(yr - y0).max() > 0.05 * np.abs(y0).max()
     say("  samples", len(y), len(yr))
     assert len(y) == len(yr) and len(y) > 50
     assert np.abs(X[:, 0] - Xr[:, 0]).max() < 1e-4
