@@ -351,6 +351,9 @@ int lbm_get_closure_rates(lbm_ctx* ctx, float* rates);
  * so a coarser run's closure has to stand in for both).  No counterpart in
  * the reference. */
 int lbm_set_closure_target(lbm_ctx* ctx, int with_model_stress);
+/* (with 1, lbm_closure_samples refuses runs whose tau0 lies within 0.1 of 1:
+ * the stored post-collision stress, (1 - omega) times the one that set a
+ * node's rate, then no longer shows that rate) */
 /* training samples from a context's current state, box-filtered by 2 (the
  * solver's own finer run): features X[n][7] and the SGS eddy viscosity
  * y[n] = max(0, -tau_dev:S/(2 S:S)) (with lbm_set_closure_target(ctx, 1):

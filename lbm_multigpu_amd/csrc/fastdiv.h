@@ -21,6 +21,7 @@ namespace lbm {
 // wavefront in the 16-bit kernels, which are bound by the vector unit.
 struct FastDiv {
     uint32_t d, m, l;
+    // d >= 1 (the callers' divisors are row and segment counts)
     static FastDiv of(uint32_t d)
     {
         FastDiv f;

@@ -2067,6 +2067,10 @@ extern "C" int lbm_closure_samples(lbm_ctx* ctx, uint64_t seed, float keep, floa
             if (q) (void)hipFree(q);
     };
     const int model = ctx->closure_target && ctx->cfg.cs > 0.f;
+    // the post-collision stress is (1 - omega) times the one that set a
+    // node's rate: near tau = 1 it no longer shows the rate
+    if (model && std::fabs(1.f / ctx->cfg.omega - 1.f) < 0.1f)
+        return fail(ctx, LBM_EINVAL, "lbm_closure_samples: the model-stress target needs |tau - 1| >= 0.1");
     if (hipMalloc(&bx, NC * (model ? 16 : 10) * 4) || hipMalloc(&dd, NC * 4) || hipMalloc(&dX, (size_t)max_samples * MLP_IN * 4) ||
         hipMalloc(&dy, (size_t)max_samples * 4) || hipMalloc(&di, (size_t)max_samples * 4) || hipMalloc(&dcount, 8)) {
         cleanup();
