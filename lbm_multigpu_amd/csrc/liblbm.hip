@@ -14,6 +14,7 @@
 #include <fstream>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/lbm.h"
@@ -819,6 +820,7 @@ static StepParams step_params(lbm_ctx* ctx, const Level& L, float omega)
     // next step's loads can hit it (128x96x96 FP32: 31 vs 25 GLUPS); beyond
     // it, nontemporal stores (512x128x128 FP32: 35.3 vs 32.6 GLUPS)
     p.nt_store = 2 * Q * L.qs * (long long)L.esize > (256LL << 20);
+    p.whole_ok = Q * L.qs * (long long)L.esize < (4LL << 30);
     p.stat = L.stat;
     p.stat_mode = 0;
     p.stat_n = 0.f;
@@ -832,16 +834,32 @@ static StepParams step_params(lbm_ctx* ctx, const Level& L, float omega)
     return p;
 }
 
-template <class S, bool NT>
-static void launch_sc_policy(int model, const StepParams& p, int blocks, hipStream_t s)
+template <class S, bool NT, bool WHOLE>
+static void launch_sc_kernel(int model, const StepParams& p, int blocks, hipStream_t s)
 {
     const dim3 blk(Shape<S>::THREADS), grd(blocks);
     switch (model) {
-    case LBM_MODEL_BGK: hipLaunchKernelGGL((k_stream_collide<MODEL_BGK, S, NT>), grd, blk, 0, s, p); break;
-    case LBM_MODEL_MRT_CPU3D: hipLaunchKernelGGL((k_stream_collide<MODEL_MRT_CPU3D, S, NT>), grd, blk, 0, s, p); break;
-    case LBM_MODEL_MRT_LES: hipLaunchKernelGGL((k_stream_collide<MODEL_MRT_LES, S, NT>), grd, blk, 0, s, p); break;
-    default: hipLaunchKernelGGL((k_stream_collide<MODEL_MRT_DHUM, S, NT>), grd, blk, 0, s, p); break;
+    case LBM_MODEL_BGK: hipLaunchKernelGGL((k_stream_collide<MODEL_BGK, S, NT, WHOLE>), grd, blk, 0, s, p); break;
+    case LBM_MODEL_MRT_CPU3D:
+        hipLaunchKernelGGL((k_stream_collide<MODEL_MRT_CPU3D, S, NT, WHOLE>), grd, blk, 0, s, p);
+        break;
+    case LBM_MODEL_MRT_LES: hipLaunchKernelGGL((k_stream_collide<MODEL_MRT_LES, S, NT, WHOLE>), grd, blk, 0, s, p); break;
+    default: hipLaunchKernelGGL((k_stream_collide<MODEL_MRT_DHUM, S, NT, WHOLE>), grd, blk, 0, s, p); break;
     }
+}
+
+// The whole-lattice load descriptor serves FP32 lattices below 4 GiB (it
+// measured no gain for the other formats).
+template <class S, bool NT>
+static void launch_sc_policy(int model, const StepParams& p, int blocks, hipStream_t s)
+{
+    if constexpr (std::is_same_v<S, float>) {
+        if (p.whole_ok) {
+            launch_sc_kernel<S, NT, true>(model, p, blocks, s);
+            return;
+        }
+    }
+    launch_sc_kernel<S, NT, false>(model, p, blocks, s);
 }
 
 template <class S>

@@ -102,6 +102,11 @@ template <>
 struct Store<float> {
     using Raw = float;
     __device__ static __forceinline__ Raw load_raw(const float* p) { return ld(p); }
+    // buffer load at byte offset voff + soff of the descriptor's range
+    __device__ static __forceinline__ Raw load_buf(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+    }
     __device__ static __forceinline__ float decode(Raw r, int, const PopShift&) { return r; }
     __device__ static __forceinline__ float decode_shift(Raw r, float) { return r; }
     // the value a store of v leaves in memory
