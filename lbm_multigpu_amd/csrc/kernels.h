@@ -730,13 +730,13 @@ __global__ void __launch_bounds__(Shape<S>::THREADS, Shape<S>::MIN_WAVES) k_stre
 // second stream and no cross-stream wait.  The store policy is each level's
 // own, chosen at run time (p.nt_store; as template arguments it measured
 // 0.5 % faster for 2.5 times the code).
-template <int MODEL, class S>
+template <int MODEL, class S, bool WHOLE = false>
 __global__ void __launch_bounds__(Shape<S>::THREADS, Shape<S>::MIN_WAVES)
     k_stream_collide_two(StepParams a, StepParams b, int blocks_a)
 {
     const int hw = blockIdx.x;
-    if (hw < blocks_a) sc_block<MODEL, S, -1>(a, logical_block_map(hw, a.map));
-    else sc_block<MODEL, S, -1>(b, hw - blocks_a);
+    if (hw < blocks_a) sc_block<MODEL, S, -1, WHOLE>(a, logical_block_map(hw, a.map));
+    else sc_block<MODEL, S, -1, WHOLE>(b, hw - blocks_a);
 }
 
 // Deterministic FP64 reduction of ncomp-component partials (fixed order:

@@ -870,21 +870,38 @@ static void launch_sc(int model, const StepParams& p, int blocks, hipStream_t s)
 }
 
 // Level a (blocks_a workgroups) and level b (blocks_b) in one launch.
+template <class S, bool WHOLE>
+static void launch_sc_two_kernel(int model, const StepParams& a, int blocks_a, const StepParams& b, int blocks_b,
+                                 hipStream_t s)
+{
+    const dim3 blk(Shape<S>::THREADS), grd(blocks_a + blocks_b);
+    switch (model) {
+    case LBM_MODEL_BGK:
+        hipLaunchKernelGGL((k_stream_collide_two<MODEL_BGK, S, WHOLE>), grd, blk, 0, s, a, b, blocks_a);
+        break;
+    case LBM_MODEL_MRT_CPU3D:
+        hipLaunchKernelGGL((k_stream_collide_two<MODEL_MRT_CPU3D, S, WHOLE>), grd, blk, 0, s, a, b, blocks_a);
+        break;
+    case LBM_MODEL_MRT_LES:
+        hipLaunchKernelGGL((k_stream_collide_two<MODEL_MRT_LES, S, WHOLE>), grd, blk, 0, s, a, b, blocks_a);
+        break;
+    default:
+        hipLaunchKernelGGL((k_stream_collide_two<MODEL_MRT_DHUM, S, WHOLE>), grd, blk, 0, s, a, b, blocks_a);
+        break;
+    }
+}
+
 template <class S>
 static void launch_sc_two(int model, const StepParams& a, int blocks_a, const StepParams& b, int blocks_b,
                           hipStream_t s)
 {
-    const dim3 blk(Shape<S>::THREADS), grd(blocks_a + blocks_b);
-    switch (model) {
-    case LBM_MODEL_BGK: hipLaunchKernelGGL((k_stream_collide_two<MODEL_BGK, S>), grd, blk, 0, s, a, b, blocks_a); break;
-    case LBM_MODEL_MRT_CPU3D:
-        hipLaunchKernelGGL((k_stream_collide_two<MODEL_MRT_CPU3D, S>), grd, blk, 0, s, a, b, blocks_a);
-        break;
-    case LBM_MODEL_MRT_LES:
-        hipLaunchKernelGGL((k_stream_collide_two<MODEL_MRT_LES, S>), grd, blk, 0, s, a, b, blocks_a);
-        break;
-    default: hipLaunchKernelGGL((k_stream_collide_two<MODEL_MRT_DHUM, S>), grd, blk, 0, s, a, b, blocks_a); break;
+    if constexpr (std::is_same_v<S, float>) {
+        if (a.whole_ok && b.whole_ok) {
+            launch_sc_two_kernel<S, true>(model, a, blocks_a, b, blocks_b, s);
+            return;
+        }
     }
+    launch_sc_two_kernel<S, false>(model, a, blocks_a, b, blocks_b, s);
 }
 
 // Launches the fused closure collide over level L; returns its workgroup
