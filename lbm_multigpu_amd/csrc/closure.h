@@ -210,8 +210,9 @@ __global__ void __launch_bounds__(256, 2) k_mlp_forward(const float* X, long lon
 // Box filter of the fine run by 2: velocity, products and solid fraction of
 // the 8 fine nodes under coarse cell (X,Y,Z).  Output planes (coarse dense):
 // 0..2 u, 3..8 uu uv uw vv vw ww, 9 solid fraction; with model = 1 also
-// 10..15 the fine run's own Smagorinsky stress over two, nu_f S_f (xx yy zz
-// xy xz yz), at each fine node from its stored post-collision populations:
+// 10..15 nu_f S_f (xx yy zz xy xz yz; minus half the fine run's own
+// Smagorinsky stress), at each fine node from its stored post-collision
+// populations:
 // Pi' = P - rho/3 I - j j is (1 - omega) times the pre-collision
 // non-equilibrium stress Pi that set the node's rate (les_omega: tau =
 // tau0/2 + sqrt(tau0^2 + 18 sqrt2 CS |Pi|)/2), so tau solves
