@@ -40,8 +40,8 @@ enum Bc : int { BC_PERIODIC = 0, BC_CLAMP = 1, BC_WALL = 2 };
 
 // Workgroup shape per population type: NW wavefronts of 64 lanes on NW
 // consecutive row segments, one node per lane.  FP32 streams best with four
-// (one contiguous 1 KB stretch per plane and workgroup); the VALU-bound
-// 16-bit kernels run best with one (measured on the 512x128x128 channel:
+// (one contiguous 1 KB stretch per plane and workgroup); the 16-bit
+// kernels, with twice the vector work per byte, run best with one (measured on the 512x128x128 channel:
 // 35.1 vs 32.2 GLUPS FP32, 46.2 vs 41.7 GLUPS FP16).  The 16-bit kernels
 // fit 64 VGPRs without spilling, so they are held to 8 wavefronts per SIMD
 // (56.3 vs 54.4 GLUPS at 7); the FP32 kernel at 8 would spill.
