@@ -21,6 +21,17 @@ def header_functions():
     return sorted(set(re.findall(r"\b(lbm_[a-z0-9_]+)\s*\(", txt)))
 
 
+def test_missing_library_fails_loudly():
+    """With no built library the binding raises; nothing falls back to a CPU path."""
+    saved = L._lib
+    L._lib = None
+    try:
+        with pytest.raises(OSError, match="not built"):
+            L.load_library(os.path.join(ROOT, "no_such_dir", "liblbm.so"))
+    finally:
+        L._lib = saved
+
+
 def test_library_exports_every_declared_symbol():
     lib = L.load_library()
     names = header_functions()
