@@ -417,17 +417,20 @@ def main():
     import lbm_multigpu_amd as L
     from lbm_multigpu_amd import config as C
 
-    def make_cfg(storage):
-        if a.workload == "refined":
+    def make_cfg(storage, workload=None):
+        workload = workload or a.workload
+        if workload == "refined":
             return C.refined_channel(storage=storage)
-        if a.workload == "closure":
+        if workload == "closure":
             return C.channel(nx=256, ny=64, nz=64, side=8, x_center=64, tau=0.51, storage=storage)
         return C.channel(storage=storage)
 
-    def measure(cfg, forces, warmup=None):
+    def measure(cfg, forces, warmup=None, workload=None, closure_every=None):
         """(ms per step from HIP events on the library's stream, wall s, FP64
         mass sum, fine-level sizes or None)"""
         warmup = a.warmup if warmup is None else warmup
+        workload = workload or a.workload
+        closure_every = a.closure_every if closure_every is None else closure_every
         if forces:
             # record the momentum-exchange force of every step, as the
             # reference does from STARTF on
@@ -437,21 +440,21 @@ def main():
             s.set_geometry(None)
             s.init_equilibrium(L.INIT_BGK)
             mlp = None
-            if a.workload == "closure":
+            if workload == "closure":
                 # a seeded network over the whole input range: every fluid node
                 # runs the network (the cost does not depend on the weights)
                 import numpy as np
                 mlp = L.Mlp(seed=1)
                 big = np.full(7, 3e38, np.float32)
                 mlp.set_norm(np.zeros(7, np.float32), np.ones(7, np.float32), -big, big, 1e-4)
-                s.set_closure(mlp, every=a.closure_every)
+                s.set_closure(mlp, every=closure_every)
             s.step(warmup)
             s.step_timed(1)  # drains the warm-up
             t0 = time.perf_counter()
             ms = s.step_timed(a.steps)
             wall = time.perf_counter() - t0
             mm = s.mass_momentum()
-            fine = s.sizes(1) if a.workload == "refined" else None
+            fine = s.sizes(1) if workload == "refined" else None
             if forces:
                 assert len(s.forces()) >= a.steps
         return ms / a.steps, wall, mm, fine
@@ -470,22 +473,41 @@ def main():
         ach = BYTES_PER_NODE[a.storage] * updates / (ms_step * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic["bytes_per_launch"] if traffic else None}
-    # the 16-bit storage variant of the same workload, measured in the same
-    # run as a secondary figure (76 algorithmic bytes per node update)
-    secondary = None
+    # the 16-bit storage variants of the same workload and the refinement and
+    # closure workloads, measured in the same run as secondary figures
+    secondary = secondary_wl = None
     if a.workload == "channel" and a.storage == "f32" and not a.no_secondary and not a.forces:
         sec = {}
-        for st in ("f16",):
-            # past the start-up transient: the 16-bit kernels, bound by the
-            # vector ALUs, run up to 25 % slower over the first ~150 steps
-            # from the uniform start (127 us/step after it on the 512x128x128
-            # channel, 133-160 us within it); FP32 does not vary
+        for st in ("f16", "bf16"):
+            # past the start-up transient: the 16-bit kernels run up to 25 %
+            # slower over the first ~150 steps from the uniform start (127
+            # us/step after it on the 512x128x128 channel, 133-160 us within
+            # it); FP32 does not vary
             ms2, _, _, _ = measure(make_cfg(st), False, warmup=max(a.warmup, 200))
             g2 = cells / (ms2 * 1e-3) / 1e9
             ach2 = BYTES_PER_NODE[st] * cells / (ms2 * 1e-3) / 1e9
             sec[st] = {"value": round(g2, 3), "unit": "GLUPS", "ms_per_step": round(ms2, 4),
                        "roofline_frac": round(ach2 / HBM_PEAK_GBS, 4)}
         secondary = sec
+        wl = {}
+        # refined channel: coarse plus both fine sub-steps per coarse step
+        rc = make_cfg("f32", "refined")
+        ms3, _, _, f3 = measure(rc, False, workload="refined")
+        up3 = rc.nx * rc.ny * rc.nz + 2 * f3.nx * f3.ny * f3.nz
+        wl["refined"] = {"workload": WORKLOADS["refined"].format(c=rc), "value": round(up3 / (ms3 * 1e-3) / 1e9, 3),
+                         "unit": "GLUPS", "ms_per_step": round(ms3, 4),
+                         "roofline_frac": round(BYTES_PER_NODE["f32"] * up3 / (ms3 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        # learned closure fused into the collide, every step and every 10 steps
+        cc = make_cfg("f32", "closure")
+        n4 = cc.nx * cc.ny * cc.nz
+        for every in (1, 10):
+            ms4, _, _, _ = measure(make_cfg("f32", "closure"), False, workload="closure", closure_every=every)
+            ach4 = MLP_FLOP_PER_NODE * n4 / every / (ms4 * 1e-3) / 1e12
+            wl[f"closure_every_{every}"] = {"workload": WORKLOADS["closure"].format(c=cc), "closure_every": every,
+                                            "value": round(n4 / (ms4 * 1e-3) / 1e9, 3), "unit": "GLUPS",
+                                            "ms_per_step": round(ms4, 4),
+                                            "mfma_frac": round(ach4 / MFMA_BF16_PEAK_TFLOPS, 4)}
+        secondary_wl = wl
     cpu = None if (a.no_cpu or a.workload != "channel") else cpu_baseline(cfg)
     line = {
         "metric": METRIC, "value": round(glups, 3), "unit": "GLUPS", "n_gpus": 1, "steps": a.steps,
@@ -498,6 +520,8 @@ def main():
     }
     if secondary:
         line["secondary_storage"] = secondary
+    if secondary_wl:
+        line["secondary_workloads"] = secondary_wl
     if traffic or traffic_err:
         line["traffic_counters"] = traffic if traffic else {"error": traffic_err}
     if a.forces:
