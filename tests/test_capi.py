@@ -32,6 +32,12 @@ def test_missing_library_fails_loudly():
         L._lib = saved
 
 
+def test_closure_setters_reject_a_null_context():
+    lib = L.load_library()
+    assert lib.lbm_set_closure_target(None, 1) == -22
+    assert lib.lbm_set_closure_base(None, 1) == -22
+
+
 def test_library_exports_every_declared_symbol():
     lib = L.load_library()
     names = header_functions()

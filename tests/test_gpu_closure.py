@@ -574,6 +574,25 @@ def test_sampler_matches_reference(with_model):
     assert np.abs(y - yr).max() < 1e-3 * np.abs(yr).max() + 1e-9
 
 
+def test_model_stress_target_refused_near_tau_one():
+    """The post-collision stress is (1 - omega) times the one that set a
+    node's rate, so at |tau - 1| < 0.1 it no longer shows the rate and the
+    model-stress target is refused; the plain target still samples there,
+    and a flag other than 0 or 1 is rejected."""
+    cfg = cfgmod.channel(nx=32, ny=16, nz=16, side=4, x_center=10, tau=1.05, u0=0.05, seed=3)
+    say("closure target refusal at tau 1.05")
+    with L.Solver(cfg) as s:
+        s.set_geometry()
+        s.init_equilibrium(L.INIT_BGK)
+        assert s.lib.lbm_set_closure_target(s.h, 2) == -22
+        s.set_closure_target(True)
+        with pytest.raises(L.LBMError, match=r"\|tau - 1\|"):
+            s.closure_samples(seed=1, keep=0.5, max_samples=64)
+        s.set_closure_target(False)
+        X, y = s.closure_samples(seed=1, keep=0.5, max_samples=4096)
+    assert X.shape == (len(y), 7)
+
+
 def test_trained_closure_beats_smagorinsky_apriori():
     """End to end at a quarter of the evaluation grids: a 128x32x32 fine run
     generates box-filtered samples, the network trains on the GPU, and on
