@@ -574,6 +574,29 @@ def test_sampler_matches_reference(with_model):
     assert np.abs(y - yr).max() < 1e-3 * np.abs(yr).max() + 1e-9
 
 
+def test_sampler_refuses_a_short_buffer_with_the_count():
+    """More qualifying cells than max_samples: the call is refused with the
+    number needed (the kept subset would depend on atomic order), and the
+    binding's refetch at that size returns the same samples as a large buffer."""
+    import ctypes
+    cfg = cfgmod.channel(nx=32, ny=16, nz=16, side=4, x_center=10, tau=0.52, u0=0.05, seed=3)
+    say("closure sampler, short buffer")
+    rng = np.random.default_rng(8)
+    with L.Solver(cfg) as s:
+        s.set_geometry()
+        s.set_populations(random_state(rng, s.get_mask().shape, amp=2e-3, umax=0.04))
+        X, y = s.closure_samples(seed=17, keep=0.6, max_samples=4096)
+        n = len(y)
+        assert n > 50
+        Xs = np.empty((n // 2, 7), np.float32)
+        ys = np.empty(n // 2, np.float32)
+        cnt = ctypes.c_int()
+        rc = s.lib.lbm_closure_samples(s.h, 17, 0.6, Xs.reshape(-1), ys, n // 2, ctypes.byref(cnt))
+        assert rc == -22 and cnt.value == n
+        X2, y2 = s.closure_samples(seed=17, keep=0.6, max_samples=8)
+    assert np.array_equal(X2, X) and np.array_equal(y2, y)
+
+
 def test_model_stress_target_refused_near_tau_one():
     """The post-collision stress is (1 - omega) times the one that set a
     node's rate, so at |tau - 1| < 0.1 it no longer shows the rate and the
