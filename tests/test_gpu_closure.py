@@ -413,6 +413,30 @@ def test_closure_out_of_range_is_smagorinsky():
     assert np.abs(g - h).max() <= 1e-6 * np.abs(h).max()
 
 
+def test_closure_rates_report_network_and_fallback():
+    """lbm_get_closure_rates after one closure step: with every input inside
+    the training range each node reports the network's rate, 1/(1/omega +
+    3 nu_t) in (0, omega]; with every input outside it, minus the Smagorinsky
+    rate, in [-omega, 0)."""
+    c = _closure_case()
+    say("closure rates")
+    rng = np.random.default_rng(6)
+    f = random_state(rng, (c.nz, c.ny, c.nx), amp=2e-3, umax=0.03)
+    big = np.full(7, 3e38, np.float32)
+    far = np.full(7, 1e9, np.float32)
+    for lo, hi, sign in ((-big, big, 1.0), (far, 2 * far, -1.0)):
+        m = L.Mlp(seed=1)
+        m.set_norm(np.zeros(7), np.ones(7), lo, hi, 0.01)
+        with L.Solver(c) as s:
+            s.set_populations(f)
+            s.set_closure(m)
+            s.step(1)
+            r = s.closure_rates()
+        assert r.shape == (c.nz, c.ny, c.nx)
+        w = sign * r
+        assert (w > 0).all() and (w <= np.float32(c.omega) * (1 + 1e-6)).all(), (sign, w.min(), w.max())
+
+
 @pytest.mark.parametrize("storage, bar", [("f16", 1e-3), ("bf16", 1e-2)])
 def test_closure_on_16bit_storage_tracks_fp32(storage, bar):
     """The fused closure collide with 16-bit population storage against the
